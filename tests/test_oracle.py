@@ -1,0 +1,344 @@
+"""Pins for the CPU oracle (oracle/).
+
+Each check below is independent of the oracle's own formulas: the reference's
+printed known-answer pairs (test/test_mps.py, test/test_mpo.py,
+test/test_syn.py) turned into asserts, dense closed forms, Eckart-Young,
+Schmidt values of the dense tensor and the Pythagoras identity of sequential
+orthogonal projections.
+"""
+import numpy as np
+import pytest
+
+from oracle import reference as R
+from oracle import rounding as RD
+
+
+@pytest.fixture(autouse=True)
+def _compress_default():
+    R.MPO.COMPRESS = True
+    yield
+    R.MPO.COMPRESS = True
+
+
+# --------------------------------------------------------------------------
+# dense closed forms used as independent checks
+# --------------------------------------------------------------------------
+def proj(A):
+    """Orthogonal projector onto range(A) (pseudo-inverse form, no QR)."""
+    return A @ np.linalg.pinv(A)
+
+
+def dense_decompose_left(T, bonds):
+    """Left QR-truncation TT of a dense tensor: at step k project the
+    (d_0..d_k, rest) unfolding onto the span of its first b_k columns."""
+    shape = T.shape
+    P = T.reshape(shape[0], -1)
+    for k, b in enumerate(bonds):
+        rows = int(np.prod(shape[: k + 1]))
+        M = P.reshape(rows, -1)
+        M = proj(M[:, :b]) @ M
+        P = M
+    return P.reshape(shape)
+
+
+def dense_qr_truncate(sites, dim):
+    """Reference >> (strict, left) from the ORIGINAL cores: the left block of
+    bond k is projected onto the span of its first ``dim`` columns, then the
+    untouched core k+1 is attached."""
+    A = sites[0].reshape(-1, sites[0].shape[-1])
+    for k in range(len(sites) - 1):
+        A = proj(A[:, :dim]) @ A
+        nxt = sites[k + 1]
+        A = (A @ nxt.reshape(nxt.shape[0], -1)).reshape(-1, nxt.shape[-1])
+    return A
+
+
+def dense_sequential_truncation(T, kept):
+    """TT-rounding in dense form: right to left, truncate the Schmidt
+    decomposition at bond k to kept[k] terms (np.linalg.svd)."""
+    shape = T.shape
+    n = len(shape)
+    for k in range(n - 1, 0, -1):
+        M = T.reshape(int(np.prod(shape[:k])), -1)
+        U, S, Vt = np.linalg.svd(M, full_matrices=False)
+        c = kept[k]
+        T = ((U[:, :c] * S[:c]) @ Vt[:c]).reshape(shape)
+    return T
+
+
+def random_sites(rng, dims, bonds):
+    b = [1] + list(bonds) + [1]
+    return [rng.standard_normal((b[k], d, b[k + 1])) for k, d in enumerate(dims)]
+
+
+# --------------------------------------------------------------------------
+# known-answer pairs from the reference's test scripts
+# --------------------------------------------------------------------------
+def test_mps_add_arange():
+    # test/test_mps.py:51-70
+    x = np.arange(8).reshape((2, 2, 2))
+    X = R.MPS(x, bond_shape=(2, 2)).decompose()
+    Y = R.MPS(x.copy(), bond_shape=(2, 2)).decompose()
+    Z = X + Y
+    np.testing.assert_allclose(Z[1, 1, 1], 14.0, rtol=1e-12)
+    np.testing.assert_allclose(Z.to_tensor(), 2 * x, atol=1e-12)
+    np.testing.assert_allclose(R.retrieve_all(Z), 2 * x, atol=1e-12)
+
+
+def test_mps_dot_and_normalize():
+    # test/test_mps.py:100-126
+    x = np.arange(4 ** 3).reshape((4, 4, 4)).astype(float)
+    X = R.MPS(x, bond_shape=(4, 4)).decompose()
+    np.testing.assert_allclose(X | X, np.sum(x * x), rtol=1e-12)
+    np.testing.assert_allclose(X.dot(), np.linalg.norm(x), rtol=1e-12)
+    X.normalize()
+    np.testing.assert_allclose(X.dot(), 1.0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["left", "right"])
+def test_mps_decompose_arange_exact(mode):
+    # test/test_mps.py:129-157 (rank-2 unfoldings, bonds 2 -> exact)
+    x = np.arange(4 ** 3).reshape((4, 4, 4)).astype(float)
+    X = R.MPS(x, bond_shape=(2, 2)).decompose(mode=mode)
+    np.testing.assert_allclose(X.to_tensor(), x, atol=1e-9)
+
+
+def test_mpo_decompose_elements():
+    # test/test_mpo.py:44-64
+    w = np.arange(2 ** 4).reshape((2, 2, 2, 2))
+    W = R.MPO(w, bond_shape=(4,)).decompose()
+    np.testing.assert_allclose(W[(1, 1), (1, 1)], w[1, 1, 1, 1], rtol=1e-12)
+    z = np.arange(4 ** 3 * 3 ** 3).reshape((4, 4, 4, 3, 3, 3))
+    Z = R.MPO(z, bond_shape=(3, 3)).decompose()
+    np.testing.assert_allclose(Z[(3, 1, 0), (1, 2, 1)], z[3, 1, 0, 1, 2, 1], rtol=1e-10)
+    np.testing.assert_allclose(Z.to_tensor(), z, atol=1e-8)
+
+
+def test_mpo_orthogonality():
+    # test/test_mpo.py:66-112
+    w = np.arange(4 ** 6).reshape((4,) * 6)
+    W = R.MPO(w, bond_shape=(2, 2)).decompose()
+    W.left_orthonormalization()
+    np.testing.assert_allclose(np.diag(W.left_orthogonality(0)), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.diag(W.left_orthogonality(1)), 1.0, atol=1e-12)
+    W.right_orthonormalization()
+    np.testing.assert_allclose(np.diag(W.right_orthogonality(1)), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.diag(W.right_orthogonality(2)), 1.0, atol=1e-12)
+    np.testing.assert_allclose(W[(1, 0, 0), (1, 0, 0)], w[1, 0, 0, 1, 0, 0], rtol=1e-9)
+
+
+def test_mpo_dot():
+    # test/test_mpo.py:160-175
+    x = np.arange(2 ** 2).reshape((2, 2))
+    X = R.MPS(x, bond_shape=(2,)).decompose()
+    np.testing.assert_allclose(X | X, 14.0, rtol=1e-12)
+
+
+def test_mpo_matmul_uncompressed():
+    # test/test_mpo.py:177-196 with the implicit truncation disabled
+    w = np.arange(2 ** 4).reshape((2, 2, 2, 2))
+    x = np.arange(2 ** 2).reshape((2, 2))
+    W = R.MPO(w, bond_shape=(2,)).decompose()
+    X = R.MPS(x, bond_shape=(2,)).decompose()
+    R.MPO.COMPRESS = False
+    Z = (W @ W) @ X
+    np.testing.assert_allclose(Z.to_tensor().reshape(4), [1680, 1940, 2200, 2460], rtol=1e-12)
+
+
+def test_mpo_add_chain():
+    # test/test_mpo.py:256-282
+    x = np.arange(1, 17).reshape((2, 2, 2, 2))
+    y = np.arange(18, 34).reshape((2, 2, 2, 2))
+    X = R.MPO(x, bond_shape=(3,)).decompose()
+    Y = R.MPO(y, bond_shape=(3,)).decompose()
+    Z = X + Y + Y + Y + Y + Y + Y + Y + Y + Y
+    z = x + 9 * y
+    np.testing.assert_allclose(Z[(1, 1), (1, 1)], z[1, 1, 1, 1], rtol=1e-10)
+    np.testing.assert_allclose(Z.to_tensor(), z, atol=1e-8)
+
+
+def test_mpo_diag_right_orthonormalization():
+    # test/test_mpo.py:296-327
+    y = np.zeros((4, 4))
+    np.fill_diagonal(y, [1, 2, 3, 4])
+    y = y.reshape((2, 2, 2, 2))
+    Y = R.MPO(y, bond_shape=(4,)).decompose().right_orthonormalization()
+    assert np.max(np.abs(y - Y.to_tensor())) < 1e-12
+
+
+def test_syn_mul():
+    # test/test_syn.py:10-27
+    x = np.arange(2 ** 2)
+    X = R.MPS(x.reshape((2, 2)), bond_shape=(2,)).decompose()
+    w = np.arange(2 ** 4).reshape((4, 4))
+    W = R.MPO(w.reshape((2, 2, 2, 2)), bond_shape=(2,)).decompose()
+    np.testing.assert_allclose(R.mul(W, X).to_tensor().reshape(4), x @ w, rtol=1e-12)
+    np.testing.assert_allclose(R.mul(X, W).to_tensor().reshape(4), x @ w, rtol=1e-12)
+    np.testing.assert_allclose(R.mul(X, X), x @ x, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# dense closed forms on random inputs
+# --------------------------------------------------------------------------
+def test_decompose_projector_closed_form():
+    rng = np.random.default_rng(1)
+    T = rng.standard_normal((3, 4, 5, 3))
+    bonds = (2, 5, 2)
+    X = R.MPS(T, bond_shape=bonds).decompose()
+    np.testing.assert_allclose(X.to_tensor(), dense_decompose_left(T, bonds), atol=1e-12)
+    # an MPO decomposes its interleaved (i0,o0,i1,o1,..) tensor the same way
+    t = rng.standard_normal((2, 3, 2, 2, 2, 3))
+    W = R.MPO(t, bond_shape=(3, 2)).decompose()
+    ti = np.transpose(t, (0, 3, 1, 4, 2, 5)).reshape(4, 6, 6)
+    ref = dense_decompose_left(ti, (3, 2)).reshape(2, 2, 3, 2, 2, 3).transpose(0, 2, 4, 1, 3, 5)
+    np.testing.assert_allclose(W.to_tensor(), ref, atol=1e-12)
+
+
+def test_qr_truncation_projector_closed_form():
+    rng = np.random.default_rng(2)
+    sites = random_sites(rng, (3, 4, 3, 2), (6, 7, 5))
+    X = R.MPS.from_sites(sites)
+    Z = X >> 3
+    np.testing.assert_allclose(Z.to_tensor().reshape(-1), dense_qr_truncate(sites, 3).reshape(-1), atol=1e-12)
+    # stale bond_shape of the returned copy, updated shape list
+    assert Z.bond_shape == (6, 7, 5)
+    assert [s[2] for s in Z.shape[:-1]] == [3, 3, 3]
+    # dim >= min(bond_shape) returns self unchanged
+    assert (X >> 5) is X
+
+
+def test_mpo_rshift_mutates_receiver():
+    rng = np.random.default_rng(3)
+    sites = [rng.standard_normal(s) for s in [(1, 2, 2, 5), (5, 2, 2, 4), (4, 2, 2, 1)]]
+    W = R.MPO.from_sites(sites)
+    Z = W >> 2
+    assert W.bond_shape == (2, 2)          # overwritten on the receiver
+    assert Z.bond_shape == (5, 4)          # stale on the result
+    assert Z.sites is W.sites              # aliasing copy
+    assert Z.sites[0].shape == (1, 2, 2, 2)
+
+
+def test_rshift_too_small_unfolding_raises():
+    # (lrank*d) < dim cannot be reshaped back, the reference raises ValueError
+    rng = np.random.default_rng(4)
+    sites = random_sites(rng, (2, 2, 2), (16, 16))
+    with pytest.raises(ValueError):
+        R.MPS.from_sites(sites) >> 4
+
+
+def test_apply_dense_identity():
+    rng = np.random.default_rng(5)
+    xs = random_sites(rng, (3, 2, 4), (3, 2))
+    os_ = [rng.standard_normal(s) for s in [(1, 3, 2, 2), (2, 2, 3, 3), (3, 4, 2, 1)]]
+    X = R.MPS.from_sites(xs)
+    O = R.MPO.from_sites(os_)
+    R.MPO.COMPRESS = False
+    Y = O @ X
+    x = X.to_tensor()
+    o = O.to_tensor()  # (i0,i1,i2,o0,o1,o2)
+    y = np.einsum("abc,abcdef->def", x, o)
+    np.testing.assert_allclose(Y.to_tensor(), y, atol=1e-12)
+    # MPO @ MPO is the matrix product in the (in, out) view
+    os2 = [rng.standard_normal(s) for s in [(1, 3, 3, 2), (2, 2, 2, 2), (2, 4, 4, 1)]]
+    O2 = R.MPO.from_sites(os2)
+    os3 =[rng.standard_normal(s) for s in [(1, 3, 3, 2), (2, 2, 2, 2), (2, 4, 4, 1)]]
+    O3 = R.MPO.from_sites(os3)
+    P = O2 @ O3
+    a = O2.to_tensor().reshape(24, 24)
+    b = O3.to_tensor().reshape(24, 24)
+    np.testing.assert_allclose(P.to_tensor().reshape(24, 24), a @ b, atol=1e-12)
+    assert R.mul(O3, O2).to_tensor().shape == (3, 2, 4, 3, 2, 4)
+    np.testing.assert_allclose(R.mul(O3, O2).to_tensor().reshape(24, 24), a @ b, atol=1e-12)
+
+
+def test_add_dense():
+    rng = np.random.default_rng(6)
+    A = R.MPS.from_sites(random_sites(rng, (2, 3, 2), (2, 3)))
+    B = R.MPS.from_sites(random_sites(rng, (2, 3, 2), (4, 2)))
+    np.testing.assert_allclose((A + B).to_tensor(), A.to_tensor() + B.to_tensor(), atol=1e-12)
+    assert (A + B).bond_shape == (6, 5)
+    R.MPO.COMPRESS = False
+    oa = R.MPO.from_sites([rng.standard_normal(s) for s in [(1, 2, 3, 2), (2, 2, 2, 1)]])
+    ob = R.MPO.from_sites([rng.standard_normal(s) for s in [(1, 2, 3, 3), (3, 2, 2, 1)]])
+    np.testing.assert_allclose((oa + ob).to_tensor(), oa.to_tensor() + ob.to_tensor(), atol=1e-12)
+
+
+def test_overlap_dense_and_elementwise():
+    rng = np.random.default_rng(7)
+    A = R.MPS.from_sites(random_sites(rng, (3, 2, 4, 2), (3, 5, 2)))
+    B = R.MPS.from_sites(random_sites(rng, (3, 2, 4, 2), (2, 4, 3)))
+    a, b = A.to_tensor(), B.to_tensor()
+    np.testing.assert_allclose(R.retrieve_all(A), a, atol=1e-12)
+    np.testing.assert_allclose(A | B, np.sum(a * b), rtol=1e-12)
+    np.testing.assert_allclose(A.left_orthogonality(1), A._left(A.sites[1], 1).T @ A._left(A.sites[1], 1))
+
+
+def test_left_orthonormalization_keeps_tensor():
+    rng = np.random.default_rng(8)
+    A = R.MPS.from_sites(random_sites(rng, (3, 4, 4, 3), (3, 5, 3)))
+    t = A.to_tensor()
+    A.left_orthonormalization()
+    np.testing.assert_allclose(A.to_tensor(), t, atol=1e-12)
+    for k in range(3):
+        np.testing.assert_allclose(A.left_orthogonality(k), np.eye(A.shape[k][2]), atol=1e-12)
+    A.right_orthonormalization()
+    np.testing.assert_allclose(A.to_tensor(), t, atol=1e-12)
+    for k in range(1, 4):
+        np.testing.assert_allclose(A.right_orthogonality(k), np.eye(A.shape[k][0]), atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# TT-rounding (SVD form)
+# --------------------------------------------------------------------------
+def test_keep_count_rule():
+    s = np.array([4.0, 2.0, 1.0, 0.5])
+    w = s ** 2
+    assert RD.keep_count(s) == 4
+    assert RD.keep_count(s, chi_max=2) == 2
+    # tail after 3 kept = 0.25 ; relative = 0.25/21.25
+    assert RD.keep_count(s, cutoff=0.25 / w.sum()) == 3
+    assert RD.keep_count(s, cutoff=0.25 / w.sum() * 0.999) == 4
+    assert RD.keep_count(s, cutoff=1.0) == 1
+    assert RD.keep_count(np.zeros(3), cutoff=1e-3) == 1
+
+
+def test_round_eckart_young_two_sites():
+    rng = np.random.default_rng(9)
+    sites = random_sites(rng, (6, 5), (4,))
+    cores, sv, kept, disc = RD.tt_round(sites, chi_max=2)
+    M = R.dense_mps(sites)
+    U, S, Vt = np.linalg.svd(M)
+    best = (U[:, :2] * S[:2]) @ Vt[:2]
+    np.testing.assert_allclose(R.dense_mps(cores), best, atol=1e-12)
+    np.testing.assert_allclose(sv[1], S[:4], rtol=1e-12)
+    np.testing.assert_allclose(disc[1], np.sum(S[2:] ** 2), rtol=1e-12)
+
+
+def test_round_dense_sequential_and_pythagoras():
+    rng = np.random.default_rng(10)
+    sites = random_sites(rng, (2, 3, 2, 3, 2), (2, 6, 6, 3))
+    T = R.dense_mps(sites)
+    cores, sv, kept, disc = RD.tt_round(sites, chi_max=3, cutoff=1e-3)
+    np.testing.assert_allclose(R.dense_mps(cores), dense_sequential_truncation(T, kept), atol=1e-11)
+    # singular values at the first truncated bond are the Schmidt values of T
+    S = np.linalg.svd(T.reshape(-1, 2), compute_uv=False)
+    np.testing.assert_allclose(sv[4], S, rtol=1e-12)
+    n0 = np.sum(T * T)
+    n1 = RD.norm2(cores)
+    np.testing.assert_allclose(n0 - n1, sum(disc[1:]), rtol=1e-9, atol=1e-12 * n0)
+    # the result is right-orthonormal from core 1 on
+    for k in range(1, 5):
+        c = cores[k].reshape(cores[k].shape[0], -1)
+        np.testing.assert_allclose(c @ c.T, np.eye(c.shape[0]), atol=1e-12)
+
+
+def test_apply_and_round_exact_when_untruncated():
+    rng = np.random.default_rng(11)
+    xs = random_sites(rng, (2, 2, 2, 2), (2, 3, 2))
+    os_ = [rng.standard_normal(s) for s in [(1, 2, 2, 2), (2, 2, 2, 2), (2, 2, 2, 2), (2, 2, 2, 1)]]
+    cores, sv, kept, disc = RD.apply_and_round(os_, xs)
+    x = R.dense_mps(xs)
+    o = R.dense_mpo(os_)
+    y = np.einsum("abcd,abcdefgh->efgh", x, o)
+    np.testing.assert_allclose(R.dense_mps(cores), y, atol=1e-11)
+    assert all(d < 1e-20 for d in disc[1:])
