@@ -1,0 +1,217 @@
+/*
+ * syngular_amd.h -- C ABI of the MI355X (gfx950) matrix-product core.
+ *
+ * The reference (antoine311200/Syngular) is pure Python on NumPy/opt_einsum;
+ * every function below replaces one NumPy/LAPACK/opt_einsum call site on the
+ * path of syngular.tensor.MatrixProductState / MatrixProductOperator.  The
+ * reference file:line each entry point replaces is given beside it.
+ *
+ * Conventions
+ *   - All matrices are dense, row-major, addressed by a base pointer and a row
+ *     stride ("ld", in elements).  A core (l, d, r) is the row-major array
+ *     l*d*r; its left unfolding is the (l*d) x r matrix with ld = r, its right
+ *     unfolding the l x (d*r) matrix with ld = d*r.
+ *   - Every batched entry point takes a DEVICE pointer to an array of item
+ *     descriptors (the caller owns that memory), the number of items and the
+ *     total number of work-groups the items need (the caller fills each item's
+ *     `block_start` with the running sum, in item order).  One launch serves one
+ *     problem or thousands of independent problems of uneven shape.
+ *   - Pointers inside items are device pointers.  Nothing here allocates device
+ *     memory except where an entry point says so; callers pass workspaces.
+ *   - Every function returns 0 on success or a negative SYN_E* code; the
+ *     message of the last failure on the calling thread is syn_last_error().
+ *   - `stream` is a hipStream_t passed as void* (0 = the null stream).
+ */
+#ifndef SYNGULAR_AMD_H
+#define SYNGULAR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SYN_OK 0
+#define SYN_EINVAL (-1)   /* bad argument / shape (the reference raises ValueError/Exception) */
+#define SYN_EHIP (-2)     /* HIP runtime error */
+#define SYN_ENOTSUP (-3)  /* unsupported dtype / configuration */
+
+/* ------------------------------------------------------------------ */
+/* library                                                              */
+/* ------------------------------------------------------------------ */
+const char *syn_last_error(void);
+int syn_version(void);
+/* number of gfx950 devices visible; fails with SYN_EHIP when none */
+int syn_device_count(int *count);
+/* ABI self-check: writes sizeof() of the item structs below, in declaration
+ * order (gemm, apply, mpo_mpo, dsum, qr, jacobi, trunc, rowscale), into
+ * out[0..n-1]; returns the number of structs. */
+int syn_struct_sizes(int64_t *out, int n);
+
+/* ------------------------------------------------------------------ */
+/* GEMM  C = alpha * op(A) * op(B) + beta * C  (grouped, MFMA)           */
+/* Replaces the absorb products  S @ R, B @ R, L @ U.T                   */
+/*   tensor/matrix_product_state.py:383,415,448,561,575                  */
+/*   tensor/matrix_product_operator.py:493,526,560,682,705               */
+/* and the Gram products L.T @ L, R @ R.T (matrix_product_state.py:626,630;*/
+/* matrix_product_operator.py:773,777).                                  */
+/* op(A) is m x k: A[i*lda+l] (transA=0) or A[l*lda+i] (transA=1).       */
+/* op(B) is k x n: B[l*ldb+j] (transB=0) or B[j*ldb+l] (transB=1).       */
+/* block_start counts tiles of the tile configuration `tile` (0: 64x64,  */
+/* 1: 128x128); tiles_n = ceil(n / tile width).                          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const void *A;
+    const void *B;
+    void *C;
+    int64_t m, n, k;
+    int64_t lda, ldb, ldc;
+    double alpha, beta;
+    int32_t transA, transB;
+    int64_t block_start;
+    int64_t tiles_n;
+} syn_gemm_item;
+
+/* f64 x f64 -> f64 on v_mfma_f64_16x16x4 */
+int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* MPO x MPS site contraction, materialised                              */
+/*   P[(a*Dl+c), o, (b*Dr+e)] = sum_s X[a,s,b] * O[c,s,o,e]               */
+/* Replaces tensor/matrix_product_operator.py:184-190 and                */
+/* tensor/utils.py:35-41,49-55 (contract + reshape).                    */
+/* One work-group covers SYN_APPLY_BLOCK output elements.                */
+/* ------------------------------------------------------------------ */
+#define SYN_APPLY_BLOCK 1024
+typedef struct {
+    const double *x;  /* (chil, din, chir) */
+    const double *o;  /* (dl, din, dout, dr) */
+    double *p;        /* (chil*dl, dout, chir*dr) */
+    int64_t chil, din, chir;
+    int64_t dl, dout, dr;
+    int64_t block_start;
+} syn_apply_item;
+int syn_mpo_apply_f64(const syn_apply_item *items, int n_items, int64_t total_blocks, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* MPO x MPO site contraction                                            */
+/*   P[(a*Cl+c), i, p, (b*Cr+e)] = sum_j A[a,i,j,b] * B[c,j,p,e]          */
+/* Replaces tensor/matrix_product_operator.py:280-287 and                */
+/* tensor/utils.py:62-69.                                                */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *a;  /* (al, din, dmid, ar) */
+    const double *b;  /* (bl, dmid, dout, br) */
+    double *p;        /* (al*bl, din, dout, ar*br) */
+    int64_t al, din, dmid, ar;
+    int64_t bl, dout, br;
+    int64_t block_start;
+} syn_mpo_mpo_item;
+int syn_mpo_mpo_f64(const syn_mpo_mpo_item *items, int n_items, int64_t total_blocks, void *stream);
+
+/* Bond-leg Kronecker product for every physical index pair (MPO Hadamard  */
+/* product): P[(a*bl+c), i, p, (b*br+e)] = A[a,i,p,b] * B[c,i,p,e], with   */
+/* A (al, din, dout, ar), B (bl, din, dout, br); the item's dmid is unused.*/
+/* Replaces the np.kron loop of tensor/matrix_product_operator.py:136-154.*/
+int syn_kron_bonds_f64(const syn_mpo_mpo_item *items, int n_items, int64_t total_blocks, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Direct-sum core packing (one output core per item)                   */
+/*   edge==1 (first site):  out[:, m, :] = [A | B]                       */
+/*   edge==2 (last site):   out[:, m, :] = [A ; B]                       */
+/*   edge==0 (interior):    out[:, m, :] = blockdiag(A, B)               */
+/* `mid` is d for an MPS core and d_in*d_out for an MPO core.            */
+/* Replaces tensor/matrix_product_state.py:82-96 and                    */
+/* tensor/matrix_product_operator.py:90-106.                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *a;  /* (la, mid, ra) */
+    const double *b;  /* (lb, mid, rb) */
+    double *out;      /* (lo, mid, ro) */
+    int64_t la, ra, lb, rb, mid, lo, ro;
+    int32_t edge;
+    int64_t block_start;
+} syn_dsum_item;
+int syn_direct_sum_f64(const syn_dsum_item *items, int n_items, int64_t total_blocks, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Householder QR, one work-group per item.                              */
+/* Factors the leading nb = min(n, keep) columns of the m x n matrix A   */
+/* (element (i,j) at A[i*rs + j*cs]) and writes the first kq = min(keep, m)*/
+/* columns of the complete orthogonal factor Q (m x kq, row-major, ldq). */
+/* The caller forms R = Q^T A with syn_gemm_f64.  Reflectors follow      */
+/* Golub & Van Loan 5.1.2 with beta = -sign(alpha) * ||x||, the LAPACK    */
+/* convention, so Q agrees with numpy.linalg.qr up to rounding.          */
+/* work: m*nb doubles (m*(nb+kq) when q_trans); tau: nb doubles.        */
+/* Replaces numpy.linalg.qr at tensor/matrix_product_state.py:307,332,  */
+/* 443,560,574 and tensor/matrix_product_operator.py:436,555,680,703.     */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *a;
+    int64_t rs, cs;
+    int64_t m, n, keep;
+    double *q;
+    int64_t ldq;
+    int32_t q_trans;  /* 1: store Q^T (kq x m, row-major, ldq) instead of Q */
+    double *work;
+    double *tau;
+} syn_qr_item;
+int syn_qr_f64(const syn_qr_item *items, int n_items, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* One-sided Jacobi SVD (Hestenes), one work-group per item.             */
+/* x holds nv vectors of length len as rows (x[i*ldx + t]).  The rows are */
+/* rotated pairwise until mutually orthogonal; the same rotations are    */
+/* applied to rt (nv x nv, initialised to I by the kernel).  On exit the */
+/* rows are sorted by decreasing norm, s[i] = ||x_i||, and               */
+/*     x_in = rt^T * x_out   with   x_out rows orthogonal.               */
+/* With gram = 1 the input is a symmetric Gram matrix G = C^T C and      */
+/* s[i] = sqrt(||x_i||) are the singular values of C (rt = V^T).         */
+/* No reference call site (north-star addition: SVD rounding).           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double *x;
+    int64_t ldx;
+    int64_t nv, len;
+    double *rt;
+    int64_t ldr;
+    double *s;
+    int32_t *sweeps;  /* out: sweeps used (may be null) */
+    int32_t gram;
+} syn_jacobi_item;
+int syn_jacobi_f64(const syn_jacobi_item *items, int n_items, double tol, int max_sweeps, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Device-side truncation: for sorted singular values s[0..n-1] keep the */
+/* smallest c with sum_{i>=c} s_i^2 <= cutoff * sum_i s_i^2 (cutoff>0),  */
+/* then c = min(c, chi_max) (chi_max>0), c >= 1.  Writes c and the       */
+/* discarded weight sum_{i>=c} s_i^2 (accumulated in FP64).              */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *s;
+    int64_t n;
+    int64_t chi_max;
+    double cutoff;
+    int32_t *keep;
+    double *discarded;
+} syn_trunc_item;
+int syn_truncate_f64(const syn_trunc_item *items, int n_items, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Row scaling  X[i, :] *= s[i]  (mode 0)  or  X[i, :] /= s[i]  (mode 1; */
+/* rows with s[i] == 0 are zeroed).  Used to turn the Jacobi factors into */
+/* U*S and S^-1*U^T*C in the rounding sweep.                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double *x;
+    int64_t rows, cols, ld;
+    const double *s;
+    int32_t mode;
+    int64_t block_start;
+} syn_rowscale_item;
+int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_blocks, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SYNGULAR_AMD_H */

@@ -425,6 +425,21 @@ class MPO:
         out = MPO.from_sites(sites)
         return out >> min_bond if MPO.COMPRESS else out
 
+    def __mul__(self, mp):
+        # tensor/matrix_product_operator.py:126-161: per (in, out) pair the
+        # bond matrices are Kronecker-multiplied, then >> min_bond
+        min_bond = min(min(self.bond_shape), min(mp.bond_shape))
+        sites = []
+        for k in range(self.sites_number):
+            A, B = self.sites[k], mp.sites[k]
+            site = np.zeros((A.shape[0] * B.shape[0], A.shape[1], A.shape[2], A.shape[3] * B.shape[3]))
+            for i in range(A.shape[1]):
+                for o in range(A.shape[2]):
+                    site[:, i, o, :] = np.kron(A[:, i, o, :], B[:, i, o, :])
+            sites.append(site)
+        out = MPO.from_sites(sites)
+        return out >> min_bond if MPO.COMPRESS else out
+
     def __matmul__(self, mp):
         # tensor/matrix_product_operator.py:175-192 (MPS) and :278-289 (MPO)
         min_bond = min(min(self.bond_shape), min(mp.bond_shape))

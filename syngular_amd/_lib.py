@@ -1,0 +1,119 @@
+"""ctypes binding of libsyngular_amd.so (the gfx950 C ABI in include/syngular_amd.h).
+
+The shared object is built in-tree by ``__graft_entry__.build()`` (or ``make``
+in syngular_amd/csrc).  There is no fallback: if the library is missing or no
+GPU is visible, every device operation raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsyngular_amd.so")
+
+SYN_OK = 0
+SYN_EINVAL = -1
+SYN_EHIP = -2
+SYN_ENOTSUP = -3
+
+APPLY_BLOCK = 1024  # SYN_APPLY_BLOCK
+
+_P = np.uint64  # device pointer
+_I = np.int64
+
+# numpy mirrors of the item structs (align=True reproduces the C layout)
+GEMM_ITEM = np.dtype(
+    [("A", _P), ("B", _P), ("C", _P), ("m", _I), ("n", _I), ("k", _I), ("lda", _I), ("ldb", _I), ("ldc", _I),
+     ("alpha", np.float64), ("beta", np.float64), ("transA", np.int32), ("transB", np.int32),
+     ("block_start", _I), ("tiles_n", _I)], align=True)
+APPLY_ITEM = np.dtype(
+    [("x", _P), ("o", _P), ("p", _P), ("chil", _I), ("din", _I), ("chir", _I), ("dl", _I), ("dout", _I),
+     ("dr", _I), ("block_start", _I)], align=True)
+MPO_MPO_ITEM = np.dtype(
+    [("a", _P), ("b", _P), ("p", _P), ("al", _I), ("din", _I), ("dmid", _I), ("ar", _I), ("bl", _I),
+     ("dout", _I), ("br", _I), ("block_start", _I)], align=True)
+DSUM_ITEM = np.dtype(
+    [("a", _P), ("b", _P), ("out", _P), ("la", _I), ("ra", _I), ("lb", _I), ("rb", _I), ("mid", _I),
+     ("lo", _I), ("ro", _I), ("edge", np.int32), ("block_start", _I)], align=True)
+QR_ITEM = np.dtype(
+    [("a", _P), ("rs", _I), ("cs", _I), ("m", _I), ("n", _I), ("keep", _I), ("q", _P), ("ldq", _I),
+     ("q_trans", np.int32), ("work", _P), ("tau", _P)], align=True)
+JACOBI_ITEM = np.dtype(
+    [("x", _P), ("ldx", _I), ("nv", _I), ("len", _I), ("rt", _P), ("ldr", _I), ("s", _P), ("sweeps", _P),
+     ("gram", np.int32)], align=True)
+TRUNC_ITEM = np.dtype(
+    [("s", _P), ("n", _I), ("chi_max", _I), ("cutoff", np.float64), ("keep", _P), ("discarded", _P)],
+    align=True)
+ROWSCALE_ITEM = np.dtype(
+    [("x", _P), ("rows", _I), ("cols", _I), ("ld", _I), ("s", _P), ("mode", np.int32), ("block_start", _I)],
+    align=True)
+
+ITEM_DTYPES = [GEMM_ITEM, APPLY_ITEM, MPO_MPO_ITEM, DSUM_ITEM, QR_ITEM, JACOBI_ITEM, TRUNC_ITEM, ROWSCALE_ITEM]
+
+# every symbol the header declares, with its ctypes signature
+_VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+SIGNATURES = {
+    "syn_last_error": (ctypes.c_char_p, []),
+    "syn_version": (_I32, []),
+    "syn_device_count": (_I32, [ctypes.POINTER(ctypes.c_int)]),
+    "syn_struct_sizes": (_I32, [ctypes.POINTER(ctypes.c_int64), _I32]),
+    "syn_gemm_f64": (_I32, [_VP, _I32, _I64, _I32, _VP]),
+    "syn_mpo_apply_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_mpo_mpo_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_kron_bonds_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_direct_sum_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_qr_f64": (_I32, [_VP, _I32, _VP]),
+    "syn_jacobi_f64": (_I32, [_VP, _I32, _D, _I32, _VP]),
+    "syn_truncate_f64": (_I32, [_VP, _I32, _VP]),
+    "syn_rowscale_f64": (_I32, [_VP, _I32, _I64, _VP]),
+}
+
+
+class SyngularError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+
+_lib = None
+
+
+def load():
+    """Load the shared object (no GPU needed) and bind every declared symbol."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SyngularError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()' or make -C syngular_amd/csrc)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check_abi():
+    """Compare the C struct sizes with the numpy mirrors; raises on mismatch."""
+    lib = load()
+    n = lib.syn_struct_sizes(None, 0)
+    buf = (ctypes.c_int64 * n)()
+    lib.syn_struct_sizes(buf, n)
+    sizes = list(buf)
+    mine = [dt.itemsize for dt in ITEM_DTYPES]
+    if sizes != mine:
+        raise SyngularError(f"ABI mismatch: C sizes {sizes} vs Python {mine}")
+    return sizes
+
+
+def call(name, *args):
+    rc = getattr(load(), name)(*args)
+    if rc != SYN_OK:
+        msg = load().syn_last_error().decode(errors="replace")
+        if rc == SYN_EINVAL:
+            raise ValueError(f"{name}: {msg}")
+        raise SyngularError(f"{name} failed ({rc}): {msg}")
+    return rc

@@ -1,0 +1,158 @@
+// Grouped FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// C = alpha * op(A) * op(B) + beta * C for a table of independent problems of
+// any shape.  A work-group of 4 waves owns one BM x BN output tile; each wave
+// owns a WM x WN sub-tile held as (WM/16) x (WN/16) MFMA accumulators.  Operand
+// tiles are staged through LDS stored k-major ([k][i] and [k][j]) so that the
+// 16 lanes that feed one MFMA row read 16 consecutive doubles; rows are padded
+// by one double to spread the transposed stores over the banks.  The K loop is
+// double-buffered: the next global tile is loaded into registers while the
+// current LDS tile feeds the MFMAs.
+//
+// MFMA f64 16x16x4 operand layout (lane l of 64):
+//   A: A[i = l%16][k = l/16]     B: B[k = l/16][j = l%16]
+//   D: reg r of lane l holds D[i = l/16 + 4*r][j = l%16]  (checked against
+//      NumPy on the device: rows 4g+r and g+4r differ unless g == r)
+#include "common.hpp"
+
+namespace syn {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
+    const syn_gemm_item *__restrict__ items, int n_items, int64_t total) {
+    constexpr int NW = (BM / WM) * (BN / WN);
+    constexpr int NT = NW * 64;
+    constexpr int TI = WM / 16, TJ = WN / 16;
+    constexpr int EA = BM * BK / NT, EB = BK * BN / NT;
+    static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "tile/threads mismatch");
+    __shared__ double As[2][BK][BM + 1];
+    __shared__ double Bs[2][BK][BN + 1];
+
+    const int64_t bid = xcd_remap(blockIdx.x, total);
+    const syn_gemm_item &P = items[find_item(items, n_items, bid)];
+    const int64_t t = bid - P.block_start;
+    const int64_t row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
+    const int64_t m = P.m, n = P.n, k = P.k, lda = P.lda, ldb = P.ldb;
+    const int ta = P.transA, tb = P.transB;
+    const double *__restrict__ A = (const double *)P.A;
+    const double *__restrict__ B = (const double *)P.B;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm0 = (w / (BN / WN)) * WM, wn0 = (w % (BN / WN)) * WN;
+
+    double ra[EA], rb[EB];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int s = 0; s < EA; ++s) {
+            int e = tid + NT * s;
+            int r, c;
+            if (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
+            int64_t gi = row0 + r, gk = k0 + c;
+            ra[s] = (gi < m && gk < k) ? (ta ? A[gk * lda + gi] : A[gi * lda + gk]) : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < EB; ++s) {
+            int e = tid + NT * s;
+            int j, c;
+            if (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
+            int64_t gj = col0 + j, gk = k0 + c;
+            rb[s] = (gj < n && gk < k) ? (tb ? B[gj * ldb + gk] : B[gk * ldb + gj]) : 0.0;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < EA; ++s) {
+            int e = tid + NT * s;
+            int r, c;
+            if (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
+            As[buf][c][r] = ra[s];
+        }
+#pragma unroll
+        for (int s = 0; s < EB; ++s) {
+            int e = tid + NT * s;
+            int j, c;
+            if (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
+            Bs[buf][c][j] = rb[s];
+        }
+    };
+
+    v4d acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    const int64_t nk = (k + BK - 1) / BK;
+    load(0);
+    store(0);
+    __syncthreads();
+    const int li = lane & 15, lk = lane >> 4;
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            double a[TI], b[TJ];
+#pragma unroll
+            for (int i = 0; i < TI; ++i) a[i] = As[cur][kk * 4 + lk][wm0 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) b[j] = Bs[cur][kk * 4 + lk][wn0 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) store(cur ^ 1);
+        __syncthreads();
+    }
+
+    double *__restrict__ C = (double *)P.C;
+    const int64_t ldc = P.ldc;
+    const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int64_t gi = row0 + wm0 + i * 16 + lk + 4 * r;
+                int64_t gj = col0 + wn0 + j * 16 + li;
+                if (gi < m && gj < n) {
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v += beta * C[gi * ldc + gj];
+                    C[gi * ldc + gj] = v;
+                }
+            }
+}
+
+}  // namespace syn
+
+extern "C" int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile,
+                            void *stream) {
+    using namespace syn;
+    if (n_items < 0 || total_blocks < 0) {
+        set_error("syn_gemm_f64: negative item or block count");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0 || total_blocks == 0) return SYN_OK;
+    if (total_blocks > 0x7fffffffLL) {
+        set_error("syn_gemm_f64: %lld tiles exceed the grid limit", (long long)total_blocks);
+        return SYN_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (tile == 0)
+        hipLaunchKernelGGL((gemm_f64_kernel<64, 64, 16, 32, 32>), dim3((unsigned)total_blocks), dim3(256), 0, s,
+                           items, n_items, total_blocks);
+    else if (tile == 1)
+        hipLaunchKernelGGL((gemm_f64_kernel<128, 128, 16, 64, 64>), dim3((unsigned)total_blocks), dim3(256), 0,
+                           s, items, n_items, total_blocks);
+    else {
+        set_error("syn_gemm_f64: unknown tile configuration %d", tile);
+        return SYN_EINVAL;
+    }
+    SYN_LAUNCH_CHECK("syn_gemm_f64 launch");
+    return SYN_OK;
+}

@@ -1,0 +1,332 @@
+"""Device operations: thin Python wrappers that pack item tables and call the C ABI.
+
+Every function works on float64 CUDA (HIP) tensors, launches on torch's
+current stream and returns device tensors.  Item tables are packed on the host
+as numpy structured arrays and copied to the device once per launch.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call
+
+F64 = torch.float64
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise _lib.SyngularError("no HIP device visible: syngular_amd runs on an MI355X (gfx950) only")
+    _lib.load()
+
+
+def device():
+    require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _upload(items: np.ndarray) -> torch.Tensor:
+    """Copy an item table to the device without blocking the host: the bytes
+    go through torch's cached pinned-host allocator, which keeps the staging
+    buffer alive until the asynchronous copy on the current stream is done."""
+    raw = np.ascontiguousarray(items).view(np.uint8).reshape(-1)
+    host = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True)
+    host.numpy()[:] = raw
+    return host.to(device(), non_blocking=True)
+
+
+def _ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+def as_matrix(t: torch.Tensor):
+    """(ptr, ld, trans) so that element (i, j) of the 2-D view t is
+    ptr[i*ld + j] (trans=0) or ptr[j*ld + i] (trans=1)."""
+    if t.dim() != 2:
+        raise ValueError("expected a matrix")
+    if t.dtype != F64:
+        raise TypeError(f"expected float64, got {t.dtype}")
+    r, c = t.shape
+    s0, s1 = t.stride()
+    if s1 == 1 or c == 1:
+        return _ptr(t), (s0 if r > 1 else max(c, 1)), 0
+    if s0 == 1 or r == 1:
+        return _ptr(t), (s1 if c > 1 else max(r, 1)), 1
+    raise ValueError("matrix view must be row- or column-contiguous")
+
+
+# --------------------------------------------------------------------------
+# GEMM
+# --------------------------------------------------------------------------
+_TILES = {0: 64, 1: 128}
+
+
+def gemm_many(problems, tile=None):
+    """problems: iterable of (A, B, C, alpha, beta) with A (m x k), B (k x n),
+    C (m x n) 2-D float64 views; C is overwritten with alpha*A@B + beta*C."""
+    problems = [p for p in problems if p[2].numel() > 0]
+    if not problems:
+        return
+    if tile is None:
+        work = sum(p[2].shape[0] * p[2].shape[1] for p in problems)
+        tile = 1 if work >= 256 * 128 * 128 else 0
+    bm = _TILES[tile]
+    items = np.zeros(len(problems), dtype=_lib.GEMM_ITEM)
+    start = 0
+    for i, (A, B, C, alpha, beta) in enumerate(problems):
+        m, k = A.shape
+        k2, n = B.shape
+        if k != k2 or C.shape != (m, n):
+            raise ValueError(f"gemm shape mismatch {tuple(A.shape)} @ {tuple(B.shape)} -> {tuple(C.shape)}")
+        pa, lda, ta = as_matrix(A)
+        pb, ldb, tb = as_matrix(B)
+        pc, ldc, tc = as_matrix(C)
+        if tc:
+            raise ValueError("gemm output must be row-major")
+        tn = (n + bm - 1) // bm
+        tm = (m + bm - 1) // bm
+        it = items[i]
+        it["A"], it["B"], it["C"] = pa, pb, pc
+        it["m"], it["n"], it["k"] = m, n, k
+        it["lda"], it["ldb"], it["ldc"] = lda, ldb, ldc
+        it["alpha"], it["beta"] = alpha, beta
+        it["transA"], it["transB"] = ta, tb
+        it["block_start"], it["tiles_n"] = start, tn
+        start += tm * tn
+    dev = _upload(items)
+    call("syn_gemm_f64", _ptr(dev), len(problems), start, tile, _stream())
+    return dev
+
+
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, tile=None):
+    if C is None:
+        C = torch.empty((A.shape[0], B.shape[1]), dtype=F64, device=A.device)
+        beta = 0.0
+    gemm_many([(A, B, C, alpha, beta)], tile=tile)
+    return C
+
+
+# --------------------------------------------------------------------------
+# site contractions and packing
+# --------------------------------------------------------------------------
+def _blocks(nel):
+    return (nel + _lib.APPLY_BLOCK - 1) // _lib.APPLY_BLOCK
+
+
+def apply_many(pairs):
+    """pairs: list of (O (dl,din,dout,dr), X (chil,din,chir)); returns the
+    materialised cores P (chil*dl, dout, chir*dr)."""
+    outs = []
+    items = np.zeros(len(pairs), dtype=_lib.APPLY_ITEM)
+    start = 0
+    for i, (O, X) in enumerate(pairs):
+        dl, din, dout, dr = O.shape
+        chil, din2, chir = X.shape
+        if din != din2:
+            raise ValueError(f"physical dimension mismatch: MPO input {din} vs MPS {din2}")
+        O = O.contiguous()
+        X = X.contiguous()
+        P = torch.empty((chil * dl, dout, chir * dr), dtype=F64, device=X.device)
+        outs.append((P, O, X))
+        it = items[i]
+        it["x"], it["o"], it["p"] = _ptr(X), _ptr(O), _ptr(P)
+        it["chil"], it["din"], it["chir"], it["dl"], it["dout"], it["dr"] = chil, din, chir, dl, dout, dr
+        it["block_start"] = start
+        start += _blocks(P.numel())
+    if start:
+        dev = _upload(items)
+        call("syn_mpo_apply_f64", _ptr(dev), len(pairs), start, _stream())
+    return [o[0] for o in outs]
+
+
+def mpo_mpo_many(pairs):
+    """pairs: list of (A (al,din,dmid,ar), B (bl,dmid,dout,br)); returns
+    P (al*bl, din, dout, ar*br) with P[a*bl+c, i, p, b*br+e] = sum_j A[a,i,j,b] B[c,j,p,e]."""
+    outs = []
+    items = np.zeros(len(pairs), dtype=_lib.MPO_MPO_ITEM)
+    start = 0
+    for i, (A, B) in enumerate(pairs):
+        al, din, dm, ar = A.shape
+        bl, dm2, dout, br = B.shape
+        if dm != dm2:
+            raise ValueError(f"operator dimension mismatch {dm} vs {dm2}")
+        A = A.contiguous()
+        B = B.contiguous()
+        P = torch.empty((al * bl, din, dout, ar * br), dtype=F64, device=A.device)
+        outs.append((P, A, B))
+        it = items[i]
+        it["a"], it["b"], it["p"] = _ptr(A), _ptr(B), _ptr(P)
+        it["al"], it["din"], it["dmid"], it["ar"] = al, din, dm, ar
+        it["bl"], it["dout"], it["br"] = bl, dout, br
+        it["block_start"] = start
+        start += _blocks(P.numel())
+    if start:
+        dev = _upload(items)
+        call("syn_mpo_mpo_f64", _ptr(dev), len(pairs), start, _stream())
+    return [o[0] for o in outs]
+
+
+def kron_bonds(A, B):
+    """P[(a*bl+c), i, o, (b*br+e)] = A[a,i,o,b] * B[c,i,o,e] (one launch)."""
+    al, din, dout, ar = A.shape
+    bl, _, _, br = B.shape
+    A = A.contiguous()
+    B = B.contiguous()
+    P = torch.empty((al * bl, din, dout, ar * br), dtype=F64, device=A.device)
+    items = np.zeros(1, dtype=_lib.MPO_MPO_ITEM)
+    it = items[0]
+    it["a"], it["b"], it["p"] = _ptr(A), _ptr(B), _ptr(P)
+    it["al"], it["din"], it["dmid"], it["ar"] = al, din, dout, ar
+    it["bl"], it["dout"], it["br"] = bl, dout, br
+    nb = _blocks(P.numel())
+    if nb:
+        dev = _upload(items)
+        call("syn_kron_bonds_f64", _ptr(dev), 1, nb, _stream())
+    return P
+
+
+def direct_sum_many(triples):
+    """triples: list of (A, B, edge) with cores (l, *mid, r); edge 1 = first
+    site, 2 = last site, 0 = interior.  Returns the packed cores."""
+    outs = []
+    items = np.zeros(len(triples), dtype=_lib.DSUM_ITEM)
+    start = 0
+    for i, (A, B, edge) in enumerate(triples):
+        A = A.contiguous()
+        B = B.contiguous()
+        mid_shape = tuple(A.shape[1:-1])
+        if tuple(B.shape[1:-1]) != mid_shape:
+            raise ValueError("could not broadcast: physical dimensions differ")
+        mid = int(np.prod(mid_shape))
+        la, ra, lb, rb = A.shape[0], A.shape[-1], B.shape[0], B.shape[-1]
+        if edge == 1:
+            if la != lb:
+                raise ValueError("could not broadcast: left boundary bonds differ")
+            lo, ro = la, ra + rb
+        elif edge == 2:
+            if ra != rb:
+                raise ValueError("could not broadcast: right boundary bonds differ")
+            lo, ro = la + lb, ra
+        else:
+            lo, ro = la + lb, ra + rb
+        out = torch.empty((lo, *mid_shape, ro), dtype=F64, device=A.device)
+        outs.append((out, A, B))
+        it = items[i]
+        it["a"], it["b"], it["out"] = _ptr(A), _ptr(B), _ptr(out)
+        it["la"], it["ra"], it["lb"], it["rb"], it["mid"], it["lo"], it["ro"] = la, ra, lb, rb, mid, lo, ro
+        it["edge"] = edge
+        it["block_start"] = start
+        start += _blocks(out.numel())
+    if start:
+        dev = _upload(items)
+        call("syn_direct_sum_f64", _ptr(dev), len(triples), start, _stream())
+    return [o[0] for o in outs]
+
+
+def rowscale_many(problems):
+    """problems: list of (X (rows x cols view, row-major), s (rows,), mode)."""
+    items = np.zeros(len(problems), dtype=_lib.ROWSCALE_ITEM)
+    start = 0
+    for i, (X, s, mode) in enumerate(problems):
+        px, ld, tr = as_matrix(X)
+        if tr:
+            raise ValueError("rowscale needs a row-major view")
+        it = items[i]
+        it["x"], it["rows"], it["cols"], it["ld"] = px, X.shape[0], X.shape[1], ld
+        it["s"], it["mode"], it["block_start"] = _ptr(s), mode, start
+        start += _blocks(X.numel())
+    if start:
+        dev = _upload(items)
+        call("syn_rowscale_f64", _ptr(dev), len(problems), start, _stream())
+
+
+# --------------------------------------------------------------------------
+# factorizations
+# --------------------------------------------------------------------------
+def qr_many(problems, q_trans=False):
+    """problems: list of (A (m x n view), keep).  Returns, per problem, the
+    first kq = min(keep, m) columns of the complete Householder Q (m x kq),
+    or its transpose (kq x m) when q_trans."""
+    items = np.zeros(len(problems), dtype=_lib.QR_ITEM)
+    outs = []
+    for i, (A, keep) in enumerate(problems):
+        m, n = A.shape
+        if keep <= 0:
+            raise ValueError("keep must be positive")
+        if A.dtype != F64:
+            raise TypeError("qr expects float64")
+        nb = min(n, keep)
+        kq = min(keep, m)
+        Q = torch.empty((kq, m) if q_trans else (m, kq), dtype=F64, device=A.device)
+        work = torch.empty(max(m * (nb + (kq if q_trans else 0)), 1), dtype=F64, device=A.device)
+        tau = torch.empty(max(nb, 1), dtype=F64, device=A.device)
+        outs.append((Q, work, tau, A))
+        s0, s1 = A.stride()
+        it = items[i]
+        it["a"], it["rs"], it["cs"] = _ptr(A), s0, s1
+        it["m"], it["n"], it["keep"] = m, n, keep
+        it["q"], it["ldq"], it["q_trans"] = _ptr(Q), (m if q_trans else kq), int(q_trans)
+        it["work"], it["tau"] = _ptr(work), _ptr(tau)
+    if len(problems):
+        dev = _upload(items)
+        call("syn_qr_f64", _ptr(dev), len(problems), _stream())
+    return [o[0] for o in outs]
+
+
+JACOBI_TOL = 1e-15
+JACOBI_SWEEPS = 60
+
+
+def jacobi_many(mats, tol=JACOBI_TOL, max_sweeps=JACOBI_SWEEPS, gram=False):
+    """mats: list of row-major (nv x len) float64 tensors, orthogonalised in
+    place.  Returns per problem (rt (nv x nv), s (nv,), sweeps (1,) int32) with
+    x_in = rt^T x_out and rows of x_out sorted by decreasing norm s.  With
+    gram=True each input is a Gram matrix C^T C and s holds the singular
+    values of C (square roots of the row norms)."""
+    items = np.zeros(len(mats), dtype=_lib.JACOBI_ITEM)
+    outs = []
+    for i, X in enumerate(mats):
+        nv, ln = X.shape
+        px, ldx, tr = as_matrix(X)
+        if tr:
+            raise ValueError("jacobi needs a row-major matrix")
+        rt = torch.empty((nv, nv), dtype=F64, device=X.device)
+        s = torch.empty(nv, dtype=F64, device=X.device)
+        sw = torch.zeros(1, dtype=torch.int32, device=X.device)
+        outs.append((rt, s, sw))
+        it = items[i]
+        it["x"], it["ldx"], it["nv"], it["len"] = px, ldx, nv, ln
+        it["rt"], it["ldr"], it["s"], it["sweeps"] = _ptr(rt), nv, _ptr(s), _ptr(sw)
+        it["gram"] = int(gram)
+    if len(mats):
+        dev = _upload(items)
+        call("syn_jacobi_f64", _ptr(dev), len(mats), float(tol), int(max_sweeps), _stream())
+    return outs
+
+
+def truncate_many(problems):
+    """problems: list of (s sorted desc, chi_max or None, cutoff).  Returns
+    (keep int32 (n,), discarded f64 (n,)) device tensors."""
+    n = len(problems)
+    dev_ = device()
+    keep = torch.zeros(n, dtype=torch.int32, device=dev_)
+    disc = torch.zeros(n, dtype=F64, device=dev_)
+    items = np.zeros(n, dtype=_lib.TRUNC_ITEM)
+    for i, (s, chi_max, cutoff) in enumerate(problems):
+        it = items[i]
+        it["s"], it["n"] = _ptr(s), s.numel()
+        it["chi_max"] = 0 if chi_max is None else int(chi_max)
+        it["cutoff"] = float(cutoff or 0.0)
+        it["keep"] = _ptr(keep) + 4 * i
+        it["discarded"] = _ptr(disc) + 8 * i
+    if n:
+        dev = _upload(items)
+        call("syn_truncate_f64", _ptr(dev), n, _stream())
+    return keep, disc

@@ -1,0 +1,185 @@
+"""Kernel-level parity of the gfx950 library against NumPy / the oracle.
+
+Floating-point tolerances (stated per test): FP64 kernels must agree with the
+FP64 oracle to relative error <= 1e-12 on products and <= 1e-10 on
+factorisation results (which depend on the summation order of long dot
+products).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import reference as oref
+from oracle import rounding as ornd
+
+pytestmark = pytest.mark.gpu
+
+
+def dev():
+    from syngular_amd import ops
+
+    return ops.device()
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev())
+
+
+def case(name):
+    print(f"[gpu case] {name}", flush=True)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (7, 5, 3), (64, 64, 64), (130, 67, 129), (257, 300, 31), (512, 384, 700)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("tile", [0, 1])
+def test_gemm_f64(m, n, k, ta, tb, tile):
+    from syngular_amd import ops
+
+    case(f"gemm m={m} n={n} k={k} ta={ta} tb={tb} tile={tile}")
+    rng = np.random.default_rng(m * 131 + n * 7 + k + ta * 3 + tb)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    B = rng.standard_normal((n, k) if tb else (k, n))
+    C0 = rng.standard_normal((m, n))
+    a, b = T(A), T(B)
+    c = T(C0)
+    ops.gemm_many([(a.t() if ta else a, b.t() if tb else b, c, 0.75, -0.5)], tile=tile)
+    ref = 0.75 * ((A.T if ta else A) @ (B.T if tb else B)) - 0.5 * C0
+    got = c.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref))) * np.sqrt(k)
+
+
+def test_gemm_grouped_uneven():
+    from syngular_amd import ops
+
+    case("gemm grouped uneven")
+    rng = np.random.default_rng(3)
+    probs, refs = [], []
+    for (m, n, k) in [(3, 9, 4), (100, 1, 17), (65, 65, 65), (1, 200, 50), (33, 129, 2)]:
+        A, B = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+        C = T(np.zeros((m, n)))
+        probs.append((T(A), T(B), C, 1.0, 0.0))
+        refs.append(A @ B)
+    ops.gemm_many(probs)
+    for (_, _, C, _, _), r in zip(probs, refs):
+        np.testing.assert_allclose(C.cpu().numpy(), r, rtol=1e-12, atol=1e-12)
+
+
+def test_apply_and_mpo_mpo_and_kron():
+    from syngular_amd import ops
+
+    case("apply / mpo_mpo / kron")
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((3, 4, 5))
+    O = rng.standard_normal((2, 4, 3, 6))
+    (P,) = ops.apply_many([(T(O), T(X))])
+    np.testing.assert_allclose(P.cpu().numpy(), oref.apply_site(O, X), rtol=1e-13, atol=1e-13)
+    A = rng.standard_normal((2, 3, 4, 3))
+    B = rng.standard_normal((3, 4, 2, 2))
+    (Q,) = ops.mpo_mpo_many([(T(A), T(B))])
+    ref = np.einsum("aijb,cjpe->acipbe", A, B).reshape(6, 3, 2, 6)
+    np.testing.assert_allclose(Q.cpu().numpy(), ref, rtol=1e-13, atol=1e-13)
+    B2 = rng.standard_normal((2, 3, 4, 5))
+    K = ops.kron_bonds(T(A), T(B2)).cpu().numpy()
+    for i in range(3):
+        for o in range(4):
+            np.testing.assert_allclose(K[:, i, o, :], np.kron(A[:, i, o, :], B2[:, i, o, :]), rtol=1e-14)
+
+
+def test_direct_sum_edges():
+    from syngular_amd import ops
+
+    case("direct sum")
+    rng = np.random.default_rng(5)
+    a0, b0 = rng.standard_normal((1, 3, 2)), rng.standard_normal((1, 3, 4))
+    a1, b1 = rng.standard_normal((2, 3, 5)), rng.standard_normal((4, 3, 1))
+    a2, b2 = rng.standard_normal((5, 3, 1)), rng.standard_normal((1, 3, 1))
+    outs = ops.direct_sum_many([(T(a0), T(b0), 1), (T(a1), T(b1), 0), (T(a2), T(b2), 2)])
+    A = oref.MPS.from_sites([a0, a1, a2])
+    B = oref.MPS.from_sites([b0, b1, b2])
+    for got, ref in zip(outs, (A + B).sites):
+        np.testing.assert_array_equal(got.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("m,n,keep", [(6, 4, 4), (20, 7, 3), (9, 9, 9), (4, 16, 2), (300, 40, 40), (40, 3, 10),
+                                      (1100, 64, 64)])
+def test_householder_qr_matches_lapack(m, n, keep):
+    from syngular_amd import ops
+
+    case(f"qr m={m} n={n} keep={keep}")
+    rng = np.random.default_rng(m + n + keep)
+    L = rng.standard_normal((m, n))
+    (Q,) = ops.qr_many([(T(L), keep)])
+    Qn, Rn = np.linalg.qr(L, mode="complete")
+    kq = min(keep, m)
+    got = Q.cpu().numpy()
+    assert got.shape == (m, kq)
+    # same reflector convention as LAPACK: the columns agree, not only their span
+    np.testing.assert_allclose(got, Qn[:, :kq], atol=1e-12)
+    (Qt,) = ops.qr_many([(T(L), keep)], q_trans=True)
+    np.testing.assert_allclose(Qt.cpu().numpy(), Qn[:, :kq].T, atol=1e-12)
+
+
+def test_householder_qr_strided_and_rank_deficient():
+    from syngular_amd import ops
+
+    case("qr strided / rank deficient")
+    rng = np.random.default_rng(6)
+    L = rng.standard_normal((7, 12))
+    (Q,) = ops.qr_many([(T(L).t(), 7)])  # QR of the transposed view
+    Qn, _ = np.linalg.qr(L.T, mode="complete")
+    np.testing.assert_allclose(Q.cpu().numpy(), Qn[:, :7], atol=1e-12)
+    x = np.arange(64).reshape(4, 16).astype(float)  # rank 2
+    (Q,) = ops.qr_many([(T(x), 4)])
+    q = Q.cpu().numpy()
+    np.testing.assert_allclose(q.T @ q, np.eye(4), atol=1e-12)
+    np.testing.assert_allclose(q @ (q.T @ x), x, atol=1e-10)
+
+
+@pytest.mark.parametrize("nv,ln", [(2, 3), (5, 8), (16, 40), (33, 33), (64, 200), (100, 120)])
+def test_jacobi_svd(nv, ln):
+    from syngular_amd import ops
+
+    case(f"jacobi nv={nv} len={ln}")
+    rng = np.random.default_rng(nv * 100 + ln)
+    X0 = rng.standard_normal((nv, ln))
+    x = T(X0)
+    rt, s, sw = ops.jacobi_many([x])[0]
+    s_ref = np.linalg.svd(X0, compute_uv=False)
+    k = min(nv, ln)
+    np.testing.assert_allclose(s.cpu().numpy()[:k], s_ref, rtol=1e-12)
+    R = rt.cpu().numpy()
+    np.testing.assert_allclose(R @ R.T, np.eye(nv), atol=1e-12)
+    np.testing.assert_allclose(R.T @ x.cpu().numpy(), X0, atol=1e-11)
+    assert 0 < int(sw.item()) < 60
+
+
+def test_jacobi_gram_mode():
+    from syngular_amd import ops
+
+    case("jacobi gram")
+    rng = np.random.default_rng(8)
+    C = rng.standard_normal((50, 12))
+    G = T(C.T @ C)
+    rt, s, _ = ops.jacobi_many([G], gram=True)[0]
+    np.testing.assert_allclose(s.cpu().numpy(), np.linalg.svd(C, compute_uv=False), rtol=1e-11)
+    V = rt.cpu().numpy().T
+    U = C @ V / s.cpu().numpy()
+    np.testing.assert_allclose(U.T @ U, np.eye(12), atol=1e-10)
+
+
+def test_truncate_matches_oracle_rule():
+    from syngular_amd import ops
+
+    case("truncate")
+    rng = np.random.default_rng(9)
+    cases = []
+    for n in (1, 5, 64, 65, 200):
+        s = np.sort(np.exp(rng.uniform(-12, 0, n)))[::-1].copy()
+        for chi, cut in [(None, 0.0), (3, 0.0), (None, 1e-6), (10, 1e-3), (None, 0.5), (1000, 1e-12)]:
+            cases.append((s, chi, cut))
+    keep, disc = ops.truncate_many([(T(s), chi, cut) for s, chi, cut in cases])
+    keep, disc = keep.cpu().numpy(), disc.cpu().numpy()
+    for i, (s, chi, cut) in enumerate(cases):
+        c = ornd.keep_count(s, chi, cut)
+        assert keep[i] == c, (len(s), chi, cut, keep[i], c)
+        np.testing.assert_allclose(disc[i], np.sum(s[c:] ** 2), rtol=1e-12, atol=1e-300)

@@ -251,6 +251,15 @@ def test_apply_dense_identity():
     np.testing.assert_allclose(R.mul(O3, O2).to_tensor().reshape(24, 24), a @ b, atol=1e-12)
 
 
+def test_mpo_hadamard_dense():
+    # tensor/matrix_product_operator.py:126-161 with the implicit truncation off
+    rng = np.random.default_rng(12)
+    a = R.MPO.from_sites([rng.standard_normal(s) for s in [(1, 2, 3, 2), (2, 2, 2, 1)]])
+    b = R.MPO.from_sites([rng.standard_normal(s) for s in [(1, 2, 3, 3), (3, 2, 2, 1)]])
+    R.MPO.COMPRESS = False
+    np.testing.assert_allclose((a * b).to_tensor(), a.to_tensor() * b.to_tensor(), atol=1e-12)
+
+
 def test_add_dense():
     rng = np.random.default_rng(6)
     A = R.MPS.from_sites(random_sites(rng, (2, 3, 2), (2, 3)))
