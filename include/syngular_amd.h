@@ -182,6 +182,36 @@ typedef struct {
 int syn_jacobi_f64(const syn_jacobi_item *items, int n_items, double tol, int max_sweeps, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Block one-sided Jacobi for large problems: same contract as            */
+/* syn_jacobi_f64 for one problem (x: nv x len rows, rt: nv x nv, s: nv),  */
+/* but every step spreads the disjoint 16-row block pairs over many        */
+/* work-groups.  The host loop synchronises the stream once per sweep.    */
+/* work: syn_jacobi_blocked_workspace(nv, len) doubles.  *sweeps (host    */
+/* pointer, may be null) receives the sweeps used.                        */
+/* ------------------------------------------------------------------ */
+int64_t syn_jacobi_blocked_workspace(int64_t nv, int64_t len);
+int syn_jacobi_blocked_f64(double *x, int64_t nv, int64_t len, int64_t ldx, double *rt, int64_t ldr, double *s,
+                           double *work, double tol, int max_sweeps, int gram, int *sweeps, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Cholesky / CholeskyQR (left-orthonormalisation of large unfoldings)   */
+/* syn_potrf_f64: G = L L^T (lower) by blocks of 64; G is destroyed, L is */
+/* written in full (zeros above the diagonal).  A pivot below            */
+/* tolrel * max_i G_ii is dropped (zero column of L); *dropped (device   */
+/* int) counts them.  work: syn_cholqr_workspace(n) doubles suffice.     */
+/* syn_cholqr_f64: one CholeskyQR pass on the m x n matrix A (destroyed): */
+/* G = A^T A, G = L L^T, Q = A L^{-T} (Q: m x n, ldq).  Two passes give   */
+/* the reduced QR A = Q (L1 L2)^T with Q orthonormal to working          */
+/* precision.  Replaces numpy.linalg.qr on large left unfoldings          */
+/* (tensor/matrix_product_state.py:560, the left sweep of the rounding). */
+/* ------------------------------------------------------------------ */
+int64_t syn_cholqr_workspace(int64_t n);
+int syn_potrf_f64(double *g, int64_t n, int64_t ldg, double *l, int64_t ldl, double *work, double tolrel,
+                  int *dropped, void *stream);
+int syn_cholqr_f64(double *a, int64_t m, int64_t n, int64_t lda, double *q, int64_t ldq, double *l, int64_t ldl,
+                   double *work, double tolrel, int *dropped, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Device-side truncation: for sorted singular values s[0..n-1] keep the */
 /* smallest c with sum_{i>=c} s_i^2 <= cutoff * sum_i s_i^2 (cutoff>0),  */
 /* then c = min(c, chi_max) (chi_max>0), c >= 1.  Writes c and the       */
@@ -210,6 +240,25 @@ typedef struct {
     int64_t block_start;
 } syn_rowscale_item;
 int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_blocks, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Live per-kernel timing (HIP events on the launch stream).             */
+/* Between syn_prof_begin and syn_prof_end every launch of the heavy     */
+/* kernels is bracketed by events; syn_prof_end fills, per kernel tag,   */
+/* launches, milliseconds and algorithmic flops (arrays of ntags) and    */
+/* returns the number of tags.  Tags: 0/1 GEMM by value 64/128 tiles,     */
+/* 2/3 grouped GEMM 64/128 tiles, 4 block-Jacobi step, 5 Cholesky        */
+/* diagonal block, 6 Householder QR, 7 LDS Jacobi, 8 MPO x MPS apply.    */
+/* syn_prof_note_flops sets the count attached to the next grouped GEMM */
+/* or apply launch (their tables live on the device).                    */
+/* ------------------------------------------------------------------ */
+#define SYN_PROF_NTAGS 16
+int syn_prof_begin(void);
+int syn_prof_note_flops(double flops);
+int syn_prof_end(int64_t *launches, double *ms, double *flops, int ntags);
+/* Tuning knob of syn_jacobi_blocked_f64: rows per Jacobi block (8 or 16;  */
+/* anything else restores the size-based default).                       */
+void syn_tune_jacobi_block_rows(int rows);
 
 #ifdef __cplusplus
 }

@@ -69,7 +69,48 @@ SIGNATURES = {
     "syn_jacobi_f64": (_I32, [_VP, _I32, _D, _I32, _VP]),
     "syn_truncate_f64": (_I32, [_VP, _I32, _VP]),
     "syn_rowscale_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_jacobi_blocked_workspace": (_I64, [_I64, _I64]),
+    "syn_jacobi_blocked_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _VP, _D, _I32, _I32,
+                                      ctypes.POINTER(ctypes.c_int), _VP]),
+    "syn_cholqr_workspace": (_I64, [_I64]),
+    "syn_potrf_f64": (_I32, [_VP, _I64, _I64, _VP, _I64, _VP, _D, _VP, _VP]),
+    "syn_cholqr_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _D, _VP, _VP]),
+    "syn_prof_begin": (_I32, []),
+    "syn_prof_note_flops": (_I32, [_D]),
+    "syn_prof_end": (_I32, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double), _I32]),
+    "syn_tune_jacobi_block_rows": (None, [_I32]),
 }
+
+PROF_NTAGS = 16
+PROF_TAGS = {
+    0: "gemm_f64_single<64,64>",
+    1: "gemm_f64_single<128,128>",
+    2: "gemm_f64_grouped<64,64>",
+    3: "gemm_f64_grouped<128,128>",
+    4: "bjacobi_step_kernel",
+    5: "chol_diag_kernel",
+    6: "qr_kernel",
+    7: "jacobi_kernel",
+    8: "apply_kernel",
+}
+
+
+def prof_begin():
+    call("syn_prof_begin")
+
+
+def prof_end():
+    """Stop profiling; returns {tag_name: (launches, ms, flops)} for the tags
+    that recorded launches."""
+    n = PROF_NTAGS
+    la = (ctypes.c_int64 * n)()
+    ms = (ctypes.c_double * n)()
+    fl = (ctypes.c_double * n)()
+    rc = load().syn_prof_end(la, ms, fl, n)
+    if rc < 0:
+        raise SyngularError(f"syn_prof_end failed ({rc}): {load().syn_last_error().decode()}")
+    return {PROF_TAGS.get(i, f"tag{i}"): (la[i], ms[i], fl[i]) for i in range(n) if la[i]}
 
 
 class SyngularError(RuntimeError):
@@ -111,6 +152,8 @@ def check_abi():
 
 def call(name, *args):
     rc = getattr(load(), name)(*args)
+    if rc is None:
+        return rc
     if rc != SYN_OK:
         msg = load().syn_last_error().decode(errors="replace")
         if rc == SYN_EINVAL:

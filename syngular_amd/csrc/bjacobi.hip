@@ -1,0 +1,348 @@
+// Block one-sided Jacobi SVD for large problems (many work-groups per step).
+//
+// The nv rows of X are split into blocks of kB rows.  One sweep pairs every
+// two blocks once (round-robin over the blocks); in a step the pairs are
+// disjoint, so each runs on its own work-group:
+//   S = Y Y^T for the 2kB rows Y of the pair (f64 MFMA, operands from L2),
+//   S = W diag W^T by cyclic two-sided Jacobi in LDS (kInner sweeps at most),
+//   Y <- W^T Y and the same rows of rt <- W^T rt (f64 MFMA).
+// A pair whose scaled off-diagonal max |S_ik| / sqrt(S_ii S_kk) is below tol
+// is left alone; the sweep loop stops after a sweep without rotations.  This
+// is the block form of Hestenes' method (Hari & Veselic; Becka, Oksa &
+// Vajtersic).  Rows end orthogonal with x_in = rt^T x_out; norms and the sort
+// by decreasing norm finish the decomposition.
+#include "common.hpp"
+#include "prof.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace syn {
+
+constexpr int kInner = 3;     // max inner Jacobi sweeps on S (the outer sweep repeats)
+constexpr int kChunk = 128;   // columns of the pair staged in LDS at a time
+
+__device__ __forceinline__ void rr_pair(int64_t step, int64_t w, int64_t np, int64_t &a, int64_t &b) {
+    if (w == 0) {
+        a = step;
+        b = np - 1;
+    } else {
+        a = (step + w) % (np - 1);
+        b = (step - w + np - 1) % (np - 1);
+    }
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Stage rows[0..P) x columns [c0, c0+kChunk) of M into Ys.  All of a
+// thread's loads are issued before the first LDS store, so their latencies
+// overlap instead of adding up.
+template <int P, int NT>
+__device__ __forceinline__ void stage_chunk(double (*Ys)[kChunk + 4], const int *rows, const double *M, int64_t ld,
+                                            int64_t cols, int64_t c0, int tid) {
+    constexpr int PER = P * kChunk / NT;
+    static_assert(PER * NT == P * kChunk, "chunk / threads mismatch");
+    double v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * NT, i = e / kChunk, t = e % kChunk;
+        const int r = rows[i];
+        v[q] = (r >= 0 && c0 + t < cols) ? M[(int64_t)r * ld + c0 + t] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * NT;
+        Ys[e / kChunk][e % kChunk] = v[q];
+    }
+}
+
+// MFMA f64 16x16x4 layout: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
+// D reg r of lane l = D[l/16 + 4r][l%16].
+//
+// KB rows per block, NT threads per work-group (NT/64 waves).  KB = 8 with a
+// single wave keeps the inner 16 x 16 problem free of cross-wave barriers.
+template <int KB, int NT>
+__global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv, int64_t len, int64_t ldx, double *R,
+                                                          int64_t ldr, int64_t nblk, int64_t step, double tol,
+                                                          int *counter) {
+    constexpr int P = 2 * KB;        // rows of a pair
+    constexpr int RT = P / 16;       // 16-row MFMA tiles
+    constexpr int NWV = NT / 64;     // waves
+    __shared__ double S[P][P + 1];
+    __shared__ double W[P][P + 1];
+    __shared__ double Ys[P][kChunk + 4];
+    __shared__ double cs[P / 2][2];
+    __shared__ int rows[P];
+    __shared__ unsigned char pq[P - 1][P / 2][2];  // inner round-robin pairs
+    __shared__ int do_rot;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int64_t np = nblk + (nblk & 1);
+    int64_t ba, bb;
+    rr_pair(step, blockIdx.x, np, ba, bb);
+    if (ba >= nblk && bb >= nblk) return;  // a block paired with the padding slot still
+    if (tid < P) {                           // gets its own rows orthogonalised
+        const int64_t blk = tid < KB ? ba : bb;
+        const int64_t r = blk * KB + (tid % KB);
+        rows[tid] = (blk < nblk && r < nv) ? (int)r : -1;
+    }
+    for (int e = tid; e < P * P; e += NT) W[e / P][e % P] = (e / P == e % P) ? 1.0 : 0.0;
+    for (int e = tid; e < (P - 1) * (P / 2); e += NT) {
+        const int r = e / (P / 2), pw = e % (P / 2);
+        int64_t a, b;
+        rr_pair(r, pw, P, a, b);
+        pq[r][pw][0] = (unsigned char)(a < b ? a : b);
+        pq[r][pw][1] = (unsigned char)(a < b ? b : a);
+    }
+    if (tid == 0) do_rot = 0;
+    __syncthreads();
+
+    // S = Y Y^T on the matrix cores, rows staged through LDS in chunks
+    v4d sacc[(RT * RT + NWV - 1) / NWV];
+#pragma unroll
+    for (int q = 0; q < (RT * RT + NWV - 1) / NWV; ++q) sacc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int64_t c0 = 0; c0 < len; c0 += kChunk) {
+        stage_chunk<P, NT>(Ys, rows, X, ldx, len, c0, tid);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < (RT * RT + NWV - 1) / NWV; ++q) {
+            const int tile = w + q * NWV;
+            if (tile < RT * RT) {
+                const int ta = tile / RT, tb = tile % RT;
+#pragma unroll 8
+                for (int k = 0; k < kChunk; k += 4)
+                    sacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ys[ta * 16 + li][k + lk], Ys[tb * 16 + li][k + lk],
+                                                                   sacc[q], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < (RT * RT + NWV - 1) / NWV; ++q) {
+        const int tile = w + q * NWV;
+        if (tile < RT * RT) {
+            const int ta = tile / RT, tb = tile % RT;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) S[ta * 16 + lk + 4 * r][tb * 16 + li] = sacc[q][r];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < P * P; e += NT) {
+        const int i = e / P, k = e % P;
+        if (i < k && fabs(S[i][k]) > tol * sqrt(S[i][i]) * sqrt(S[k][k])) do_rot = 1;
+    }
+    __syncthreads();
+    if (!do_rot) return;
+
+    // cyclic two-sided Jacobi on S, accumulating W (S_final = W^T S W)
+    for (int sweep = 0; sweep < kInner; ++sweep) {
+        int any = 0;
+        for (int r = 0; r < P - 1; ++r) {
+            if (tid < P / 2) {
+                const int p = pq[r][tid][0], q = pq[r][tid][1];
+                const double apq = S[p][q], app = S[p][p], aqq = S[q][q];
+                double c = 1.0, s = 0.0;
+                if (fabs(apq) > tol * sqrt(fabs(app)) * sqrt(fabs(aqq)) && apq != 0.0) {
+                    const double zeta = (aqq - app) / (2.0 * apq);
+                    const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(fma(zeta, zeta, 1.0)));
+                    c = 1.0 / sqrt(fma(t, t, 1.0));
+                    s = c * t;
+                }
+                cs[tid][0] = c;
+                cs[tid][1] = s;
+            }
+            __syncthreads();
+            for (int e = tid; e < (P / 2) * P; e += NT) {  // rows p, q of S
+                const int pr = e / P, k = e % P;
+                const int p = pq[r][pr][0], q = pq[r][pr][1];
+                const double c = cs[pr][0], s = cs[pr][1];
+                if (s != 0.0) {
+                    const double u = S[p][k], v = S[q][k];
+                    S[p][k] = c * u - s * v;
+                    S[q][k] = s * u + c * v;
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < (P / 2) * P; e += NT) {  // columns p, q of S and W
+                const int pr = e / P, i = e % P;
+                const int p = pq[r][pr][0], q = pq[r][pr][1];
+                const double c = cs[pr][0], s = cs[pr][1];
+                if (s != 0.0) {
+                    double u = S[i][p], v = S[i][q];
+                    S[i][p] = c * u - s * v;
+                    S[i][q] = s * u + c * v;
+                    u = W[i][p];
+                    v = W[i][q];
+                    W[i][p] = c * u - s * v;
+                    W[i][q] = s * u + c * v;
+                }
+            }
+            for (int pr = 0; pr < P / 2; ++pr) any |= (cs[pr][1] != 0.0);
+            __syncthreads();
+        }
+        if (!any) break;
+    }
+
+    // Y <- W^T Y on the matrix cores, chunk by chunk through LDS; a wave forms
+    // all 16-row output tiles of its 16-column blocks before writing them.
+    for (int pass = 0; pass < 2; ++pass) {
+        double *M = pass ? R : X;
+        const int64_t ld = pass ? ldr : ldx, cols = pass ? nv : len;
+        for (int64_t c0 = 0; c0 < cols; c0 += kChunk) {
+            stage_chunk<P, NT>(Ys, rows, M, ld, cols, c0, tid);
+            __syncthreads();
+            for (int cb = w; cb < kChunk / 16; cb += NWV) {
+                const int tc = cb * 16 + li;
+                v4d acc[RT];
+#pragma unroll
+                for (int ti = 0; ti < RT; ++ti) acc[ti] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < P / 4; ++kk) {
+                    const int kr = kk * 4 + lk;
+                    const double b = Ys[kr][tc];
+#pragma unroll
+                    for (int ti = 0; ti < RT; ++ti)
+                        acc[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(W[kr][ti * 16 + li], b, acc[ti], 0, 0, 0);
+                }
+                const int64_t col = c0 + tc;
+                if (col < cols) {
+#pragma unroll
+                    for (int ti = 0; ti < RT; ++ti)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int rr = rows[ti * 16 + lk + 4 * r];
+                            if (rr >= 0) M[(int64_t)rr * ld + col] = acc[ti][r];
+                        }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) atomicAdd(counter, 1);
+}
+
+__global__ __launch_bounds__(256) void identity_kernel(double *R, int64_t n, int64_t ld) {
+    const int64_t nel = n * n;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nel; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e / n, j = e - i * n;
+        R[i * ld + j] = (i == j) ? 1.0 : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_int_kernel(int *c) { *c = 0; }
+
+__global__ __launch_bounds__(256) void rownorm_kernel(const double *X, int64_t nv, int64_t len, int64_t ldx,
+                                                      double *s, int gram) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t i = blockIdx.x * 4 + w;
+    if (i >= nv) return;
+    double a = 0.0;
+    for (int64_t t = lane; t < len; t += 64) a = fma(X[i * ldx + t], X[i * ldx + t], a);
+    a = wave_sum(a);
+    if (lane == 0) s[i] = gram ? sqrt(sqrt(a)) : sqrt(a);
+}
+
+// dst[i, :] = src[perm[i], :]
+__global__ __launch_bounds__(256) void gather_rows_kernel(const double *src, int64_t lds, double *dst, int64_t ldd,
+                                                          int64_t rows, int64_t cols, const int64_t *perm) {
+    const int64_t nel = rows * cols;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nel; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e / cols, j = e - i * cols;
+        dst[i * ldd + j] = src[perm[i] * lds + j];
+    }
+}
+
+static unsigned grid_for(int64_t nel) {
+    int64_t b = (nel + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+}  // namespace syn
+
+extern "C" int64_t syn_jacobi_blocked_workspace(int64_t nv, int64_t len) {
+    // sorted copies of X and rt, the norms, the permutation and a counter
+    return nv * len + nv * nv + 2 * nv + 8;
+}
+
+// Rows per block: 16 (four waves per pair) from 384 rows up, else 8 (one wave).
+static int g_block_rows_override = 0;
+
+extern "C" void syn_tune_jacobi_block_rows(int rows) { g_block_rows_override = (rows == 8 || rows == 16) ? rows : 0; }
+
+extern "C" int syn_jacobi_blocked_f64(double *X, int64_t nv, int64_t len, int64_t ldx, double *rt, int64_t ldr,
+                                      double *s, double *work, double tol, int max_sweeps, int gram, int *sweeps,
+                                      void *stream) {
+    const int block_rows = g_block_rows_override ? g_block_rows_override : (nv >= 384 ? 16 : 8);
+    using namespace syn;
+    if (nv <= 0 || len <= 0 || ldx < len || ldr < nv || !X || !rt || !s || !work || max_sweeps < 0 ||
+        !(tol >= 0.0)) {
+        set_error("syn_jacobi_blocked_f64: bad arguments (nv=%lld len=%lld)", (long long)nv, (long long)len);
+        return SYN_EINVAL;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    double *Xw = work;
+    double *Rw = Xw + nv * len;
+    double *sw = Rw + nv * nv;
+    int64_t *perm = (int64_t *)(sw + nv);
+    int *counter = (int *)(perm + nv);
+
+    hipLaunchKernelGGL(identity_kernel, dim3(grid_for(nv * nv)), dim3(256), 0, st, rt, nv, ldr);
+    SYN_LAUNCH_CHECK("identity");
+    const int kb = (block_rows == 16) ? 16 : 8;
+    const int64_t nblk = (nv + kb - 1) / kb;
+    const int64_t np = nblk + (nblk & 1);
+    int used = 0;
+    for (; used < max_sweeps && nblk > 0; ++used) {
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, counter);
+        for (int64_t step = 0; step < np - 1; ++step) {
+            // flops of a step: the pair Grams and the row updates of X and rt
+            ProfScope ps(kTagBJacobiStep,
+                         (double)(np / 2) * 2.0 * (2.0 * kb) * (2.0 * kb) * (double)(2 * len + nv), st);
+            if (kb == 16)
+                hipLaunchKernelGGL((bjacobi_step_kernel<16, 256>), dim3((unsigned)(np / 2)), dim3(256), 0, st, X, nv,
+                                   len, ldx, rt, ldr, nblk, step, tol, counter);
+            else
+                hipLaunchKernelGGL((bjacobi_step_kernel<8, 64>), dim3((unsigned)(np / 2)), dim3(64), 0, st, X, nv,
+                                   len, ldx, rt, ldr, nblk, step, tol, counter);
+        }
+        SYN_LAUNCH_CHECK("bjacobi_step");
+        int host_count = 0;
+        if (int rc = hip_check(hipMemcpyAsync(&host_count, counter, sizeof(int), hipMemcpyDeviceToHost, st),
+                               "bjacobi counter copy"))
+            return rc;
+        if (int rc = hip_check(hipStreamSynchronize(st), "bjacobi sync")) return rc;
+        if (host_count == 0) {
+            ++used;
+            break;
+        }
+    }
+    if (sweeps) *sweeps = used;
+
+    // norms, host argsort (descending, stable), gather into the workspace and back
+    hipLaunchKernelGGL(rownorm_kernel, dim3((unsigned)((nv + 3) / 4)), dim3(256), 0, st, X, nv, len, ldx, sw, gram);
+    SYN_LAUNCH_CHECK("rownorm");
+    std::vector<double> hs(nv);
+    std::vector<int64_t> hp(nv);
+    if (int rc = hip_check(hipMemcpyAsync(hs.data(), sw, nv * sizeof(double), hipMemcpyDeviceToHost, st), "norms"))
+        return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "norms sync")) return rc;
+    std::iota(hp.begin(), hp.end(), 0);
+    std::stable_sort(hp.begin(), hp.end(), [&](int64_t a, int64_t b) { return hs[a] > hs[b]; });
+    if (int rc = hip_check(hipMemcpyAsync(perm, hp.data(), nv * sizeof(int64_t), hipMemcpyHostToDevice, st), "perm"))
+        return rc;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for(nv * len)), dim3(256), 0, st, X, ldx, Xw, len, nv, len, perm);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for(nv * nv)), dim3(256), 0, st, rt, ldr, Rw, nv, nv, nv, perm);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for(nv)), dim3(256), 0, st, sw, 1, s, 1, nv, 1, perm);
+    SYN_LAUNCH_CHECK("gather");
+    // copy the sorted rows back
+    if (int rc = hip_check(hipMemcpy2DAsync(X, ldx * sizeof(double), Xw, len * sizeof(double), len * sizeof(double),
+                                            nv, hipMemcpyDeviceToDevice, st),
+                           "copy X"))
+        return rc;
+    if (int rc = hip_check(hipMemcpy2DAsync(rt, ldr * sizeof(double), Rw, nv * sizeof(double), nv * sizeof(double),
+                                            nv, hipMemcpyDeviceToDevice, st),
+                           "copy rt"))
+        return rc;
+    // the host vectors are read by the copies above: wait before they go out of scope
+    return hip_check(hipStreamSynchronize(st), "bjacobi final sync");
+}

@@ -14,14 +14,14 @@
 //   D: reg r of lane l holds D[i = l/16 + 4*r][j = l%16]  (checked against
 //      NumPy on the device: rows 4g+r and g+4r differ unless g == r)
 #include "common.hpp"
+#include "prof.hpp"
 
 namespace syn {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
-    const syn_gemm_item *__restrict__ items, int n_items, int64_t total) {
+__device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t) {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NT = NW * 64;
     constexpr int TI = WM / 16, TJ = WN / 16;
@@ -30,9 +30,6 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
     __shared__ double As[2][BK][BM + 1];
     __shared__ double Bs[2][BK][BN + 1];
 
-    const int64_t bid = xcd_remap(blockIdx.x, total);
-    const syn_gemm_item &P = items[find_item(items, n_items, bid)];
-    const int64_t t = bid - P.block_start;
     const int64_t row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
     const int64_t m = P.m, n = P.n, k = P.k, lda = P.lda, ldb = P.ldb;
     const int ta = P.transA, tb = P.transB;
@@ -128,6 +125,59 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_kernel(
             }
 }
 
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_grouped(
+    const syn_gemm_item *__restrict__ items, int n_items, int64_t total) {
+    const int64_t bid = xcd_remap(blockIdx.x, total);
+    const syn_gemm_item &P = items[find_item(items, n_items, bid)];
+    gemm_f64_tile<BM, BN, BK, WM, WN>(P, bid - P.block_start);
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_single(syn_gemm_item P) {
+    const int64_t total = (int64_t)gridDim.x;
+    gemm_f64_tile<BM, BN, BK, WM, WN>(P, xcd_remap(blockIdx.x, total));
+}
+
+// One GEMM launched with its descriptor passed by value (no device table):
+// used by the library's own multi-step routines (Cholesky, CholeskyQR, ...).
+int gemm_f64_launch(syn_gemm_item P, hipStream_t s) {
+    if (P.m <= 0 || P.n <= 0) return SYN_OK;
+    if (P.k <= 0 && P.beta == 1.0) return SYN_OK;
+    const int tile = (P.m * P.n >= (int64_t)256 * 128 * 128) ? 1 : 0;
+    const int64_t bm = tile ? 128 : 64;
+    P.tiles_n = (P.n + bm - 1) / bm;
+    const int64_t tiles = ((P.m + bm - 1) / bm) * P.tiles_n;
+    P.block_start = 0;
+    ProfScope ps(tile ? kTagGemmSingle128 : kTagGemmSingle64, 2.0 * (double)P.m * (double)P.n * (double)P.k, s);
+    if (tile)
+        hipLaunchKernelGGL((gemm_f64_single<128, 128, 16, 64, 64>), dim3((unsigned)tiles), dim3(256), 0, s, P);
+    else
+        hipLaunchKernelGGL((gemm_f64_single<64, 64, 16, 32, 32>), dim3((unsigned)tiles), dim3(256), 0, s, P);
+    SYN_LAUNCH_CHECK("gemm_f64_launch");
+    return SYN_OK;
+}
+
+syn_gemm_item make_gemm(const double *A, int64_t lda, int ta, const double *B, int64_t ldb, int tb, double *C,
+                        int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta) {
+    syn_gemm_item P;
+    memset(&P, 0, sizeof(P));
+    P.A = A;
+    P.B = B;
+    P.C = C;
+    P.m = m;
+    P.n = n;
+    P.k = k;
+    P.lda = lda;
+    P.ldb = ldb;
+    P.ldc = ldc;
+    P.alpha = alpha;
+    P.beta = beta;
+    P.transA = ta;
+    P.transB = tb;
+    return P;
+}
+
 }  // namespace syn
 
 extern "C" int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile,
@@ -143,11 +193,12 @@ extern "C" int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t tot
         return SYN_EINVAL;
     }
     hipStream_t s = (hipStream_t)stream;
+    ProfScope ps(tile == 1 ? kTagGemmGrouped128 : kTagGemmGrouped64, -1.0, s);
     if (tile == 0)
-        hipLaunchKernelGGL((gemm_f64_kernel<64, 64, 16, 32, 32>), dim3((unsigned)total_blocks), dim3(256), 0, s,
+        hipLaunchKernelGGL((gemm_f64_grouped<64, 64, 16, 32, 32>), dim3((unsigned)total_blocks), dim3(256), 0, s,
                            items, n_items, total_blocks);
     else if (tile == 1)
-        hipLaunchKernelGGL((gemm_f64_kernel<128, 128, 16, 64, 64>), dim3((unsigned)total_blocks), dim3(256), 0,
+        hipLaunchKernelGGL((gemm_f64_grouped<128, 128, 16, 64, 64>), dim3((unsigned)total_blocks), dim3(256), 0,
                            s, items, n_items, total_blocks);
     else {
         set_error("syn_gemm_f64: unknown tile configuration %d", tile);

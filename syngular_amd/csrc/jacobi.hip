@@ -13,6 +13,7 @@
 // case) and in the caller's buffers (L2) otherwise.  Finally rows are sorted
 // by decreasing norm with in-place swaps.
 #include "common.hpp"
+#include "prof.hpp"
 
 namespace syn {
 
@@ -200,6 +201,7 @@ extern "C" int syn_jacobi_f64(const syn_jacobi_item *items, int n_items, double 
                             (int)(kJacLdsDoubles * sizeof(double)));
         attr_set = true;
     }
+    ProfScope ps(kTagJacobi, 0.0, (hipStream_t)stream);
     hipLaunchKernelGGL(jacobi_kernel, dim3(n_items), dim3(kJacThreads), kJacLdsDoubles * sizeof(double),
                        (hipStream_t)stream, items, tol, max_sweeps);
     SYN_LAUNCH_CHECK("syn_jacobi_f64 launch");

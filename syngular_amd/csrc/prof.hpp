@@ -1,0 +1,37 @@
+// Event-based per-kernel timing used by syn_prof_begin / syn_prof_end.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace syn {
+
+// kernel tags (indices of the arrays filled by syn_prof_end)
+enum ProfTag {
+    kTagGemmSingle64 = 0,
+    kTagGemmSingle128 = 1,
+    kTagGemmGrouped64 = 2,
+    kTagGemmGrouped128 = 3,
+    kTagBJacobiStep = 4,
+    kTagCholDiag = 5,
+    kTagQr = 6,
+    kTagJacobi = 7,
+    kTagApply = 8,
+};
+
+bool prof_on();
+
+// Records an event before (constructor) and after (destructor) the launch
+// issued in its scope.  flops < 0 takes the value given by the last
+// syn_prof_note_flops call.
+class ProfScope {
+   public:
+    ProfScope(int tag, double flops, hipStream_t s);
+    ~ProfScope();
+
+   private:
+    int tag_;
+    double flops_;
+    hipStream_t s_;
+    hipEvent_t a_;
+};
+
+}  // namespace syn

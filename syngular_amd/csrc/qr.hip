@@ -14,6 +14,7 @@
 // the leading block lives in LDS when it fits and in the caller's workspace
 // (L2-resident at these sizes) otherwise.
 #include "common.hpp"
+#include "prof.hpp"
 
 namespace syn {
 
@@ -159,6 +160,7 @@ extern "C" int syn_qr_f64(const syn_qr_item *items, int n_items, void *stream) {
                             (int)(kQrLdsDoubles * sizeof(double)));
         attr_set = true;
     }
+    ProfScope ps(kTagQr, 0.0, (hipStream_t)stream);
     hipLaunchKernelGGL(qr_kernel, dim3(n_items), dim3(kQrThreads), kQrLdsDoubles * sizeof(double),
                        (hipStream_t)stream, items, n_items);
     SYN_LAUNCH_CHECK("syn_qr_f64 launch");

@@ -4,6 +4,7 @@
 // work-group covers SYN_APPLY_BLOCK consecutive output elements of one item,
 // writes them coalesced, and reads the small operands through L2.
 #include "common.hpp"
+#include "prof.hpp"
 
 namespace syn {
 
@@ -167,6 +168,7 @@ extern "C" int syn_mpo_apply_f64(const syn_apply_item *items, int n_items, int64
     using namespace syn;
     if (int rc = grid_check(n_items, total_blocks, "syn_mpo_apply_f64")) return rc;
     if (n_items == 0 || total_blocks == 0) return SYN_OK;
+    ProfScope ps(kTagApply, -1.0, (hipStream_t)stream);
     hipLaunchKernelGGL(apply_kernel, dim3((unsigned)total_blocks), dim3(kSiteThreads), 0, (hipStream_t)stream,
                        items, n_items, total_blocks);
     SYN_LAUNCH_CHECK("syn_mpo_apply_f64 launch");

@@ -15,6 +15,7 @@ from . import _lib
 from ._lib import call
 
 F64 = torch.float64
+EPS = float(np.finfo(np.float64).eps)
 
 
 def require_gpu():
@@ -101,8 +102,26 @@ def gemm_many(problems, tile=None):
         it["block_start"], it["tiles_n"] = start, tn
         start += tm * tn
     dev = _upload(items)
+    if _profiling[0]:
+        fl = sum(2.0 * p[0].shape[0] * p[0].shape[1] * p[1].shape[1] for p in problems)
+        call("syn_prof_note_flops", fl)
     call("syn_gemm_f64", _ptr(dev), len(problems), start, tile, _stream())
     return dev
+
+
+_profiling = [False]
+
+
+def profile_begin():
+    """Start live per-kernel event timing inside the library."""
+    _profiling[0] = True
+    _lib.prof_begin()
+
+
+def profile_end():
+    """Stop timing; {kernel: (launches, ms, algorithmic flops)}."""
+    _profiling[0] = False
+    return _lib.prof_end()
 
 
 def gemm(A, B, C=None, alpha=1.0, beta=0.0, tile=None):
@@ -142,6 +161,10 @@ def apply_many(pairs):
         start += _blocks(P.numel())
     if start:
         dev = _upload(items)
+        if _profiling[0]:
+            # bytes moved: every output element written once, operands read once
+            byt = sum(8.0 * (o[0].numel() + o[1].numel() + o[2].numel()) for o in outs)
+            call("syn_prof_note_flops", byt)
         call("syn_mpo_apply_f64", _ptr(dev), len(pairs), start, _stream())
     return [o[0] for o in outs]
 
@@ -284,12 +307,14 @@ JACOBI_TOL = 1e-15
 JACOBI_SWEEPS = 60
 
 
-def jacobi_many(mats, tol=JACOBI_TOL, max_sweeps=JACOBI_SWEEPS, gram=False):
+def jacobi_many(mats, tol=None, max_sweeps=JACOBI_SWEEPS, gram=False):
     """mats: list of row-major (nv x len) float64 tensors, orthogonalised in
     place.  Returns per problem (rt (nv x nv), s (nv,), sweeps (1,) int32) with
     x_in = rt^T x_out and rows of x_out sorted by decreasing norm s.  With
     gram=True each input is a Gram matrix C^T C and s holds the singular
     values of C (square roots of the row norms)."""
+    if tol is None:
+        tol = jacobi_tol(max((m.shape[1] for m in mats), default=1))
     items = np.zeros(len(mats), dtype=_lib.JACOBI_ITEM)
     outs = []
     for i, X in enumerate(mats):
@@ -330,3 +355,63 @@ def truncate_many(problems):
         dev = _upload(items)
         call("syn_truncate_f64", _ptr(dev), n, _stream())
     return keep, disc
+
+
+# --------------------------------------------------------------------------
+# large-problem paths (multi-step routines inside the library)
+# --------------------------------------------------------------------------
+def cholqr_tol(m, n):
+    """Pivot-drop threshold relative to max_i ||a_i||^2: the rounding noise of
+    a Gram pivot is a modest multiple of (m + n) eps."""
+    return 8.0 * (m + n) * EPS
+
+
+def cholqr(A, tolrel=None):
+    """One CholeskyQR pass on a row-major m x n matrix (A is destroyed).
+    Returns (Q (m x n), L (n x n lower), dropped (1,) int32) with A = Q L^T."""
+    m, n = A.shape
+    if tolrel is None:
+        tolrel = cholqr_tol(m, n)
+    pa, lda, tr = as_matrix(A)
+    if tr:
+        raise ValueError("cholqr needs a row-major matrix")
+    Q = torch.empty((m, n), dtype=F64, device=A.device)
+    L = torch.empty((n, n), dtype=F64, device=A.device)
+    work = torch.empty(int(_lib.load().syn_cholqr_workspace(n)), dtype=F64, device=A.device)
+    dropped = torch.zeros(1, dtype=torch.int32, device=A.device)
+    call("syn_cholqr_f64", pa, m, n, lda, _ptr(Q), n, _ptr(L), n, _ptr(work), float(tolrel), _ptr(dropped),
+         _stream())
+    return Q, L, dropped
+
+
+def cholqr2(A, tolrel=None):
+    """Reduced QR of a tall row-major matrix by two CholeskyQR passes.
+    A is destroyed.  Returns (Q (m x n), R (n x n upper)) with A = Q R."""
+    Q1, L1, _ = cholqr(A, tolrel)
+    Q, L2, _ = cholqr(Q1, tolrel)
+    R = gemm(L2.t(), L1.t())  # R = R2 R1 = L2^T L1^T
+    return Q, R
+
+
+def jacobi_tol(length):
+    """Orthogonality threshold |x_p . x_q| <= tol ||x_p|| ||x_q||: the
+    rounding error bound of a dot product of `length` terms, length * eps."""
+    return max(JACOBI_TOL, float(length) * EPS)
+
+
+def jacobi_blocked(X, gram=False, tol=None, max_sweeps=JACOBI_SWEEPS):
+    """Block one-sided Jacobi on the rows of a row-major nv x len matrix (in
+    place).  Returns (rt (nv x nv), s (nv,), sweeps) as jacobi_many does."""
+    nv, ln = X.shape
+    if tol is None:
+        tol = jacobi_tol(ln)
+    px, ldx, tr = as_matrix(X)
+    if tr:
+        raise ValueError("jacobi needs a row-major matrix")
+    rt = torch.empty((nv, nv), dtype=F64, device=X.device)
+    s = torch.empty(nv, dtype=F64, device=X.device)
+    work = torch.empty(int(_lib.load().syn_jacobi_blocked_workspace(nv, ln)), dtype=F64, device=X.device)
+    sweeps = ctypes.c_int(0)
+    call("syn_jacobi_blocked_f64", px, nv, ln, ldx, _ptr(rt), nv, _ptr(s), _ptr(work), float(tol),
+         int(max_sweeps), int(gram), ctypes.byref(sweeps), _stream())
+    return rt, s, sweeps.value

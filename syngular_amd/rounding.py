@@ -37,17 +37,29 @@ class RoundingReport:
     discarded: list = field(default_factory=list)
 
 
-def left_orthonormalize(cores):
-    """Left QR sweep; returns new cores (left-orthonormal except the last)."""
-    cores = [c.contiguous() for c in cores]
+# Unfoldings with at least this many columns take the CholeskyQR2 path (all
+# MFMA GEMM work); smaller ones the one-work-group Householder kernel.
+CHOLQR_MIN_COLS = 96
+# Right-sweep Gram matrices up to this size are diagonalised in LDS by one
+# work-group; larger ones by the block Jacobi spread over many work-groups.
+LDS_JACOBI_MAX = 64
+
+
+def left_orthonormalize(cores, own=False):
+    """Left QR sweep; returns new cores (left-orthonormal except the last).
+    With own=False the input cores are not modified."""
+    cores = list(cores)
     for k in range(len(cores) - 1):
-        c = cores[k]
+        c = cores[k].contiguous()
         l, mid, r = c.shape[0], c.shape[1:-1], c.shape[-1]
         L = c.reshape(-1, r)
         m = L.shape[0]
         if m <= r:
             # a wide unfolding: Q = I_m and R = L, no factorisation needed
             Q, S = torch.eye(m, dtype=F64, device=L.device), L
+        elif r >= CHOLQR_MIN_COLS:
+            work = L if (own or k > 0) else L.clone()
+            Q, S = ops.cholqr2(work)
         else:
             Q, S = _qr(L)
         nxt = cores[k + 1]
@@ -77,7 +89,10 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None):
             G = ops.gemm(C.t(), C)       # (ncol x ncol) = C^T C
         else:
             G = ops.gemm(C, C.t())       # (m x m) = C C^T
-        rt, s, _ = ops.jacobi_many([G], gram=True)[0]
+        if G.shape[0] <= LDS_JACOBI_MAX:
+            rt, s, _ = ops.jacobi_many([G], gram=True)[0]
+        else:
+            rt, s, _ = ops.jacobi_blocked(G, gram=True)
         keep, disc = ops.truncate_many([(s, chi_max, cutoff)])
         cc = int(keep.item())
         prev = cores[k - 1]
@@ -101,12 +116,13 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None):
     return cores
 
 
-def tt_round(cores, chi_max=None, cutoff=0.0, report=None):
-    """TT-rounding of an MPS given as device cores (l, d, r)."""
-    return right_svd_sweep(left_orthonormalize(cores), chi_max, cutoff, report)
+def tt_round(cores, chi_max=None, cutoff=0.0, report=None, own=False):
+    """TT-rounding of an MPS given as device cores (l, d, r).  The input
+    cores are left untouched unless own=True."""
+    return right_svd_sweep(left_orthonormalize(cores, own=own), chi_max, cutoff, report)
 
 
 def apply_and_round(mpo_sites, mps_sites, chi_max=None, cutoff=0.0, report=None):
     """One apply+round step: materialised MPO x MPS cores, then tt_round."""
     prod = ops.apply_many(list(zip(mpo_sites, mps_sites)))
-    return tt_round(prod, chi_max, cutoff, report)
+    return tt_round(prod, chi_max, cutoff, report, own=True)

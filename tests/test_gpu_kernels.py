@@ -183,3 +183,59 @@ def test_truncate_matches_oracle_rule():
         c = ornd.keep_count(s, chi, cut)
         assert keep[i] == c, (len(s), chi, cut, keep[i], c)
         np.testing.assert_allclose(disc[i], np.sum(s[c:] ** 2), rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("m,n", [(40, 40), (200, 64), (700, 300), (1100, 513)])
+def test_cholqr2(m, n):
+    from syngular_amd import ops
+
+    case(f"cholqr2 m={m} n={n}")
+    rng = np.random.default_rng(m + n)
+    A0 = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, -3, n))
+    Q, R = ops.cholqr2(T(A0))
+    q, r = Q.cpu().numpy(), R.cpu().numpy()
+    np.testing.assert_allclose(q.T @ q, np.eye(n), atol=1e-13)
+    np.testing.assert_allclose(q @ r, A0, atol=1e-13 * np.linalg.norm(A0, 2))
+    assert np.allclose(np.tril(r, -1), 0.0)
+    # R agrees with LAPACK's up to the signs of its rows
+    _, rn = np.linalg.qr(A0)
+    np.testing.assert_allclose(np.abs(r), np.abs(rn), atol=1e-11 * np.abs(rn).max())
+
+
+def test_cholqr_rank_deficient_drops():
+    from syngular_amd import ops
+
+    case("cholqr rank deficient")
+    rng = np.random.default_rng(21)
+    B = rng.standard_normal((300, 40))
+    A0 = np.concatenate([B, B[:, :10] @ rng.standard_normal((10, 24))], axis=1)  # rank 40 of 64
+    Q, L, dropped = ops.cholqr(T(A0))
+    assert int(dropped.item()) == 24
+    Q2, L2, d2 = ops.cholqr(Q.clone())
+    q = Q2.cpu().numpy()
+    r = (L.cpu().numpy() @ L2.cpu().numpy()).T
+    np.testing.assert_allclose(q @ r, A0, atol=1e-10 * np.abs(A0).max())
+    kept = np.linalg.norm(q, axis=0) > 0.5
+    assert kept.sum() == 40
+    np.testing.assert_allclose(q[:, kept].T @ q[:, kept], np.eye(40), atol=1e-12)
+
+
+@pytest.mark.parametrize("nv,ln,gram", [(17, 40, False), (100, 130, False), (300, 300, True), (512, 512, True)])
+def test_jacobi_blocked(nv, ln, gram):
+    from syngular_amd import ops
+
+    case(f"jacobi blocked nv={nv} len={ln} gram={gram}")
+    rng = np.random.default_rng(nv + ln)
+    if gram:
+        C = rng.standard_normal((2 * nv, nv)) @ np.diag(np.logspace(0, -2, nv))
+        X0 = C.T @ C
+        s_ref = np.linalg.svd(C, compute_uv=False)
+    else:
+        X0 = rng.standard_normal((nv, ln))
+        s_ref = np.linalg.svd(X0, compute_uv=False)
+    rt, s, sweeps = ops.jacobi_blocked(T(X0), gram=gram)
+    k = min(nv, ln)
+    np.testing.assert_allclose(s.cpu().numpy()[:k], s_ref, rtol=1e-11 if not gram else 1e-9)
+    R = rt.cpu().numpy()
+    np.testing.assert_allclose(R @ R.T, np.eye(nv), atol=1e-12)
+    assert 0 < sweeps < 60
