@@ -60,6 +60,10 @@ int syn_struct_sizes(int64_t *out, int n);
 /* block_start counts tiles of the tile configuration `tile` (0: 64x64,  */
 /* 1: 128x128); tiles_n = ceil(n / tile width).                          */
 /* ------------------------------------------------------------------ */
+/* tri: structural-zero hints (bit mask), letting a tile skip work:       */
+/*   1  only the lower triangle of C is needed (tiles above it are skipped)*/
+/*   2  op(A) is lower triangular    4  op(A) is upper triangular          */
+/*   8  op(B) is upper triangular   16  op(B) is lower triangular          */
 typedef struct {
     const void *A;
     const void *B;
@@ -70,6 +74,7 @@ typedef struct {
     int32_t transA, transB;
     int64_t block_start;
     int64_t tiles_n;
+    int32_t tri;
 } syn_gemm_item;
 
 /* f64 x f64 -> f64 on v_mfma_f64_16x16x4 */
@@ -199,7 +204,7 @@ int syn_jacobi_blocked_f64(double *x, int64_t nv, int64_t len, int64_t ldx, doub
 /* written in full (zeros above the diagonal).  A pivot below            */
 /* tolrel * max_i G_ii is dropped (zero column of L); *dropped (device   */
 /* int) counts them.  work: syn_cholqr_workspace(n) doubles suffice.     */
-/* syn_cholqr_f64: one CholeskyQR pass on the m x n matrix A (destroyed): */
+/* syn_cholqr_f64: one CholeskyQR pass on the m x n matrix A (kept):     */
 /* G = A^T A, G = L L^T, Q = A L^{-T} (Q: m x n, ldq).  Two passes give   */
 /* the reduced QR A = Q (L1 L2)^T with Q orthonormal to working          */
 /* precision.  Replaces numpy.linalg.qr on large left unfoldings          */
@@ -259,6 +264,8 @@ int syn_prof_end(int64_t *launches, double *ms, double *flops, int ntags);
 /* Tuning knob of syn_jacobi_blocked_f64: rows per Jacobi block (8 or 16;  */
 /* anything else restores the size-based default).                       */
 void syn_tune_jacobi_block_rows(int rows);
+/* Inner Jacobi sweeps per block pair (1..8; anything else restores 1).    */
+void syn_tune_jacobi_inner_sweeps(int sweeps);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ _I = np.int64
 GEMM_ITEM = np.dtype(
     [("A", _P), ("B", _P), ("C", _P), ("m", _I), ("n", _I), ("k", _I), ("lda", _I), ("ldb", _I), ("ldc", _I),
      ("alpha", np.float64), ("beta", np.float64), ("transA", np.int32), ("transB", np.int32),
-     ("block_start", _I), ("tiles_n", _I)], align=True)
+     ("block_start", _I), ("tiles_n", _I), ("tri", np.int32)], align=True)
 APPLY_ITEM = np.dtype(
     [("x", _P), ("o", _P), ("p", _P), ("chil", _I), ("din", _I), ("chir", _I), ("dl", _I), ("dout", _I),
      ("dr", _I), ("block_start", _I)], align=True)
@@ -80,6 +80,7 @@ SIGNATURES = {
     "syn_prof_end": (_I32, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_double), _I32]),
     "syn_tune_jacobi_block_rows": (None, [_I32]),
+    "syn_tune_jacobi_inner_sweeps": (None, [_I32]),
 }
 
 PROF_NTAGS = 16

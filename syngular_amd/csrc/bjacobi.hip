@@ -20,7 +20,6 @@
 
 namespace syn {
 
-constexpr int kInner = 3;     // max inner Jacobi sweeps on S (the outer sweep repeats)
 constexpr int kChunk = 128;   // columns of the pair staged in LDS at a time
 
 __device__ __forceinline__ void rr_pair(int64_t step, int64_t w, int64_t np, int64_t &a, int64_t &b) {
@@ -65,7 +64,7 @@ __device__ __forceinline__ void stage_chunk(double (*Ys)[kChunk + 4], const int 
 template <int KB, int NT>
 __global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv, int64_t len, int64_t ldx, double *R,
                                                           int64_t ldr, int64_t nblk, int64_t step, double tol,
-                                                          int *counter) {
+                                                          int *counter, int inner_sweeps) {
     constexpr int P = 2 * KB;        // rows of a pair
     constexpr int RT = P / 16;       // 16-row MFMA tiles
     constexpr int NWV = NT / 64;     // waves
@@ -134,8 +133,11 @@ __global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv,
     __syncthreads();
     if (!do_rot) return;
 
-    // cyclic two-sided Jacobi on S, accumulating W (S_final = W^T S W)
-    for (int sweep = 0; sweep < kInner; ++sweep) {
+    // cyclic two-sided Jacobi on S, accumulating W (S_final = W^T S W).
+    // One round rotates P/2 disjoint index pairs; thread (pa, pb) owns the
+    // 2x2 block of S at the rows of pair pa and the columns of pair pb and
+    // applies both rotations to it at once, so a round needs two barriers.
+    for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
         int any = 0;
         for (int r = 0; r < P - 1; ++r) {
             if (tid < P / 2) {
@@ -152,27 +154,28 @@ __global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv,
                 cs[tid][1] = s;
             }
             __syncthreads();
-            for (int e = tid; e < (P / 2) * P; e += NT) {  // rows p, q of S
-                const int pr = e / P, k = e % P;
-                const int p = pq[r][pr][0], q = pq[r][pr][1];
-                const double c = cs[pr][0], s = cs[pr][1];
-                if (s != 0.0) {
-                    const double u = S[p][k], v = S[q][k];
-                    S[p][k] = c * u - s * v;
-                    S[q][k] = s * u + c * v;
+            for (int e = tid; e < (P / 2) * (P / 2); e += NT) {
+                const int pa = e / (P / 2), pb = e % (P / 2);
+                const double ca = cs[pa][0], sa = cs[pa][1], cb = cs[pb][0], sb = cs[pb][1];
+                if (sa != 0.0 || sb != 0.0) {
+                    const int p = pq[r][pa][0], q = pq[r][pa][1], u = pq[r][pb][0], v = pq[r][pb][1];
+                    const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
+                    // rows: [p; q] <- J_a^T [p; q]
+                    const double rpu = ca * spu - sa * squ, rpv = ca * spv - sa * sqv;
+                    const double rqu = sa * spu + ca * squ, rqv = sa * spv + ca * sqv;
+                    // columns: [u, v] <- [u, v] J_b
+                    S[p][u] = cb * rpu - sb * rpv;
+                    S[p][v] = sb * rpu + cb * rpv;
+                    S[q][u] = cb * rqu - sb * rqv;
+                    S[q][v] = sb * rqu + cb * rqv;
                 }
             }
-            __syncthreads();
-            for (int e = tid; e < (P / 2) * P; e += NT) {  // columns p, q of S and W
+            for (int e = tid; e < (P / 2) * P; e += NT) {  // columns p, q of W
                 const int pr = e / P, i = e % P;
-                const int p = pq[r][pr][0], q = pq[r][pr][1];
                 const double c = cs[pr][0], s = cs[pr][1];
                 if (s != 0.0) {
-                    double u = S[i][p], v = S[i][q];
-                    S[i][p] = c * u - s * v;
-                    S[i][q] = s * u + c * v;
-                    u = W[i][p];
-                    v = W[i][q];
+                    const int p = pq[r][pr][0], q = pq[r][pr][1];
+                    const double u = W[i][p], v = W[i][q];
                     W[i][p] = c * u - s * v;
                     W[i][q] = s * u + c * v;
                 }
@@ -266,8 +269,11 @@ extern "C" int64_t syn_jacobi_blocked_workspace(int64_t nv, int64_t len) {
 
 // Rows per block: 16 (four waves per pair) from 384 rows up, else 8 (one wave).
 static int g_block_rows_override = 0;
+// Inner Jacobi sweeps on the 2KB x 2KB pair Gram (the outer sweep repeats).
+static int g_inner_sweeps = 1;
 
 extern "C" void syn_tune_jacobi_block_rows(int rows) { g_block_rows_override = (rows == 8 || rows == 16) ? rows : 0; }
+extern "C" void syn_tune_jacobi_inner_sweeps(int sweeps) { g_inner_sweeps = (sweeps >= 1 && sweeps <= 8) ? sweeps : 1; }
 
 extern "C" int syn_jacobi_blocked_f64(double *X, int64_t nv, int64_t len, int64_t ldx, double *rt, int64_t ldr,
                                       double *s, double *work, double tol, int max_sweeps, int gram, int *sweeps,
@@ -300,10 +306,10 @@ extern "C" int syn_jacobi_blocked_f64(double *X, int64_t nv, int64_t len, int64_
                          (double)(np / 2) * 2.0 * (2.0 * kb) * (2.0 * kb) * (double)(2 * len + nv), st);
             if (kb == 16)
                 hipLaunchKernelGGL((bjacobi_step_kernel<16, 256>), dim3((unsigned)(np / 2)), dim3(256), 0, st, X, nv,
-                                   len, ldx, rt, ldr, nblk, step, tol, counter);
+                                   len, ldx, rt, ldr, nblk, step, tol, counter, g_inner_sweeps);
             else
                 hipLaunchKernelGGL((bjacobi_step_kernel<8, 64>), dim3((unsigned)(np / 2)), dim3(64), 0, st, X, nv,
-                                   len, ldx, rt, ldr, nblk, step, tol, counter);
+                                   len, ldx, rt, ldr, nblk, step, tol, counter, g_inner_sweeps);
         }
         SYN_LAUNCH_CHECK("bjacobi_step");
         int host_count = 0;

@@ -1,8 +1,8 @@
 // Blocked Cholesky and CholeskyQR for the left-orthonormalisation sweep.
 //
 // syn_cholqr_f64 performs one CholeskyQR pass on a tall m x n matrix A:
-//   G = A^T A (MFMA GEMM), G = L L^T (blocked right-looking Cholesky),
-//   Q = A L^{-T} (blocked right-looking triangular solve).
+//   G = A^T A (MFMA GEMM), G = L L^T together with X = L^{-1} (recursive,
+//   GEMM-rich), Q = A X^T (one MFMA GEMM).
 // Two passes (CholeskyQR2, Yamamoto et al., ETNA 44, 2015) give Q orthonormal
 // to working precision whenever cond(A) < ~1e8; the sweeps of the workloads
 // here stay below 1e5.
@@ -16,7 +16,7 @@
 //
 // Every step is a kernel launched from the host with its arguments by value;
 // the 64 x 64 diagonal blocks are factored and inverted in LDS by one
-// work-group, the panel solves and trailing updates run on the GEMM.
+// work-group, everything above them runs on the GEMM.
 #include "common.hpp"
 #include "prof.hpp"
 
@@ -52,10 +52,10 @@ __global__ __launch_bounds__(256) void maxdiag_kernel(const double *g, int64_t n
 // replaced by the unit row, so that a panel solve X L11^T = B over the kept
 // columns is X = B Z^T with zero columns at the dropped positions.
 __global__ __launch_bounds__(256) void chol_diag_kernel(const double *G, int64_t ldg, double *L, int64_t ldl,
-                                                        int64_t n, int64_t j0, double *zb, const double *maxdiag,
-                                                        double tolrel, int *dropped) {
-    __shared__ double a[kNb][kNb + 1];
-    __shared__ double z[kNb][kNb + 1];
+                                                        int64_t n, int64_t j0, double *zb, int64_t ldz,
+                                                        const double *maxdiag, double tolrel, int *dropped) {
+    __shared__ double a[kNb][kNb + 1];   // Schur complements (lower triangle)
+    __shared__ double lo[kNb][kNb + 1];  // L, then L~ (dropped rows -> unit rows)
     __shared__ int keep[kNb];
     const int nb = (int)((n - j0) < kNb ? (n - j0) : kNb);
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
@@ -71,63 +71,112 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double *G, int64_t
         for (int q = 0; q < kNb * kNb / 256; ++q) {
             const int e = tid + 256 * q, i = e >> 6, k = e & 63;
             a[i][k] = v[q];
-            z[i][k] = 0.0;
+            lo[i][k] = 0.0;
         }
     }
     __syncthreads();
+    // right-looking with the Schur complement in registers: thread (ty, tx)
+    // owns rows ty+16u and columns tx+16v (u, v < 4).  Column j is published
+    // through a double-buffered LDS column by its owners, so one step costs
+    // eight broadcast reads, sixteen register FMAs and one barrier.
+    __shared__ double col[2][kNb];
+    double r[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) r[u][v] = a[ty + 16 * u][tx + 16 * v];
+    if (tid < kNb) col[0][tid] = a[tid][0];
+    __syncthreads();
     int ndrop = 0;
     for (int j = 0; j < nb; ++j) {
-        const double d = a[j][j];  // every thread decides the same way
+        const double *cj = col[j & 1];
+        const double d = cj[j];
         const bool kp = d > thr;
-        const double inv = kp ? 1.0 / sqrt(d) : 0.0;
         ndrop += !kp;
-        // column j below the diagonal (rows j+1..nb-1) and the pivot itself
-        if (tid < nb && tid > j) a[tid][j] *= inv;
-        __syncthreads();
-        if (tid == 0) {
-            keep[j] = kp;
-            a[j][j] = kp ? sqrt(d) : 0.0;
+        const double sd = kp ? sqrt(d) : 0.0, rd = kp ? 1.0 / d : 0.0;
+        if (tid >= j && tid < nb) lo[tid][j] = (tid == j) ? sd : (kp ? cj[tid] / sd : 0.0);
+        if (tid == 0) keep[j] = kp;
+        double ci[4], ck[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ci[u] = cj[ty + 16 * u] * rd;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ck[v] = cj[tx + 16 * v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = ty + 16 * u;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int k = tx + 16 * v;
+                if (k > j && k <= i) r[u][v] = fma(-ci[u], ck[v], r[u][v]);
+            }
         }
-        // trailing lower triangle: a[i][k] -= a[i][j] a[k][j], j < k <= i,
-        // thread (ty, tx) owns rows j+1+ty+16u and columns j+1+tx+16v
-        for (int i = j + 1 + ty; i < nb; i += 16) {
-            const double aij = a[i][j];
-            for (int k = j + 1 + tx; k <= i; k += 16) a[i][k] = fma(-aij, a[k][j], a[i][k]);
+        // publish column j+1 (owners: tx == (j+1) % 16)
+        const int jn = j + 1;
+        if (jn < nb && tx == (jn & 15)) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (tx + 16 * v == jn) col[jn & 1][ty + 16 * u] = r[u][v];
         }
         __syncthreads();
     }
     if (tid == 0 && ndrop) atomicAdd(dropped, ndrop);
-    // L~ = L with every dropped row replaced by the unit row (in place)
+    // write L, then turn dropped rows of lo into unit rows (L~)
     for (int e = tid; e < nb * nb; e += 256) {
         const int i = e / nb, k = e % nb;
-        if (!keep[i] && k <= i) {
-            z[i][k] = a[i][k];  // keep L's row for the write-out below
-            a[i][k] = (i == k) ? 1.0 : 0.0;
+        L[(j0 + i) * ldl + (j0 + k)] = (k <= i) ? lo[i][k] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < nb * nb; e += 256) {
+        const int i = e / nb, k = e % nb;
+        if (!keep[i] && k <= i) lo[i][k] = (i == k) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // Z = inv(L~) by right-looking forward substitution in registers: thread
+    // (ty, tx) holds Z_acc at rows ty+16u, columns tx+16v; row k of Z is final
+    // once updated by rows < k, its owners divide by L~_kk, publish it through
+    // a double-buffered LDS row and write it out.
+    __shared__ double zrow[2][kNb];
+    double zr[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) zr[u][v] = (ty + 16 * u == tx + 16 * v) ? 1.0 : 0.0;
+    auto publish = [&](int k) {
+        if (k < nb && ty == (k & 15)) {
+            const double inv = 1.0 / lo[k][k];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (ty + 16 * u == k) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int c = tx + 16 * v;
+                        const double zv = zr[u][v] * inv;
+                        zr[u][v] = zv;
+                        zrow[k & 1][c] = zv;
+                        if (c < nb) zb[k * ldz + c] = (keep[c] && c <= k) ? zv : 0.0;
+                    }
+                }
         }
-    }
+    };
+    publish(0);
     __syncthreads();
-    // Z = inv(L~) by rows: z_i,c = -(sum_{c<=k<i} L~_ik z_kc) / L~_ii
-    __shared__ double zz[kNb][kNb + 1];
-    for (int e = tid; e < kNb * kNb; e += 256) zz[e >> 6][e & 63] = 0.0;
-    __syncthreads();
-    if (tid < nb) zz[tid][tid] = 1.0 / a[tid][tid];
-    __syncthreads();
-    for (int i = 1; i < nb; ++i) {
-        // columns c < i in parallel; four threads per column split the k range
-        const int c = tid >> 2, part = tid & 3;
-        double acc = 0.0;
-        if (c < i)
-            for (int k = c + part; k < i; k += 4) acc = fma(a[i][k], zz[k][c], acc);
-        acc += __shfl_xor(acc, 1, 64);
-        acc += __shfl_xor(acc, 2, 64);
-        if (c < i && part == 0) zz[i][c] = -acc / a[i][i];
+    for (int k = 0; k + 1 < nb; ++k) {
+        const double *zk = zrow[k & 1];
+        double lk[4], zc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) lk[u] = lo[ty + 16 * u][k];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) zc[v] = zk[tx + 16 * v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (ty + 16 * u > k) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) zr[u][v] = fma(-lk[u], zc[v], zr[u][v]);
+            }
+        publish(k + 1);
         __syncthreads();
-    }
-    for (int e = tid; e < nb * nb; e += 256) {
-        const int i = e / nb, k = e % nb;
-        const double lv = keep[i] ? a[i][k] : z[i][k];
-        L[(j0 + i) * ldl + (j0 + k)] = (k <= i) ? lv : 0.0;
-        zb[i * kNb + k] = keep[k] ? zz[i][k] : 0.0;
     }
 }
 
@@ -155,8 +204,8 @@ static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, doub
         const int64_t j1 = j0 + nb;
         double *zb = Z + b * kNb * kNb;
         ProfScope ps(kTagCholDiag, (double)nb * nb * nb * (1.0 / 3.0 + 1.0 / 3.0), s);
-        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, G, ldg, L, ldl, n, j0, zb, scal, tolrel,
-                           dropped);
+        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, G, ldg, L, ldl, n, j0, zb, (int64_t)kNb,
+                           scal, tolrel, dropped);
         SYN_LAUNCH_CHECK("chol_diag");
         if (j1 >= n) break;
         // L21 = G21 Z^T
@@ -173,27 +222,48 @@ static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, doub
     return SYN_OK;
 }
 
-// Q = A L^{-T} by block columns (A destroyed): Q_J = A_J Z_J^T,
-// A_{J+1:} -= Q_J L_{J+1:,J}^T.
-static int trsm_right_lt(double *A, int64_t lda, const double *L, int64_t ldl, const double *Z, double *Q,
-                         int64_t ldq, int64_t m, int64_t n, hipStream_t s) {
-    for (int64_t j0 = 0, b = 0; j0 < n; j0 += kNb, ++b) {
-        const int64_t nb = (n - j0) < kNb ? (n - j0) : kNb;
-        const int64_t j1 = j0 + nb;
-        int rc = gemm_f64_launch(make_gemm(A + j0, lda, 0, Z + b * kNb * kNb, kNb, 1, Q + j0, ldq, m, nb, nb, 1.0, 0.0), s);
-        if (rc) return rc;
-        if (j1 >= n) break;
-        rc = gemm_f64_launch(make_gemm(Q + j0, ldq, 0, L + j1 * ldl + j0, ldl, 1, A + j1, lda, m, n - j1, nb, -1.0, 1.0), s);
-        if (rc) return rc;
+// Recursive Cholesky with inverse: G = L L^T and X = inv(L~) with the
+// dropped columns zeroed (the convention of chol_diag_kernel), by
+//   (L11, X11) = chol_inv(G11);  L21 = G21 X11^T;  G22 -= L21 L21^T;
+//   (L22, X22) = chol_inv(G22);  X21 = -X22 (L21 X11).
+// Every step above the 64 x 64 leaves is a GEMM.  L and X must be zero above
+// the diagonal on entry; T is scratch of (n/2)^2 doubles.
+static int chol_inv(double *G, int64_t ldg, double *L, int64_t ldl, double *X, int64_t ldx, int64_t n, double *T,
+                    const double *scal, int *dropped, double tolrel, hipStream_t s) {
+    if (n <= kNb) {
+        ProfScope ps(kTagCholDiag, (double)n * n * n * (2.0 / 3.0), s);
+        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, G, ldg, L, ldl, n, (int64_t)0, X, ldx, scal,
+                           tolrel, dropped);
+        SYN_LAUNCH_CHECK("chol_diag");
+        return SYN_OK;
     }
-    return SYN_OK;
+    const int64_t n1 = ((n / kNb + 1) / 2) * kNb, n2 = n - n1;
+    int rc = chol_inv(G, ldg, L, ldl, X, ldx, n1, T, scal, dropped, tolrel, s);
+    if (rc) return rc;
+    double *L21 = L + n1 * ldl;
+    // L21 = G21 X11^T (op(B) = X11^T upper)
+    rc = gemm_f64_launch(make_gemm(G + n1 * ldg, ldg, 0, X, ldx, 1, L21, ldl, n2, n1, n1, 1.0, 0.0, 8), s);
+    if (rc) return rc;
+    // G22 -= L21 L21^T, lower triangle only
+    rc = gemm_f64_launch(make_gemm(L21, ldl, 0, L21, ldl, 1, G + n1 * ldg + n1, ldg, n2, n2, n1, -1.0, 1.0, 1), s);
+    if (rc) return rc;
+    rc = chol_inv(G + n1 * ldg + n1, ldg, L + n1 * ldl + n1, ldl, X + n1 * ldx + n1, ldx, n2, T, scal, dropped,
+                  tolrel, s);
+    if (rc) return rc;
+    // T = L21 X11 (op(B) = X11 lower);  X21 = -X22 T (op(A) = X22 lower)
+    rc = gemm_f64_launch(make_gemm(L21, ldl, 0, X, ldx, 0, T, n1, n2, n1, n1, 1.0, 0.0, 16), s);
+    if (rc) return rc;
+    return gemm_f64_launch(
+        make_gemm(X + n1 * ldx + n1, ldx, 0, T, n1, 0, X + n1 * ldx, ldx, n2, n1, n2, -1.0, 0.0, 2), s);
 }
 
 }  // namespace syn
 
 extern "C" int64_t syn_cholqr_workspace(int64_t n) {
+    // G (n x n), X = inv(L) (n x n), the recursion scratch ((n/2)^2 + n) and scalars
     const int64_t nblk = (n + syn::kNb - 1) / syn::kNb;
-    return n * n + nblk * syn::kNb * syn::kNb + 8;
+    const int64_t half = ((nblk + 1) / 2) * syn::kNb;
+    return 2 * n * n + half * half + nblk * syn::kNb * syn::kNb + 16;
 }
 
 extern "C" int syn_potrf_f64(double *G, int64_t n, int64_t ldg, double *L, int64_t ldl, double *work, double tolrel,
@@ -219,12 +289,19 @@ extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, doub
     }
     hipStream_t s = (hipStream_t)stream;
     const int64_t nblk = (n + kNb - 1) / kNb;
+    const int64_t half = ((nblk + 1) / 2) * kNb;
     double *G = work;
-    double *Z = work + n * n;
-    double *scal = Z + nblk * kNb * kNb;
-    int rc = gemm_f64_launch(make_gemm(A, lda, 1, A, lda, 0, G, n, n, n, m, 1.0, 0.0), s);
+    double *X = G + n * n;
+    double *T = X + n * n;
+    double *scal = T + half * half + nblk * kNb * kNb;
+    // G = A^T A, lower triangle only (the Cholesky reads nothing else)
+    int rc = gemm_f64_launch(make_gemm(A, lda, 1, A, lda, 0, G, n, n, n, m, 1.0, 0.0, 1), s);
     if (rc) return rc;
-    rc = potrf(G, n, L, ldl, n, Z, scal, dropped, tolrel, s);
-    if (rc) return rc;
-    return trsm_right_lt(A, lda, L, ldl, Z, Q, ldq, m, n, s);
+    if ((rc = fill(L, n, n, ldl, 0.0, s))) return rc;
+    if ((rc = fill(X, n, n, n, 0.0, s))) return rc;
+    hipLaunchKernelGGL(maxdiag_kernel, dim3(1), dim3(256), 0, s, G, n, n, scal, dropped);
+    SYN_LAUNCH_CHECK("maxdiag");
+    if ((rc = chol_inv(G, n, L, ldl, X, n, n, T, scal, dropped, tolrel, s))) return rc;
+    // Q = A X^T  (X = L^{-1} with dropped columns zeroed; op(B) = X^T upper)
+    return gemm_f64_launch(make_gemm(A, lda, 0, X, n, 1, Q, ldq, m, n, n, 1.0, 0.0, 8), s);
 }

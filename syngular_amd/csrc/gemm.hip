@@ -31,7 +31,16 @@ __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t)
     __shared__ double Bs[2][BK][BN + 1];
 
     const int64_t row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
-    const int64_t m = P.m, n = P.n, k = P.k, lda = P.lda, ldb = P.ldb;
+    const int tri = P.tri;
+    if ((tri & 1) && col0 >= row0 + BM) return;  // tile strictly above the diagonal of C
+    const int64_t m = P.m, n = P.n, lda = P.lda, ldb = P.ldb;
+    // K band of the structurally non-zero products for this tile
+    int64_t kmin = 0, k = P.k;
+    if (tri & 2) k = k < row0 + BM ? k : row0 + BM;
+    if (tri & 8) k = k < col0 + BN ? k : col0 + BN;
+    if (tri & 4) kmin = row0;
+    if (tri & 16) kmin = kmin > col0 ? kmin : col0;
+    kmin = (kmin / BK) * BK;
     const int ta = P.transA, tb = P.transB;
     const double *__restrict__ A = (const double *)P.A;
     const double *__restrict__ B = (const double *)P.B;
@@ -81,14 +90,14 @@ __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t)
 #pragma unroll
         for (int j = 0; j < TJ; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    const int64_t nk = (k + BK - 1) / BK;
-    load(0);
+    const int64_t nk = k > kmin ? (k - kmin + BK - 1) / BK : 0;
+    load(kmin);
     store(0);
     __syncthreads();
     const int li = lane & 15, lk = lane >> 4;
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
-        if (kt + 1 < nk) load((kt + 1) * BK);
+        if (kt + 1 < nk) load(kmin + (kt + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
             double a[TI], b[TJ];
@@ -149,7 +158,13 @@ int gemm_f64_launch(syn_gemm_item P, hipStream_t s) {
     P.tiles_n = (P.n + bm - 1) / bm;
     const int64_t tiles = ((P.m + bm - 1) / bm) * P.tiles_n;
     P.block_start = 0;
-    ProfScope ps(tile ? kTagGemmSingle128 : kTagGemmSingle64, 2.0 * (double)P.m * (double)P.n * (double)P.k, s);
+    // algorithmic flops: a structural-zero hint halves the product (a third
+    // for triangle x triangle)
+    double fl = 2.0 * (double)P.m * (double)P.n * (double)P.k;
+    const int nbits = __builtin_popcount((unsigned)P.tri);
+    if (nbits == 1) fl *= 0.5;
+    if (nbits >= 2) fl /= 3.0;
+    ProfScope ps(tile ? kTagGemmSingle128 : kTagGemmSingle64, fl, s);
     if (tile)
         hipLaunchKernelGGL((gemm_f64_single<128, 128, 16, 64, 64>), dim3((unsigned)tiles), dim3(256), 0, s, P);
     else
@@ -159,7 +174,7 @@ int gemm_f64_launch(syn_gemm_item P, hipStream_t s) {
 }
 
 syn_gemm_item make_gemm(const double *A, int64_t lda, int ta, const double *B, int64_t ldb, int tb, double *C,
-                        int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta) {
+                        int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta, int tri) {
     syn_gemm_item P;
     memset(&P, 0, sizeof(P));
     P.A = A;
@@ -175,6 +190,7 @@ syn_gemm_item make_gemm(const double *A, int64_t lda, int ta, const double *B, i
     P.beta = beta;
     P.transA = ta;
     P.transB = tb;
+    P.tri = tri;
     return P;
 }
 

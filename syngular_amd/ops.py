@@ -70,9 +70,10 @@ _TILES = {0: 64, 1: 128}
 
 
 def gemm_many(problems, tile=None):
-    """problems: iterable of (A, B, C, alpha, beta) with A (m x k), B (k x n),
-    C (m x n) 2-D float64 views; C is overwritten with alpha*A@B + beta*C."""
-    problems = [p for p in problems if p[2].numel() > 0]
+    """problems: iterable of (A, B, C, alpha, beta[, tri]) with A (m x k),
+    B (k x n), C (m x n) 2-D float64 views; C is overwritten with
+    alpha*A@B + beta*C.  tri is the structural-zero mask of syn_gemm_item."""
+    problems = [p if len(p) == 6 else (*p, 0) for p in problems if p[2].numel() > 0]
     if not problems:
         return
     if tile is None:
@@ -81,7 +82,7 @@ def gemm_many(problems, tile=None):
     bm = _TILES[tile]
     items = np.zeros(len(problems), dtype=_lib.GEMM_ITEM)
     start = 0
-    for i, (A, B, C, alpha, beta) in enumerate(problems):
+    for i, (A, B, C, alpha, beta, tri) in enumerate(problems):
         m, k = A.shape
         k2, n = B.shape
         if k != k2 or C.shape != (m, n):
@@ -100,10 +101,13 @@ def gemm_many(problems, tile=None):
         it["alpha"], it["beta"] = alpha, beta
         it["transA"], it["transB"] = ta, tb
         it["block_start"], it["tiles_n"] = start, tn
+        it["tri"] = tri
         start += tm * tn
     dev = _upload(items)
     if _profiling[0]:
-        fl = sum(2.0 * p[0].shape[0] * p[0].shape[1] * p[1].shape[1] for p in problems)
+        frac = {0: 1.0, 1: 0.5}
+        fl = sum(2.0 * p[0].shape[0] * p[0].shape[1] * p[1].shape[1] * frac.get(bin(p[5]).count("1"), 1.0 / 3.0)
+                 for p in problems)
         call("syn_prof_note_flops", fl)
     call("syn_gemm_f64", _ptr(dev), len(problems), start, tile, _stream())
     return dev
@@ -124,11 +128,11 @@ def profile_end():
     return _lib.prof_end()
 
 
-def gemm(A, B, C=None, alpha=1.0, beta=0.0, tile=None):
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, tile=None, tri=0):
     if C is None:
         C = torch.empty((A.shape[0], B.shape[1]), dtype=F64, device=A.device)
         beta = 0.0
-    gemm_many([(A, B, C, alpha, beta)], tile=tile)
+    gemm_many([(A, B, C, alpha, beta, tri)], tile=tile)
     return C
 
 
@@ -367,7 +371,7 @@ def cholqr_tol(m, n):
 
 
 def cholqr(A, tolrel=None):
-    """One CholeskyQR pass on a row-major m x n matrix (A is destroyed).
+    """One CholeskyQR pass on a row-major m x n matrix (A is not modified).
     Returns (Q (m x n), L (n x n lower), dropped (1,) int32) with A = Q L^T."""
     m, n = A.shape
     if tolrel is None:
@@ -386,10 +390,10 @@ def cholqr(A, tolrel=None):
 
 def cholqr2(A, tolrel=None):
     """Reduced QR of a tall row-major matrix by two CholeskyQR passes.
-    A is destroyed.  Returns (Q (m x n), R (n x n upper)) with A = Q R."""
+    Returns (Q (m x n), R (n x n upper)) with A = Q R."""
     Q1, L1, _ = cholqr(A, tolrel)
     Q, L2, _ = cholqr(Q1, tolrel)
-    R = gemm(L2.t(), L1.t())  # R = R2 R1 = L2^T L1^T
+    R = gemm(L2.t(), L1.t(), tri=4 | 8)  # R = R2 R1 = L2^T L1^T, both upper
     return Q, R
 
 

@@ -58,8 +58,7 @@ def left_orthonormalize(cores, own=False):
             # a wide unfolding: Q = I_m and R = L, no factorisation needed
             Q, S = torch.eye(m, dtype=F64, device=L.device), L
         elif r >= CHOLQR_MIN_COLS:
-            work = L if (own or k > 0) else L.clone()
-            Q, S = ops.cholqr2(work)
+            Q, S = ops.cholqr2(L)
         else:
             Q, S = _qr(L)
         nxt = cores[k + 1]

@@ -239,3 +239,28 @@ def test_jacobi_blocked(nv, ln, gram):
     R = rt.cpu().numpy()
     np.testing.assert_allclose(R @ R.T, np.eye(nv), atol=1e-12)
     assert 0 < sweeps < 60
+
+
+@pytest.mark.parametrize("tri", [1, 2, 4, 8, 16, 4 | 8, 2 | 16])
+@pytest.mark.parametrize("n", [64, 200, 300])
+def test_gemm_structural_zero_hints(tri, n):
+    from syngular_amd import ops
+
+    case(f"gemm tri={tri} n={n}")
+    rng = np.random.default_rng(tri * 1000 + n)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, n))
+    if tri & 2:
+        A = np.tril(A)
+    if tri & 4:
+        A = np.triu(A)
+    if tri & 8:
+        B = np.triu(B)
+    if tri & 16:
+        B = np.tril(B)
+    C = T(np.zeros((n, n)))
+    ops.gemm_many([(T(A), T(B), C, 1.0, 0.0, tri)], tile=1 if n > 128 else 0)
+    got, ref = C.cpu().numpy(), A @ B
+    if tri & 1:
+        got, ref = np.tril(got), np.tril(ref)
+    np.testing.assert_allclose(got, ref, atol=1e-12 * n)
