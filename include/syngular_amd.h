@@ -44,8 +44,8 @@ int syn_version(void);
 /* number of gfx950 devices visible; fails with SYN_EHIP when none */
 int syn_device_count(int *count);
 /* ABI self-check: writes sizeof() of the item structs below, in declaration
- * order (gemm, apply, mpo_mpo, dsum, qr, jacobi, trunc, rowscale), into
- * out[0..n-1]; returns the number of structs. */
+ * order (gemm, apply, mpo_mpo, dsum, qr, jacobi, trunc, rowscale, bjac,
+ * eye, copy), into out[0..n-1]; returns the number of structs. */
 int syn_struct_sizes(int64_t *out, int n);
 
 /* ------------------------------------------------------------------ */
@@ -197,6 +197,47 @@ int syn_jacobi_f64(const syn_jacobi_item *items, int n_items, double tol, int ma
 int64_t syn_jacobi_blocked_workspace(int64_t nv, int64_t len);
 int syn_jacobi_blocked_f64(double *x, int64_t nv, int64_t len, int64_t ldx, double *rt, int64_t ldr, double *s,
                            double *work, double tol, int max_sweeps, int gram, int *sweeps, void *stream);
+
+/* Batched block Jacobi: one call orthogonalises the rows of many        */
+/* problems of uneven size; every step launch covers the block pairs of  */
+/* all problems still rotating (per-problem counters retire the          */
+/* converged ones after each sweep).  `items` is a DEVICE table and      */
+/* `items_host` the same table in host memory (the host loop reads the   */
+/* sizes from it); block_start and counter/active are filled by the     */
+/* library.  Same per-problem contract as syn_jacobi_f64.  work:          */
+/* syn_jacobi_blocked_many_workspace(...) doubles; counters: n_items ints.*/
+typedef struct {
+    double *x;
+    int64_t ldx, nv, len;
+    double *rt;
+    int64_t ldr;
+    double *s;
+    int64_t block_start;   /* first work-group of this problem in a step */
+    int32_t *counter;      /* rotations in the current sweep */
+    int32_t active;        /* 0 once converged */
+    int32_t gram;
+} syn_bjac_item;
+int syn_jacobi_blocked_many_f64(syn_bjac_item *items, const syn_bjac_item *items_host, int n_items, double *work,
+                                int32_t *counters, double tol, int max_sweeps, int *sweeps, void *stream);
+/* work size in doubles for problems (nv[i], len[i]) */
+int64_t syn_jacobi_blocked_many_workspace(const int64_t *nv, const int64_t *len, int n_items);
+
+/* Batched copy: dst[0..n) = src[0..n) for every item (compaction). */
+typedef struct {
+    const double *src;
+    double *dst;
+    int64_t n;
+    int64_t block_start;
+} syn_copy_item;
+int syn_copy_f64(const syn_copy_item *items, int n_items, int64_t total_blocks, void *stream);
+
+/* Batched identity: x (n x n, ld) = I for every item. */
+typedef struct {
+    double *x;
+    int64_t n, ld;
+    int64_t block_start;
+} syn_eye_item;
+int syn_eye_f64(const syn_eye_item *items, int n_items, int64_t total_blocks, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Cholesky / CholeskyQR (left-orthonormalisation of large unfoldings)   */

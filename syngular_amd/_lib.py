@@ -51,7 +51,14 @@ ROWSCALE_ITEM = np.dtype(
     [("x", _P), ("rows", _I), ("cols", _I), ("ld", _I), ("s", _P), ("mode", np.int32), ("block_start", _I)],
     align=True)
 
-ITEM_DTYPES = [GEMM_ITEM, APPLY_ITEM, MPO_MPO_ITEM, DSUM_ITEM, QR_ITEM, JACOBI_ITEM, TRUNC_ITEM, ROWSCALE_ITEM]
+BJAC_ITEM = np.dtype(
+    [("x", _P), ("ldx", _I), ("nv", _I), ("len", _I), ("rt", _P), ("ldr", _I), ("s", _P), ("block_start", _I),
+     ("counter", _P), ("active", np.int32), ("gram", np.int32)], align=True)
+EYE_ITEM = np.dtype([("x", _P), ("n", _I), ("ld", _I), ("block_start", _I)], align=True)
+COPY_ITEM = np.dtype([("src", _P), ("dst", _P), ("n", _I), ("block_start", _I)], align=True)
+
+ITEM_DTYPES = [GEMM_ITEM, APPLY_ITEM, MPO_MPO_ITEM, DSUM_ITEM, QR_ITEM, JACOBI_ITEM, TRUNC_ITEM, ROWSCALE_ITEM,
+               BJAC_ITEM, EYE_ITEM, COPY_ITEM]
 
 # every symbol the header declares, with its ctypes signature
 _VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -75,6 +82,10 @@ SIGNATURES = {
     "syn_cholqr_workspace": (_I64, [_I64]),
     "syn_potrf_f64": (_I32, [_VP, _I64, _I64, _VP, _I64, _VP, _D, _VP, _VP]),
     "syn_cholqr_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _D, _VP, _VP]),
+    "syn_jacobi_blocked_many_f64": (_I32, [_VP, _VP, _I32, _VP, _VP, _D, _I32, ctypes.POINTER(ctypes.c_int), _VP]),
+    "syn_jacobi_blocked_many_workspace": (_I64, [_VP, _VP, _I32]),
+    "syn_eye_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_copy_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_prof_begin": (_I32, []),
     "syn_prof_note_flops": (_I32, [_D]),
     "syn_prof_end": (_I32, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),

@@ -62,9 +62,9 @@ __device__ __forceinline__ void stage_chunk(double (*Ys)[kChunk + 4], const int 
 // KB rows per block, NT threads per work-group (NT/64 waves).  KB = 8 with a
 // single wave keeps the inner 16 x 16 problem free of cross-wave barriers.
 template <int KB, int NT>
-__global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv, int64_t len, int64_t ldx, double *R,
-                                                          int64_t ldr, int64_t nblk, int64_t step, double tol,
-                                                          int *counter, int inner_sweeps) {
+__device__ __forceinline__ void bjacobi_pair_step(double *X, int64_t nv, int64_t len, int64_t ldx, double *R,
+                                                  int64_t ldr, int64_t nblk, int64_t step, int64_t pair,
+                                                  double tol, int *counter, int inner_sweeps) {
     constexpr int P = 2 * KB;        // rows of a pair
     constexpr int RT = P / 16;       // 16-row MFMA tiles
     constexpr int NWV = NT / 64;     // waves
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv,
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int64_t np = nblk + (nblk & 1);
     int64_t ba, bb;
-    rr_pair(step, blockIdx.x, np, ba, bb);
+    rr_pair(step, pair, np, ba, bb);
     if (ba >= nblk && bb >= nblk) return;  // a block paired with the padding slot still
     if (tid < P) {                           // gets its own rows orthogonalised
         const int64_t blk = tid < KB ? ba : bb;
@@ -224,6 +224,84 @@ __global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv,
     if (tid == 0) atomicAdd(counter, 1);
 }
 
+template <int KB, int NT>
+__global__ __launch_bounds__(NT) void bjacobi_step_kernel(double *X, int64_t nv, int64_t len, int64_t ldx, double *R,
+                                                          int64_t ldr, int64_t nblk, int64_t step, double tol,
+                                                          int *counter, int inner_sweeps) {
+    bjacobi_pair_step<KB, NT>(X, nv, len, ldx, R, ldr, nblk, step, blockIdx.x, tol, counter, inner_sweeps);
+}
+
+// One step for many problems: work-group -> (problem, block pair).  A problem
+// with fewer blocks than the largest one repeats its round-robin schedule.
+__global__ __launch_bounds__(256) void bjacobi_many_step_kernel(const syn_bjac_item *__restrict__ items, int n,
+                                                                int64_t step, double tol, int inner_sweeps) {
+    const int64_t bid = blockIdx.x;
+    const syn_bjac_item &P = items[find_item(items, n, bid)];
+    const int64_t nblk = (P.nv + 15) / 16;
+    const int64_t np = nblk + (nblk & 1);
+    bjacobi_pair_step<16, 256>(P.x, P.nv, P.len, P.ldx, P.rt, P.ldr, nblk, step % (np - 1), bid - P.block_start,
+                               tol, P.counter, inner_sweeps);
+}
+
+__global__ __launch_bounds__(256) void bjac_many_init_kernel(const syn_bjac_item *__restrict__ items) {
+    const syn_bjac_item &P = items[blockIdx.x];
+    const int64_t nel = P.nv * P.nv;
+    for (int64_t e = threadIdx.x; e < nel; e += 256) {
+        const int64_t i = e / P.nv, j = e - i * P.nv;
+        P.rt[i * P.ldr + j] = (i == j) ? 1.0 : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void bjac_many_norms_kernel(const syn_bjac_item *__restrict__ items) {
+    const syn_bjac_item &P = items[blockIdx.x];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t i = w; i < P.nv; i += 4) {
+        double a = 0.0;
+        for (int64_t t = lane; t < P.len; t += 64) a = fma(P.x[i * P.ldx + t], P.x[i * P.ldx + t], a);
+        a = wave_sum(a);
+        if (lane == 0) P.s[i] = P.gram ? sqrt(sqrt(a)) : sqrt(a);
+    }
+}
+
+// Sort rows of problem b by perm (device, offsets in poff[b]) through the
+// workspace (xoff/roff[b]).  pass 0 gathers into the workspace, pass 1 copies back.
+__global__ __launch_bounds__(256) void bjac_many_permute_kernel(const syn_bjac_item *__restrict__ items,
+                                                                const int64_t *__restrict__ perm,
+                                                                const int64_t *__restrict__ poff, double *work,
+                                                                const int64_t *__restrict__ woff, int pass) {
+    const syn_bjac_item &P = items[blockIdx.x];
+    const int64_t *pm = perm + poff[blockIdx.x];
+    double *wx = work + woff[blockIdx.x];
+    double *wr = wx + P.nv * P.len;
+    double *ws = wr + P.nv * P.nv;
+    const int64_t nx = P.nv * P.len, nr = P.nv * P.nv;
+    for (int64_t e = threadIdx.x; e < nx + nr + P.nv; e += 256) {
+        if (e < nx) {
+            const int64_t i = e / P.len, t = e - i * P.len;
+            if (pass == 0)
+                wx[e] = P.x[pm[i] * P.ldx + t];
+            else
+                P.x[i * P.ldx + t] = wx[e];
+        } else if (e < nx + nr) {
+            const int64_t f = e - nx, i = f / P.nv, t = f - i * P.nv;
+            if (pass == 0)
+                wr[f] = P.rt[pm[i] * P.ldr + t];
+            else
+                P.rt[i * P.ldr + t] = wr[f];
+        } else {
+            const int64_t i = e - nx - nr;
+            if (pass == 0)
+                ws[i] = P.s[pm[i]];
+            else
+                P.s[i] = ws[i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_ints_kernel(int32_t *c, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) c[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void identity_kernel(double *R, int64_t n, int64_t ld) {
     const int64_t nel = n * n;
     for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nel; e += (int64_t)gridDim.x * 256) {
@@ -351,4 +429,123 @@ extern "C" int syn_jacobi_blocked_f64(double *X, int64_t nv, int64_t len, int64_
         return rc;
     // the host vectors are read by the copies above: wait before they go out of scope
     return hip_check(hipStreamSynchronize(st), "bjacobi final sync");
+}
+
+extern "C" int syn_jacobi_blocked_many_f64(syn_bjac_item *items, const syn_bjac_item *items_host, int n_items,
+                                           double *work, int32_t *counters, double tol, int max_sweeps, int *sweeps,
+                                           void *stream) {
+    using namespace syn;
+    if (n_items < 0 || !items || (n_items && (!items_host || !work || !counters)) || max_sweeps < 0 ||
+        !(tol >= 0.0)) {
+        set_error("syn_jacobi_blocked_many_f64: bad arguments");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0) return SYN_OK;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<syn_bjac_item> all(items_host, items_host + n_items);
+    for (int i = 0; i < n_items; ++i) {
+        if (all[i].nv <= 0 || all[i].len <= 0 || all[i].ldx < all[i].len || all[i].ldr < all[i].nv) {
+            set_error("syn_jacobi_blocked_many_f64: bad problem %d (nv=%lld len=%lld)", i, (long long)all[i].nv,
+                      (long long)all[i].len);
+            return SYN_EINVAL;
+        }
+        all[i].counter = counters + i;
+        all[i].active = all[i].nv > 1;
+        all[i].block_start = 0;
+    }
+    // full table on the device for init / norms / permute
+    auto upload = [&](const std::vector<syn_bjac_item> &v) -> int {
+        return hip_check(hipMemcpy(items, v.data(), v.size() * sizeof(syn_bjac_item), hipMemcpyHostToDevice),
+                         "bjac table upload");
+    };
+    if (int rc = upload(all)) return rc;
+    hipLaunchKernelGGL(bjac_many_init_kernel, dim3(n_items), dim3(256), 0, st, items);
+    SYN_LAUNCH_CHECK("bjac init");
+
+    std::vector<int32_t> hc(n_items);
+    std::vector<int> idx;  // problems still rotating
+    for (int i = 0; i < n_items; ++i)
+        if (all[i].active) idx.push_back(i);
+    int used = 0;
+    for (; used < max_sweeps && !idx.empty(); ++used) {
+        std::vector<syn_bjac_item> act;
+        int64_t start = 0, max_np = 2;
+        for (int i : idx) {
+            syn_bjac_item it = all[i];
+            const int64_t nblk = (it.nv + 15) / 16, np = nblk + (nblk & 1);
+            it.block_start = start;
+            start += np / 2;
+            if (np > max_np) max_np = np;
+            act.push_back(it);
+        }
+        if (int rc = upload(act)) return rc;
+        hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(256), 0, st, counters, n_items);
+        for (int64_t step = 0; step < max_np - 1; ++step) {
+            ProfScope ps(kTagBJacobiStep, 0.0, st);
+            hipLaunchKernelGGL(bjacobi_many_step_kernel, dim3((unsigned)start), dim3(256), 0, st, items,
+                               (int)act.size(), step, tol, g_inner_sweeps);
+        }
+        SYN_LAUNCH_CHECK("bjac many step");
+        if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                              st),
+                               "bjac counters"))
+            return rc;
+        if (int rc = hip_check(hipStreamSynchronize(st), "bjac sync")) return rc;
+        std::vector<int> next;
+        for (int i : idx)
+            if (hc[i] != 0) next.push_back(i);
+        idx.swap(next);
+        if (idx.empty()) {
+            ++used;
+            break;
+        }
+    }
+    if (sweeps) *sweeps = used;
+
+    // norms, per-problem argsort on the host, permute through the workspace
+    if (int rc = upload(all)) return rc;
+    hipLaunchKernelGGL(bjac_many_norms_kernel, dim3(n_items), dim3(256), 0, st, items);
+    SYN_LAUNCH_CHECK("bjac norms");
+    int64_t tot_nv = 0, tot_w = 0;
+    std::vector<int64_t> poff(n_items), woff(n_items);
+    for (int i = 0; i < n_items; ++i) {
+        poff[i] = tot_nv;
+        tot_nv += all[i].nv;
+        woff[i] = tot_w;
+        tot_w += all[i].nv * all[i].len + all[i].nv * all[i].nv + all[i].nv;
+    }
+    std::vector<double> hs(tot_nv);
+    for (int i = 0; i < n_items; ++i)
+        if (int rc = hip_check(hipMemcpyAsync(hs.data() + poff[i], all[i].s, all[i].nv * sizeof(double),
+                                              hipMemcpyDeviceToHost, st),
+                               "bjac norms copy"))
+            return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "bjac norms sync")) return rc;
+    std::vector<int64_t> hp(tot_nv);
+    for (int i = 0; i < n_items; ++i) {
+        int64_t *p = hp.data() + poff[i];
+        const double *sv = hs.data() + poff[i];
+        std::iota(p, p + all[i].nv, 0);
+        std::stable_sort(p, p + all[i].nv, [&](int64_t a, int64_t b) { return sv[a] > sv[b]; });
+    }
+    // device scratch for perm and offsets lives after the row workspace
+    int64_t *dperm = (int64_t *)(work + tot_w);
+    int64_t *dpoff = dperm + tot_nv;
+    int64_t *dwoff = dpoff + n_items;
+    if (int rc = hip_check(hipMemcpy(dperm, hp.data(), tot_nv * sizeof(int64_t), hipMemcpyHostToDevice), "perm"))
+        return rc;
+    if (int rc = hip_check(hipMemcpy(dpoff, poff.data(), n_items * sizeof(int64_t), hipMemcpyHostToDevice), "poff"))
+        return rc;
+    if (int rc = hip_check(hipMemcpy(dwoff, woff.data(), n_items * sizeof(int64_t), hipMemcpyHostToDevice), "woff"))
+        return rc;
+    hipLaunchKernelGGL(bjac_many_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dwoff, 0);
+    hipLaunchKernelGGL(bjac_many_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dwoff, 1);
+    SYN_LAUNCH_CHECK("bjac permute");
+    return hip_check(hipStreamSynchronize(st), "bjac final sync");
+}
+
+extern "C" int64_t syn_jacobi_blocked_many_workspace(const int64_t *nv, const int64_t *len, int n_items) {
+    int64_t tot = 0;
+    for (int i = 0; i < n_items; ++i) tot += nv[i] * len[i] + nv[i] * nv[i] + nv[i] + 2 * nv[i] + 2;
+    return tot + 2 * (int64_t)n_items + 16;
 }

@@ -40,7 +40,9 @@ extern "C" int syn_struct_sizes(int64_t *out, int n) {
     const int64_t sizes[] = {(int64_t)sizeof(syn_gemm_item),    (int64_t)sizeof(syn_apply_item),
                              (int64_t)sizeof(syn_mpo_mpo_item), (int64_t)sizeof(syn_dsum_item),
                              (int64_t)sizeof(syn_qr_item),      (int64_t)sizeof(syn_jacobi_item),
-                             (int64_t)sizeof(syn_trunc_item),   (int64_t)sizeof(syn_rowscale_item)};
+                             (int64_t)sizeof(syn_trunc_item),   (int64_t)sizeof(syn_rowscale_item),
+                             (int64_t)sizeof(syn_bjac_item),    (int64_t)sizeof(syn_eye_item),
+                             (int64_t)sizeof(syn_copy_item)};
     const int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return count;
     for (int i = 0; i < n && i < count; ++i) out[i] = sizes[i];

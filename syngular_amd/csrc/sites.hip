@@ -154,6 +154,31 @@ __global__ __launch_bounds__(kSiteThreads) void rowscale_kernel(const syn_rowsca
     }
 }
 
+__global__ __launch_bounds__(kSiteThreads) void eye_kernel(const syn_eye_item *__restrict__ items, int n,
+                                                           int64_t total) {
+    const int64_t bid = blockIdx.x;
+    const syn_eye_item &P = items[find_item(items, n, bid)];
+    const int64_t nel = P.n * P.n;
+    const int64_t base = (bid - P.block_start) * SYN_APPLY_BLOCK + threadIdx.x;
+    for (int q = 0; q < kPerThread; ++q) {
+        const int64_t idx = base + q * kSiteThreads;
+        if (idx >= nel) break;
+        const int64_t i = idx / P.n, j = idx - i * P.n;
+        P.x[i * P.ld + j] = (i == j) ? 1.0 : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kSiteThreads) void copy_kernel(const syn_copy_item *__restrict__ items, int n,
+                                                            int64_t total) {
+    const int64_t bid = blockIdx.x;
+    const syn_copy_item &P = items[find_item(items, n, bid)];
+    const int64_t base = (bid - P.block_start) * SYN_APPLY_BLOCK + threadIdx.x;
+    for (int q = 0; q < kPerThread; ++q) {
+        const int64_t idx = base + q * kSiteThreads;
+        if (idx < P.n) P.dst[idx] = P.src[idx];
+    }
+}
+
 static int grid_check(int n_items, int64_t total_blocks, const char *who) {
     if (n_items < 0 || total_blocks < 0 || total_blocks > 0x7fffffffLL) {
         set_error("%s: bad item count %d or block count %lld", who, n_items, (long long)total_blocks);
@@ -212,5 +237,25 @@ extern "C" int syn_kron_bonds_f64(const syn_mpo_mpo_item *items, int n_items, in
     hipLaunchKernelGGL(kron_kernel, dim3((unsigned)total_blocks), dim3(kSiteThreads), 0, (hipStream_t)stream, items,
                        n_items, total_blocks);
     SYN_LAUNCH_CHECK("syn_kron_bonds_f64 launch");
+    return SYN_OK;
+}
+
+extern "C" int syn_eye_f64(const syn_eye_item *items, int n_items, int64_t total_blocks, void *stream) {
+    using namespace syn;
+    if (int rc = grid_check(n_items, total_blocks, "syn_eye_f64")) return rc;
+    if (n_items == 0 || total_blocks == 0) return SYN_OK;
+    hipLaunchKernelGGL(eye_kernel, dim3((unsigned)total_blocks), dim3(kSiteThreads), 0, (hipStream_t)stream, items,
+                       n_items, total_blocks);
+    SYN_LAUNCH_CHECK("syn_eye_f64 launch");
+    return SYN_OK;
+}
+
+extern "C" int syn_copy_f64(const syn_copy_item *items, int n_items, int64_t total_blocks, void *stream) {
+    using namespace syn;
+    if (int rc = grid_check(n_items, total_blocks, "syn_copy_f64")) return rc;
+    if (n_items == 0 || total_blocks == 0) return SYN_OK;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)total_blocks), dim3(kSiteThreads), 0, (hipStream_t)stream, items,
+                       n_items, total_blocks);
+    SYN_LAUNCH_CHECK("syn_copy_f64 launch");
     return SYN_OK;
 }

@@ -1,0 +1,275 @@
+"""NumPy stand-in for the C ABI, for host-side tests of the grouped path.
+
+Test infrastructure only.  It replaces the library calls of
+``syngular_amd.batch`` with NumPy emulations that read and write CPU tensors
+at the addresses the item tables carry, and it checks every access of every
+item against the allocations the module made (``batch._empty`` /
+``batch._h2d`` / item-table uploads are routed to registered CPU tensors).
+So a host test can run the whole grouped sweep on the CPU and catch a wrong
+leading dimension, a short buffer or a bad block count before any GPU run.
+
+The emulated kernels follow the contracts in include/syngular_amd.h:
+GEMM (op(A) op(B) with the transA/transB and leading-dimension rules),
+MPO x MPS apply, Householder QR (numpy's QR has the same sign convention),
+one-sided Jacobi in Gram mode (x_in = rt^T x_out, rows sorted by decreasing
+norm, s = sqrt(row norm)), truncation (the oracle's keep rule), row scaling,
+identity and copy.
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+
+import numpy as np
+import torch
+
+from oracle.rounding import keep_count
+
+
+class BoundsError(AssertionError):
+    pass
+
+
+class Emulator:
+    def __init__(self):
+        self.regions = []          # (start, end, tensor)
+        self.calls = []
+
+    # ---- memory ------------------------------------------------------------
+    def register(self, t):
+        start = t.data_ptr()
+        self.regions.append((start, start + t.numel() * t.element_size(), t))
+        return t
+
+    def _check(self, addr, nbytes, what):
+        addr = int(addr)
+        if nbytes <= 0:
+            return
+        for s, e, _ in self.regions:
+            if s <= addr and addr + nbytes <= e:
+                return
+        raise BoundsError(f"{what}: access [{addr:#x}, +{nbytes}) outside every allocation")
+
+    def buf(self, addr, count, dtype=np.float64, what="buffer"):
+        dt = np.dtype(dtype)
+        self._check(addr, count * dt.itemsize, what)
+        if count == 0:
+            return np.zeros(0, dt)
+        raw = (ctypes.c_char * (count * dt.itemsize)).from_address(int(addr))
+        return np.frombuffer(raw, dtype=dt)
+
+    def mat(self, addr, rows, cols, ld, trans, what):
+        """View of op(X) (rows x cols) with X row-major, leading dim ld."""
+        if rows == 0 or cols == 0:
+            return np.zeros((rows, cols))
+        if trans:   # op(X)[i, j] = X[j*ld + i]
+            ext = (cols - 1) * ld + rows
+            b = self.buf(addr, ext, what=what)
+            return np.lib.stride_tricks.as_strided(b, (rows, cols), (8, 8 * ld))
+        ext = (rows - 1) * ld + cols
+        b = self.buf(addr, ext, what=what)
+        return np.lib.stride_tricks.as_strided(b, (rows, cols), (8 * ld, 8))
+
+    def table(self, addr, n, dtype):
+        return np.frombuffer(self.buf(addr, n * dtype.itemsize, np.uint8, "item table").tobytes(), dtype=dtype)
+
+    # ---- dispatch ----------------------------------------------------------
+    def call(self, name, *args):
+        self.calls.append(name)
+        fn = getattr(self, name, None)
+        if fn is None:
+            raise NotImplementedError(f"emulator has no {name}")
+        return fn(*args)
+
+    def syn_prof_note_flops(self, fl):
+        return 0
+
+    def _blocks_ok(self, it, nel_or_tiles, total, name, per=None):
+        nb = nel_or_tiles if per is None else (np.asarray(nel_or_tiles) + per - 1) // per
+        st = np.concatenate([[0], np.cumsum(nb)[:-1]]) if len(nb) else np.zeros(0)
+        if not np.array_equal(it["block_start"], st) or int(np.sum(nb)) != int(total):
+            raise BoundsError(f"{name}: block_start/total_blocks do not match the items")
+
+    def syn_gemm_f64(self, items, n, total, tile, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.GEMM_ITEM)
+        bm = 128 if tile else 64
+        tiles = ((it["m"] + bm - 1) // bm) * ((it["n"] + bm - 1) // bm)
+        self._blocks_ok(it, tiles, total, "gemm")
+        if not np.array_equal(it["tiles_n"], (it["n"] + bm - 1) // bm):
+            raise BoundsError("gemm: tiles_n")
+        for r in it:
+            m, nn, k = int(r["m"]), int(r["n"]), int(r["k"])
+            A = self.mat(r["A"], m, k, int(r["lda"]), int(r["transA"]), "gemm A")
+            Bm = self.mat(r["B"], k, nn, int(r["ldb"]), int(r["transB"]), "gemm B")
+            C = self.mat(r["C"], m, nn, int(r["ldc"]), 0, "gemm C")
+            res = float(r["alpha"]) * (A @ Bm)
+            if float(r["beta"]) != 0.0:
+                res = res + float(r["beta"]) * C
+            C[...] = res
+        return 0
+
+    def syn_mpo_apply_f64(self, items, n, total, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.APPLY_ITEM)
+        nel = it["chil"] * it["dl"] * it["dout"] * it["chir"] * it["dr"]
+        self._blocks_ok(it, nel, total, "apply", per=_lib.APPLY_BLOCK)
+        for r in it:
+            a, s, b, c, o, e = (int(r[f]) for f in ("chil", "din", "chir", "dl", "dout", "dr"))
+            X = self.buf(r["x"], a * s * b, what="apply x").reshape(a, s, b)
+            O = self.buf(r["o"], c * s * o * e, what="apply o").reshape(c, s, o, e)
+            P = self.buf(r["p"], a * c * o * b * e, what="apply p")
+            P[:] = np.einsum("asb,csoe->acobe", X, O).reshape(-1)
+        return 0
+
+    def syn_eye_f64(self, items, n, total, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.EYE_ITEM)
+        self._blocks_ok(it, it["n"] ** 2, total, "eye", per=_lib.APPLY_BLOCK)
+        for r in it:
+            X = self.mat(r["x"], int(r["n"]), int(r["n"]), int(r["ld"]), 0, "eye")
+            X[...] = np.eye(int(r["n"]))
+        return 0
+
+    def syn_copy_f64(self, items, n, total, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.COPY_ITEM)
+        self._blocks_ok(it, it["n"], total, "copy", per=_lib.APPLY_BLOCK)
+        for r in it:
+            src = self.buf(r["src"], int(r["n"]), what="copy src").copy()
+            self.buf(r["dst"], int(r["n"]), what="copy dst")[:] = src
+        return 0
+
+    def syn_rowscale_f64(self, items, n, total, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.ROWSCALE_ITEM)
+        self._blocks_ok(it, it["rows"] * it["cols"], total, "rowscale", per=_lib.APPLY_BLOCK)
+        for r in it:
+            rows = int(r["rows"])
+            X = self.mat(r["x"], rows, int(r["cols"]), int(r["ld"]), 0, "rowscale x")
+            s = self.buf(r["s"], rows, what="rowscale s")
+            if int(r["mode"]) == 0:
+                X *= s[:, None]
+            else:
+                inv = np.where(s != 0, 1.0 / np.where(s != 0, s, 1.0), 0.0)
+                X *= inv[:, None]
+        return 0
+
+    def syn_qr_f64(self, items, n, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.QR_ITEM)
+        for r in it:
+            m, nn, keep = int(r["m"]), int(r["n"]), int(r["keep"])
+            nb, kq = min(nn, keep), min(keep, m)
+            ext = (m - 1) * int(r["rs"]) + (nn - 1) * int(r["cs"]) + 1
+            b = self.buf(r["a"], ext, what="qr a")
+            A = np.lib.stride_tricks.as_strided(b, (m, nn), (8 * int(r["rs"]), 8 * int(r["cs"])))
+            self.buf(r["work"], m * (nb + (kq if r["q_trans"] else 0)), what="qr work")
+            self.buf(r["tau"], nb, what="qr tau")
+            Q = np.linalg.qr(A[:, :nb], mode="complete")[0][:, :kq]
+            if r["q_trans"]:
+                self.mat(r["q"], kq, m, int(r["ldq"]), 0, "qr q")[...] = Q.T
+            else:
+                self.mat(r["q"], m, kq, int(r["ldq"]), 0, "qr q")[...] = Q
+        return 0
+
+    def _jacobi_one(self, x, ldx, nv, ln, rt, ldr, s, gram):
+        X = self.mat(x, nv, ln, ldx, 0, "jacobi x")
+        R = self.mat(rt, nv, nv, ldr, 0, "jacobi rt")
+        S = self.buf(s, nv, what="jacobi s")
+        if gram:
+            if nv != ln:
+                raise BoundsError("gram mode needs a square matrix")
+            w, V = np.linalg.eigh(0.5 * (X + X.T))
+            order = np.argsort(-w, kind="stable")
+            w, V = w[order].clip(min=0), V[:, order]
+            R[...] = V.T
+            X[...] = V.T @ X
+            S[:] = np.sqrt(w)
+        else:
+            U, sv, Vt = np.linalg.svd(X, full_matrices=False)
+            R[...] = np.eye(nv)
+            R[:len(sv)] = U.T
+            X[...] = 0
+            X[:len(sv)] = sv[:, None] * Vt
+            S[:] = 0
+            S[:len(sv)] = sv
+
+    def syn_jacobi_f64(self, items, n, tol, max_sweeps, stream):
+        from syngular_amd import _lib
+        for r in self.table(items, n, _lib.JACOBI_ITEM):
+            self._jacobi_one(r["x"], int(r["ldx"]), int(r["nv"]), int(r["len"]), r["rt"], int(r["ldr"]), r["s"],
+                             int(r["gram"]))
+            if int(r["sweeps"]):
+                self.buf(r["sweeps"], 1, np.int32, "jacobi sweeps")[0] = 1
+        return 0
+
+    def syn_jacobi_blocked_many_f64(self, items_dev, items_host, n, work, counters, tol, max_sweeps, sweeps, stream):
+        from syngular_amd import _lib
+        self.buf(items_dev, n * _lib.BJAC_ITEM.itemsize, np.uint8, "bjac device table")
+        addr = items_host.value if isinstance(items_host, ctypes.c_void_p) else int(items_host)
+        host = (ctypes.c_char * (n * _lib.BJAC_ITEM.itemsize)).from_address(addr)
+        it = np.frombuffer(bytes(host), dtype=_lib.BJAC_ITEM)
+        nv = it["nv"].astype(np.int64)
+        ln = it["len"].astype(np.int64)
+        self.buf(work, int(_workspace(nv, ln)), what="bjac work")
+        self.buf(counters, n, np.int32, "bjac counters")
+        for r in it:
+            self._jacobi_one(r["x"], int(r["ldx"]), int(r["nv"]), int(r["len"]), r["rt"], int(r["ldr"]), r["s"],
+                             int(r["gram"]))
+        if sweeps is not None and hasattr(sweeps, "_obj"):
+            sweeps._obj.value = 1
+        return 0
+
+    def syn_truncate_f64(self, items, n, stream):
+        from syngular_amd import _lib
+        for r in self.table(items, n, _lib.TRUNC_ITEM):
+            s = self.buf(r["s"], int(r["n"]), what="trunc s")
+            c = keep_count(s.copy(), int(r["chi_max"]) or None, float(r["cutoff"]))
+            self.buf(r["keep"], 1, np.int32, "trunc keep")[0] = c
+            self.buf(r["discarded"], 1, what="trunc disc")[0] = float(np.sum(s[c:] ** 2))
+        return 0
+
+
+def _workspace(nv, ln):
+    nv, ln = np.asarray(nv, np.int64), np.asarray(ln, np.int64)
+    return int((nv * ln + nv * nv + nv + 2 * nv + 2).sum() + 2 * len(nv) + 16)
+
+
+class _FakeLib:
+    def syn_jacobi_blocked_many_workspace(self, nv_ptr, len_ptr, n):
+        nv = np.ctypeslib.as_array(ctypes.cast(nv_ptr, ctypes.POINTER(ctypes.c_int64)), (n,))
+        ln = np.ctypeslib.as_array(ctypes.cast(len_ptr, ctypes.POINTER(ctypes.c_int64)), (n,))
+        return _workspace(nv, ln)
+
+
+def install(monkeypatch):
+    """Route syngular_amd.batch / serving through the emulator on the CPU."""
+    from syngular_amd import _lib, batch, ops
+
+    emu = Emulator()
+    cpu = torch.device("cpu")
+
+    def empty(n, dtype=torch.float64, zero=False):
+        f = torch.zeros if zero else torch.empty
+        return emu.register(f(max(int(n), 1), dtype=dtype, device=cpu))
+
+    def h2d(host):
+        return emu.register(host.clone())
+
+    def upload(items):
+        raw = np.ascontiguousarray(items).view(np.uint8).reshape(-1)
+        return emu.register(torch.from_numpy(raw.copy()))
+
+    monkeypatch.setattr(batch, "_empty", empty)
+    monkeypatch.setattr(batch, "_h2d", h2d)
+    monkeypatch.setattr(batch, "call", emu.call)
+    monkeypatch.setattr(ops, "_upload", upload)
+    monkeypatch.setattr(ops, "device", lambda: cpu)
+    monkeypatch.setattr(ops, "_stream", lambda: ctypes.c_void_p(0))
+    monkeypatch.setattr(_lib, "load", lambda: _FakeLib())
+    return emu
+
+
+__all__ = ["Emulator", "install", "BoundsError"]
+sys.modules.setdefault("abi_emulator", sys.modules[__name__])

@@ -1,0 +1,145 @@
+"""The grouped many-chain sweep run on the CPU through the NumPy stand-in of
+the C ABI (tests/abi_emulator.py): item tables, leading dimensions, block
+counts and buffer extents are checked on every call, and the results are
+compared with the NumPy FP64 oracle.  No GPU; the real kernels are covered by
+tests/test_gpu_batch.py."""
+import numpy as np
+import pytest
+
+import abi_emulator
+from oracle import reference as oref
+from oracle import rounding as ornd
+from syngular_amd import batch as B
+from syngular_amd import serving
+from syngular_amd import workloads as W
+
+TOL = 1e-10
+
+
+@pytest.fixture
+def emu(monkeypatch):
+    return abi_emulator.install(monkeypatch)
+
+
+def _chain(rng, n, chi, d=2):
+    b = [1] + [chi] * (n - 1) + [1]
+    return [rng.standard_normal((b[k], d, b[k + 1])) / np.sqrt(chi) for k in range(n)]
+
+
+def _mpo(rng, n, D, d=2):
+    w = [1] + [D] * (n - 1) + [1]
+    return [rng.standard_normal((w[k], d, d, w[k + 1])) / np.sqrt(D * d) for k in range(n)]
+
+
+SPEC = [(3, 2, 2), (5, 4, 3), (7, 8, 4), (4, 16, 2), (9, 6, 4), (14, 40, 3), (2, 5, 4), (8, 12, 1), (1, 3, 2)]
+
+
+def test_emulated_batched_round_matches_oracle(emu):
+    rng = np.random.default_rng(0)
+    jobs = [(_chain(rng, n, c), _mpo(rng, n, D), c) for n, c, D in SPEC]
+    X = B.ChainBatch.from_numpy([j[0] for j in jobs])
+    O = B.OperatorBatch.from_numpy([j[1] for j in jobs])
+    recs = []
+    Z = B.apply_and_round(X, O, np.array([j[2] for j in jobs]), 0.0, records=recs)
+    out = Z.to_numpy()
+    assert "syn_jacobi_blocked_many_f64" in emu.calls and "syn_qr_f64" in emu.calls and "syn_eye_f64" in emu.calls
+    for q, (xs, os_, c) in enumerate(jobs):
+        ref, sv, kept, disc = ornd.apply_and_round(os_, xs, chi_max=c)
+        assert [o.shape for o in out[q]] == [r.shape for r in ref]
+        a, b = oref.dense_mps(out[q]), oref.dense_mps(ref)
+        assert np.abs(a - b).max() <= TOL * np.abs(b).max()
+    for r in recs:
+        for i, q in enumerate(r.jobs):
+            assert int(r.kept[i]) == ornd.apply_and_round(jobs[q][1], jobs[q][0], chi_max=jobs[q][2])[2][int(r.bond[i])]
+
+
+def test_emulated_cutoff_and_no_cap(emu):
+    rng = np.random.default_rng(2)
+    jobs = [(_chain(rng, n, c), _mpo(rng, n, D), c) for n, c, D in SPEC[:5]]
+    X = B.ChainBatch.from_numpy([j[0] for j in jobs])
+    O = B.OperatorBatch.from_numpy([j[1] for j in jobs])
+    cut = np.array([1e-3, 0.0, 1e-2, 1e-6, 0.0])
+    Z = B.apply_and_round(X, O, np.zeros(5, np.int64), cut)
+    out = Z.to_numpy()
+    for q, (xs, os_, c) in enumerate(jobs):
+        ref = ornd.apply_and_round(os_, xs, chi_max=None, cutoff=cut[q])[0]
+        assert [o.shape for o in out[q]] == [r.shape for r in ref]
+        a, b = oref.dense_mps(out[q]), oref.dense_mps(ref)
+        assert np.abs(a - b).max() <= TOL * np.abs(b).max()
+
+
+def test_emulated_overlap(emu):
+    rng = np.random.default_rng(4)
+    A = [_chain(rng, n, c) for n, c in [(3, 2), (6, 5), (4, 9), (1, 1)]]
+    C = [_chain(rng, n, c) for n, c in [(3, 3), (6, 2), (4, 9), (1, 1)]]
+    ov = B.overlap(B.ChainBatch.from_numpy(A), B.ChainBatch.from_numpy(C)).numpy()
+    for q in range(4):
+        assert abs(ov[q] - oref.overlap(A[q], C[q])) <= 1e-12 * np.sqrt(ornd.norm2(A[q]) * ornd.norm2(C[q]))
+    with pytest.raises(ValueError, match="edge bonds"):
+        lit = [[np.ones((2, 2, 1))]]
+        B.overlap(B.ChainBatch.from_numpy(lit), B.ChainBatch.from_numpy(lit))
+
+
+def test_emulated_gate_layers(emu):
+    jobs = W.batching_jobs(10, 7, n_range=(2, 9), chis=(2, 4, 8, 16, 64))
+    X = B.ChainBatch.from_numpy([j["cores"] for j in jobs])
+    for off in (0, 1):
+        gates = [W.haar_orthogonal(np.random.default_rng(q), W.npairs(j["n"], off)) for q, j in enumerate(jobs)]
+        O = B.gate_layer([j["n"] for j in jobs], np.full(len(jobs), off), np.concatenate(gates))
+        recs = []
+        Z = B.apply_and_round(X, O, np.array([j["chi"] for j in jobs]), 0.0, records=recs)
+        out = Z.to_numpy()
+        n2 = B.overlap(Z, Z).numpy()
+        n2x = B.overlap(X, X).numpy()
+        disc = np.zeros(len(jobs))
+        for r in recs:
+            np.add.at(disc, r.jobs, r.disc.numpy())
+        for q, j in enumerate(jobs):
+            mpo = W.gate_mpo_cores(j["n"], off, gates[q])
+            ref = ornd.apply_and_round(mpo, j["cores"], chi_max=j["chi"])[0]
+            a, b = oref.dense_mps(out[q]), oref.dense_mps(ref)
+            assert np.abs(a - b).max() <= TOL * np.abs(b).max()
+            assert abs(n2[q] + disc[q] - n2x[q]) <= 1e-10 * n2x[q]
+
+
+def test_emulated_server_alone_equals_mix(emu):
+    jobs = W.serving_jobs(5, 1e6, 3, n_range=(5, 9), layer_range=(2, 4), chis=(4, 8), cutoff=1e-10)
+    rep = serving.Server(serving.DeviceExecutor(), hbm_budget=10 ** 11, keep_states=True).run(jobs)
+    mix = {r.id: r for r in rep.results}
+    for j in jobs:
+        cores = [np.eye(2)[0].reshape(1, 2, 1) for _ in range(j["n"])]
+        for l in range(j["layers"]):
+            mpo = W.gate_mpo_cores(j["n"], l % 2, W.layer_gates(j["seed"], j["id"], l, j["n"]))
+            cores = ornd.apply_and_round(mpo, cores, chi_max=j["chi_max"], cutoff=j["cutoff"])[0]
+        got = mix[j["id"]]
+        a, b = oref.dense_mps(got.cores), oref.dense_mps(cores)
+        assert np.abs(a - b).max() <= TOL
+        assert abs(got.norm2 + got.discarded - 1.0) <= 1e-10
+    solo = serving.Server(serving.DeviceExecutor(), hbm_budget=10 ** 11, keep_states=True).run(
+        [dict(jobs[2], arrival=0.0)])
+    a, b = oref.dense_mps(solo.results[0].cores), oref.dense_mps(mix[jobs[2]["id"]].cores)
+    assert np.abs(a - b).max() <= 1e-12
+
+
+def test_emulator_catches_short_buffer(emu):
+    """The stand-in must reject an item that reaches past its buffer."""
+    a, pa = B.arena([4])
+    b, pb = B.arena([4])
+    c, pc = B.arena([4])
+    with pytest.raises(abi_emulator.BoundsError):
+        B.gemm_table(pa, pb, pc, [2], 2, 200, 200, 2, 2)
+
+
+def test_emulated_server_longest_job_leaves_first(emu):
+    """After the longest chain retires, the resident tables are narrower than
+    before; later layers must still line up."""
+    jobs = [{"id": 0, "arrival": 0.0, "n": 9, "layers": 1, "chi_max": 4, "cutoff": 0.0, "seed": 1},
+            {"id": 1, "arrival": 0.0, "n": 5, "layers": 3, "chi_max": 4, "cutoff": 0.0, "seed": 1},
+            {"id": 2, "arrival": 0.0, "n": 6, "layers": 2, "chi_max": 8, "cutoff": 0.0, "seed": 1}]
+    rep = serving.Server(serving.DeviceExecutor(), hbm_budget=10 ** 11, keep_states=True).run(jobs)
+    for r, j in zip(rep.results, jobs):
+        cores = [np.eye(2)[0].reshape(1, 2, 1) for _ in range(j["n"])]
+        for l in range(j["layers"]):
+            mpo = W.gate_mpo_cores(j["n"], l % 2, W.layer_gates(j["seed"], j["id"], l, j["n"]))
+            cores = ornd.apply_and_round(mpo, cores, chi_max=j["chi_max"])[0]
+        assert np.abs(oref.dense_mps(r.cores) - oref.dense_mps(cores)).max() <= TOL
