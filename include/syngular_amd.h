@@ -45,7 +45,7 @@ int syn_version(void);
 int syn_device_count(int *count);
 /* ABI self-check: writes sizeof() of the item structs below, in declaration
  * order (gemm, apply, mpo_mpo, dsum, qr, jacobi, trunc, rowscale, bjac,
- * eye, copy), into out[0..n-1]; returns the number of structs. */
+ * eye, copy, diagmax, chol), into out[0..n-1]; returns the number of structs. */
 int syn_struct_sizes(int64_t *out, int n);
 
 /* ------------------------------------------------------------------ */
@@ -186,6 +186,16 @@ typedef struct {
 } syn_jacobi_item;
 int syn_jacobi_f64(const syn_jacobi_item *items, int n_items, double tol, int max_sweeps, void *stream);
 
+/* Two-sided Jacobi eigen-decomposition of symmetric Gram matrices held  */
+/* in LDS, one work-group per item (x: nv x nv with len == nv, read only; */
+/* nv <= nv_max <= 64 for every item).  On exit rt rows are the           */
+/* eigenvectors (rt = V^T, G = V diag(lambda) V^T) sorted by decreasing    */
+/* eigenvalue and s[i] = sqrt(max(lambda_i, 0)), the singular values of C */
+/* when G = C^T C.  Items outside the contract get s = NaN.  No reference */
+/* call site (north-star addition: SVD rounding, small bonds).           */
+int syn_jacobi_gram_f64(const syn_jacobi_item *items, int n_items, int64_t nv_max, double tol, int max_sweeps,
+                        void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Block one-sided Jacobi for large problems: same contract as            */
 /* syn_jacobi_f64 for one problem (x: nv x len rows, rt: nv x nv, s: nv),  */
@@ -252,6 +262,36 @@ int syn_eye_f64(const syn_eye_item *items, int n_items, int64_t total_blocks, vo
 /* (tensor/matrix_product_state.py:560, the left sweep of the rounding). */
 /* ------------------------------------------------------------------ */
 int64_t syn_cholqr_workspace(int64_t n);
+
+/* Batched pieces of the same factorisation, for many problems of uneven  */
+/* size (the caller drives the recursion with syn_gemm_f64):              */
+/* syn_diagmax_f64: *out = max_i g[i*ld+i] and *dropped = 0, per item.    */
+/* syn_chol_leaf_f64: per item, factor the n x n (n <= 64) block g        */
+/* (lower triangle read) as L L^T with the pivot-drop rule above           */
+/* (threshold tolrel * *maxdiag), write L (zeros above the diagonal) and  */
+/* X = inv(L~) with dropped columns zeroed (X is n x n, ld ldx), and add  */
+/* the dropped pivots to *dropped.  One work-group per item.              */
+typedef struct {
+    const double *g;
+    int64_t n, ld;
+    double *out;
+    int32_t *dropped;
+} syn_diagmax_item;
+int syn_diagmax_f64(const syn_diagmax_item *items, int n_items, void *stream);
+
+typedef struct {
+    const double *g;
+    int64_t ldg;
+    double *l;
+    int64_t ldl;
+    double *x;
+    int64_t ldx;
+    int64_t n;
+    const double *maxdiag;
+    double tolrel;
+    int32_t *dropped;
+} syn_chol_item;
+int syn_chol_leaf_f64(const syn_chol_item *items, int n_items, void *stream);
 int syn_potrf_f64(double *g, int64_t n, int64_t ldg, double *l, int64_t ldl, double *work, double tolrel,
                   int *dropped, void *stream);
 int syn_cholqr_f64(double *a, int64_t m, int64_t n, int64_t lda, double *q, int64_t ldq, double *l, int64_t ldl,

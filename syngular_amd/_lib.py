@@ -56,9 +56,13 @@ BJAC_ITEM = np.dtype(
      ("counter", _P), ("active", np.int32), ("gram", np.int32)], align=True)
 EYE_ITEM = np.dtype([("x", _P), ("n", _I), ("ld", _I), ("block_start", _I)], align=True)
 COPY_ITEM = np.dtype([("src", _P), ("dst", _P), ("n", _I), ("block_start", _I)], align=True)
+DIAGMAX_ITEM = np.dtype([("g", _P), ("n", _I), ("ld", _I), ("out", _P), ("dropped", _P)], align=True)
+CHOL_ITEM = np.dtype(
+    [("g", _P), ("ldg", _I), ("l", _P), ("ldl", _I), ("x", _P), ("ldx", _I), ("n", _I), ("maxdiag", _P),
+     ("tolrel", np.float64), ("dropped", _P)], align=True)
 
 ITEM_DTYPES = [GEMM_ITEM, APPLY_ITEM, MPO_MPO_ITEM, DSUM_ITEM, QR_ITEM, JACOBI_ITEM, TRUNC_ITEM, ROWSCALE_ITEM,
-               BJAC_ITEM, EYE_ITEM, COPY_ITEM]
+               BJAC_ITEM, EYE_ITEM, COPY_ITEM, DIAGMAX_ITEM, CHOL_ITEM]
 
 # every symbol the header declares, with its ctypes signature
 _VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -74,6 +78,7 @@ SIGNATURES = {
     "syn_direct_sum_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_qr_f64": (_I32, [_VP, _I32, _VP]),
     "syn_jacobi_f64": (_I32, [_VP, _I32, _D, _I32, _VP]),
+    "syn_jacobi_gram_f64": (_I32, [_VP, _I32, _I64, _D, _I32, _VP]),
     "syn_truncate_f64": (_I32, [_VP, _I32, _VP]),
     "syn_rowscale_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_jacobi_blocked_workspace": (_I64, [_I64, _I64]),
@@ -86,6 +91,8 @@ SIGNATURES = {
     "syn_jacobi_blocked_many_workspace": (_I64, [_VP, _VP, _I32]),
     "syn_eye_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_copy_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_diagmax_f64": (_I32, [_VP, _I32, _VP]),
+    "syn_chol_leaf_f64": (_I32, [_VP, _I32, _VP]),
     "syn_prof_begin": (_I32, []),
     "syn_prof_note_flops": (_I32, [_D]),
     "syn_prof_end": (_I32, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),

@@ -35,6 +35,7 @@ from ._lib import call
 F64 = torch.float64
 ALIGN = 32                      # doubles (256 B) between sub-allocations
 LDS_JACOBI_MAX = 64             # Gram matrices up to this size: one work-group in LDS
+CHOLQR_MIN_COLS = 96            # tall unfoldings with this many columns: CholeskyQR2
 # Jacobi orthogonality thresholds, fixed per path (not per launch) so that a
 # chain's rotations do not depend on the other chains of its batch: the
 # rounding bound length * eps of the largest problem each path takes.
@@ -183,15 +184,17 @@ def qr_table(a, m, n, q, work, tau):
         call("syn_qr_f64", dev.data_ptr(), len(it), ops._stream())
 
 
-def jacobi_table(x, g, rt, s, gram=1):
-    """LDS one-sided Jacobi of every g x g problem (one work-group each)."""
+def jacobi_table(x, g, rt, s):
+    """LDS two-sided Jacobi of every symmetric g x g Gram matrix (g <= 64,
+    one work-group each); x is read only."""
     it = np.zeros(len(x), dtype=_lib.JACOBI_ITEM)
     it["x"], it["ldx"], it["nv"], it["len"] = x, g, g, g
     it["rt"], it["ldr"], it["s"], it["sweeps"] = rt, g, s, 0
-    it["gram"] = gram
+    it["gram"] = 1
     if len(it):
         dev = ops._upload(it)
-        call("syn_jacobi_f64", dev.data_ptr(), len(it), TOL_LDS, ops.JACOBI_SWEEPS, ops._stream())
+        call("syn_jacobi_gram_f64", dev.data_ptr(), len(it), int(np.max(g)), TOL_LDS, ops.JACOBI_SWEEPS,
+             ops._stream())
 
 
 def jacobi_blocked_table(x, g, rt, s, gram=1):
@@ -215,6 +218,96 @@ def jacobi_blocked_table(x, g, rt, s, gram=1):
          work.data_ptr(), counters.data_ptr(), TOL_BLOCKED, ops.JACOBI_SWEEPS, ctypes.byref(sweeps),
          ops._stream())
     return sweeps.value
+
+
+def diagmax_table(g, n, ld, out, dropped):
+    it = np.zeros(len(g), dtype=_lib.DIAGMAX_ITEM)
+    it["g"], it["n"], it["ld"], it["out"], it["dropped"] = g, n, ld, out, dropped
+    if len(it):
+        dev = ops._upload(it)
+        call("syn_diagmax_f64", dev.data_ptr(), len(it), ops._stream())
+
+
+def chol_leaf_table(g, l, x, ld, n, maxdiag, tolrel, dropped):
+    it = np.zeros(len(g), dtype=_lib.CHOL_ITEM)
+    it["g"], it["ldg"], it["l"], it["ldl"], it["x"], it["ldx"] = g, ld, l, ld, x, ld
+    it["n"], it["maxdiag"], it["tolrel"], it["dropped"] = n, maxdiag, tolrel, dropped
+    if len(it):
+        dev = ops._upload(it)
+        call("syn_chol_leaf_f64", dev.data_ptr(), len(it), ops._stream())
+
+
+CHOL_LEAF = 64
+
+
+def _chol_inv_many(G, L, X, ld, n, T, maxdiag, tolrel, dropped):
+    """Recursive Cholesky with inverse for many problems at once:
+    G = L L^T and X = inv(L~) (pivot-drop convention of syn_chol_leaf_f64) on
+    the n x n blocks at G/L/X (leading dimension ld, L and X zero on entry):
+        (L11, X11) = chol_inv(G11);  L21 = G21 X11^T;  G22 -= L21 L21^T;
+        (L22, X22) = chol_inv(G22);  X21 = -X22 (L21 X11).
+    Problems of different size share every launch of their level; blocks of
+    at most 64 are factored by the leaf kernel.  T: per-problem scratch of at
+    least n1 * n2 doubles."""
+    leaf = n <= CHOL_LEAF
+    if leaf.any():
+        chol_leaf_table(G[leaf], L[leaf], X[leaf], ld[leaf], n[leaf], maxdiag[leaf], tolrel[leaf], dropped[leaf])
+    nl = ~leaf
+    if not nl.any():
+        return
+    G, L, X, ld, n, T, maxdiag, tolrel, dropped = (v[nl] for v in (G, L, X, ld, n, T, maxdiag, tolrel, dropped))
+    n1 = ((n // CHOL_LEAF + 1) // 2) * CHOL_LEAF
+    n2 = n - n1
+    _chol_inv_many(G, L, X, ld, n1, T, maxdiag, tolrel, dropped)
+    e = np.uint64(8)
+    down = (n1 * ld).astype(np.uint64) * e            # row offset n1
+    diag = (n1 * ld + n1).astype(np.uint64) * e       # block (n1, n1)
+    L21 = L + down
+    # L21 = G21 X11^T (op(B) = X11^T upper)
+    gemm_table(G + down, X, L21, n2, n1, n1, ld, ld, ld, tb=1, tri=8)
+    # G22 -= L21 L21^T, lower triangle only
+    gemm_table(L21, L21, G + diag, n2, n2, n1, ld, ld, ld, tb=1, alpha=-1.0, beta=1.0, tri=1)
+    _chol_inv_many(G + diag, L + diag, X + diag, ld, n2, T, maxdiag, tolrel, dropped)
+    # T = L21 X11 (op(B) = X11 lower);  X21 = -X22 T (op(A) = X22 lower)
+    gemm_table(L21, X, T, n2, n1, n1, ld, ld, n1, tri=16)
+    gemm_table(X + diag, T, X + down, n2, n1, n2, ld, n1, ld, alpha=-1.0, tri=2)
+
+
+def cholqr_table(A, m, n, lda, hold):
+    """One CholeskyQR pass on many tall row-major m x n matrices (A kept):
+    G = A^T A, G = L L^T with X = inv(L~), Q = A X^T.  Returns the addresses
+    of Q (m x n, ld n) and L (n x n lower, ld n); buffers go to `hold`."""
+    cnt = len(A)
+    gt, G = arena(n * n)
+    lt, Lp = arena(n * n)
+    xt, Xp = arena(n * n)
+    lt.zero_()
+    xt.zero_()
+    # recursion scratch: every level needs at most n1 * n2 <= n^2 / 2 + 4096
+    tt, Tp = arena(n * n // 2 + CHOL_LEAF * CHOL_LEAF)
+    st, sp = arena(np.ones(cnt, np.int64))
+    dt_ = _empty(cnt, torch.int32, zero=True)
+    dp = np.uint64(dt_.data_ptr()) + np.uint64(4) * np.arange(cnt, dtype=np.uint64)
+    qt, Q = arena(m * n)
+    gemm_table(A, A, G, n, n, m, lda, lda, n, ta=1, tri=1)
+    diagmax_table(G, n, n, sp, dp)
+    tol = np.array([ops.cholqr_tol(int(a), int(b)) for a, b in zip(m, n)])
+    _chol_inv_many(G, Lp, Xp, n.copy(), n.copy(), Tp, sp, tol, dp)
+    gemm_table(A, Xp, Q, m, n, n, lda, n, n, tb=1, tri=8)
+    hold += [gt, lt, xt, tt, st, dt_, qt]
+    return Q, Lp
+
+
+def cholqr2_table(A, m, n, lda, hold):
+    """Reduced QR A = Q R of many tall matrices by two CholeskyQR passes;
+    returns (Q (m x n, ld n), R (n x n upper, ld n)) addresses."""
+    Q1, L1 = cholqr_table(A, m, n, lda, hold)
+    Q, L2 = cholqr_table(Q1, m, n, n, hold)
+    rt_, R = arena(n * n)
+    # R = R2 R1 = L2^T L1^T, both upper
+    gemm_table(L2, L1, R, n, n, n, n, n, n, ta=1, tb=1, tri=4 | 8)
+    hold.append(rt_)
+    return Q, R
 
 
 def truncate_table(s, g, chi_max, cutoff, keep, disc):
@@ -396,24 +489,33 @@ class ChainBatch:
             wide = m <= r
             tall = ~wide
             new_r = np.where(wide, m, r)
-            # Q factors: identity for wide unfoldings, Householder for tall
-            qt, q = arena(m * new_r)
+            # Q factors: identity for wide unfoldings, Householder or
+            # CholeskyQR2 (which brings its own Q buffer) for tall ones
+            cq = tall & (r >= CHOLQR_MIN_COLS)
+            qt, q = arena(np.where(cq, 0, m * new_r))
             if wide.any():
                 wi = wide.nonzero()[0]
                 eye_table(q[wi], m[wi])
             S = ptr[a, k].copy()          # wide: S = L itself (m x r, ld r)
             lds = r.copy()
-            if tall.any():
-                ti = tall.nonzero()[0]
+            hh = tall & (r < CHOLQR_MIN_COLS)
+            if hh.any():
+                # Householder, one work-group per matrix; S = Q^T L  (r x r)
+                ti = hh.nonzero()[0]
                 mt, rt = m[ti], r[ti]
                 wk, wp = arena(mt * rt)
                 tk, tp = arena(rt)
                 qr_table(ptr[a[ti], k], mt, rt, q[ti], wp, tp)
                 st, sp = arena(rt * rt)
-                # S = Q^T L  (r x r)
                 gemm_table(q[ti], ptr[a[ti], k], sp, rt, rt, mt, rt, rt, rt, ta=1)
                 S[ti] = sp
                 hold += [wk, tk, st]
+            if cq.any():
+                # CholeskyQR2 (grouped GEMMs + batched Cholesky leaves)
+                ti = cq.nonzero()[0]
+                Qc, Rc = cholqr2_table(ptr[a[ti], k], m[ti], r[ti], r[ti], hold)
+                q[ti] = Qc
+                S[ti] = Rc
             # absorb: next = S @ core_{k+1}  (new_r x d' b'')
             dn, bn = self.phys[a, k + 1], bond[a, k + 2]
             ncols = dn * bn

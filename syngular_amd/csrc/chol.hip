@@ -51,9 +51,9 @@ __global__ __launch_bounds__(256) void maxdiag_kernel(const double *g, int64_t n
 // zeroed into zb (nb x nb, ld nb).  L~11 equals L11 with every dropped row
 // replaced by the unit row, so that a panel solve X L11^T = B over the kept
 // columns is X = B Z^T with zero columns at the dropped positions.
-__global__ __launch_bounds__(256) void chol_diag_kernel(const double *G, int64_t ldg, double *L, int64_t ldl,
-                                                        int64_t n, int64_t j0, double *zb, int64_t ldz,
-                                                        const double *maxdiag, double tolrel, int *dropped) {
+__device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, double *L, int64_t ldl, int64_t n,
+                                               int64_t j0, double *zb, int64_t ldz, const double *maxdiag,
+                                               double tolrel, int *dropped) {
     __shared__ double a[kNb][kNb + 1];   // Schur complements (lower triangle)
     __shared__ double lo[kNb][kNb + 1];  // L, then L~ (dropped rows -> unit rows)
     __shared__ int keep[kNb];
@@ -177,6 +177,32 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double *G, int64_t
             }
         publish(k + 1);
         __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void chol_diag_kernel(const double *G, int64_t ldg, double *L, int64_t ldl,
+                                                        int64_t n, int64_t j0, double *zb, int64_t ldz,
+                                                        const double *maxdiag, double tolrel, int *dropped) {
+    chol_diag_body(G, ldg, L, ldl, n, j0, zb, ldz, maxdiag, tolrel, dropped);
+}
+
+// One leaf (n <= 64) per work-group, for many problems at once.
+__global__ __launch_bounds__(256) void chol_leaf_many_kernel(const syn_chol_item *__restrict__ items) {
+    const syn_chol_item &P = items[blockIdx.x];
+    chol_diag_body(P.g, P.ldg, P.l, P.ldl, P.n, 0, P.x, P.ldx, P.maxdiag, P.tolrel, P.dropped);
+}
+
+__global__ __launch_bounds__(256) void diagmax_many_kernel(const syn_diagmax_item *__restrict__ items) {
+    const syn_diagmax_item &P = items[blockIdx.x];
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < P.n; i += 256) m = fmax(m, P.g[i * P.ld + i]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *P.out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        *P.dropped = 0;
     }
 }
 
@@ -304,4 +330,30 @@ extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, doub
     if ((rc = chol_inv(G, n, L, ldl, X, n, n, T, scal, dropped, tolrel, s))) return rc;
     // Q = A X^T  (X = L^{-1} with dropped columns zeroed; op(B) = X^T upper)
     return gemm_f64_launch(make_gemm(A, lda, 0, X, n, 1, Q, ldq, m, n, n, 1.0, 0.0, 8), s);
+}
+
+extern "C" int syn_chol_leaf_f64(const syn_chol_item *items, int n_items, void *stream) {
+    using namespace syn;
+    if (n_items < 0 || (n_items && !items)) {
+        set_error("syn_chol_leaf_f64: bad arguments");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0) return SYN_OK;
+    hipStream_t s = (hipStream_t)stream;
+    ProfScope ps(kTagCholDiag, 0.0, s);
+    hipLaunchKernelGGL(chol_leaf_many_kernel, dim3(n_items), dim3(256), 0, s, items);
+    SYN_LAUNCH_CHECK("syn_chol_leaf_f64 launch");
+    return SYN_OK;
+}
+
+extern "C" int syn_diagmax_f64(const syn_diagmax_item *items, int n_items, void *stream) {
+    using namespace syn;
+    if (n_items < 0 || (n_items && !items)) {
+        set_error("syn_diagmax_f64: bad arguments");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0) return SYN_OK;
+    hipLaunchKernelGGL(diagmax_many_kernel, dim3(n_items), dim3(256), 0, (hipStream_t)stream, items);
+    SYN_LAUNCH_CHECK("syn_diagmax_f64 launch");
+    return SYN_OK;
 }

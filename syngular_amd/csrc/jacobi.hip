@@ -185,7 +185,162 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const syn_jacobi_it
         jacobi_one<false>(P, lds, tol, max_sweeps);
 }
 
+// ---------------------------------------------------------------------------
+// Two-sided (symmetric) Jacobi for Gram matrices in LDS.
+//
+// G (g x g, symmetric) and the accumulated rotations R = V^T stay in LDS.  A
+// round of the round-robin order rotates g/2 disjoint planes (p, q): every
+// angle comes from the 2 x 2 diagonal block [G_pp G_pq; G_pq G_qq] alone
+// (Rutishauser's formula, as in the one-sided kernel), so no dot products or
+// reductions are needed.  The update G <- J^T G J is applied as independent
+// 2 x 2 block updates (rows of plane a, columns of plane b: both rotations at
+// once), and R <- J^T R row pair by row pair; a round costs two barriers.
+// Convergence: no plane with |G_pq| > tol * sqrt(|G_pp G_qq|) in a sweep.
+// On exit the eigenvalues lambda_i = G_ii are sorted decreasingly; s_i =
+// sqrt(max(lambda_i, 0)) are the singular values of C when G = C^T C, and the
+// rows of rt are the matching eigenvectors.
+// ---------------------------------------------------------------------------
+constexpr int kGramThreads = 256;
+constexpr int kGramMax = 64;
+
+__global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jacobi_item *__restrict__ items,
+                                                                   int nv_max, double tol, int max_sweeps) {
+    extern __shared__ double lds[];
+    __shared__ double cs[kGramMax / 2][2];
+    __shared__ int pp[kGramMax / 2], qq[kGramMax / 2];
+    __shared__ int rank[kGramMax];
+    __shared__ int rotated;
+    const syn_jacobi_item &P = items[blockIdx.x];
+    const int tid = threadIdx.x;
+    if (P.nv < 1 || P.nv > nv_max || P.len != P.nv) {
+        // not a problem this kernel takes: flag it in s and leave the rest
+        for (int64_t i = tid; i < P.nv; i += kGramThreads) P.s[i] = __builtin_nan("");
+        return;
+    }
+    const int g = (int)P.nv, ld = g + 1;
+    double *A = lds;
+    double *R = lds + g * ld;
+    for (int e = tid; e < g * g; e += kGramThreads) {
+        const int i = e / g, j = e - i * g;
+        A[i * ld + j] = P.x[i * P.ldx + j];
+        R[i * ld + j] = (i == j) ? 1.0 : 0.0;
+    }
+    const int np = g + (g & 1), half = np / 2;
+    int sweep = 0;
+    for (; sweep < max_sweeps && g > 1; ++sweep) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int r = 0; r < np - 1; ++r) {
+            if (tid < half) {
+                int a, b;
+                if (tid == 0) {
+                    a = r;
+                    b = np - 1;
+                } else {
+                    a = (r + tid) % (np - 1);
+                    b = (r - tid + np - 1) % (np - 1);
+                }
+                int p = a < b ? a : b, q = a < b ? b : a;
+                double c = 1.0, sn = 0.0;
+                if (q < g) {
+                    const double apq = A[p * ld + q], app = A[p * ld + p], aqq = A[q * ld + q];
+                    if (fabs(apq) > tol * sqrt(fabs(app) * fabs(aqq))) {
+                        const double zeta = (aqq - app) / (2.0 * apq);
+                        const double t = copysign(1.0, zeta) / (fabs(zeta) + hypot(1.0, zeta));
+                        c = 1.0 / sqrt(fma(t, t, 1.0));
+                        sn = c * t;
+                        rotated = 1;
+                    }
+                } else {
+                    q = -1;  // p is paired with the padding slot
+                }
+                cs[tid][0] = c;
+                cs[tid][1] = sn;
+                pp[tid] = p;
+                qq[tid] = q;
+            }
+            __syncthreads();
+            for (int e = tid; e < half * half; e += kGramThreads) {
+                const int ka = e / half, kb = e - ka * half;
+                const double ca = cs[ka][0], sa = cs[ka][1], cb = cs[kb][0], sb = cs[kb][1];
+                if (sa == 0.0 && sb == 0.0) continue;
+                const int p = pp[ka], q = qq[ka], u = pp[kb], v = qq[kb];
+                const bool r2 = q >= 0, c2 = v >= 0;
+                const double apu = A[p * ld + u], apv = c2 ? A[p * ld + v] : 0.0;
+                const double aqu = r2 ? A[q * ld + u] : 0.0, aqv = (r2 && c2) ? A[q * ld + v] : 0.0;
+                const double rpu = ca * apu - sa * aqu, rpv = ca * apv - sa * aqv;
+                const double rqu = sa * apu + ca * aqu, rqv = sa * apv + ca * aqv;
+                A[p * ld + u] = cb * rpu - sb * rpv;
+                if (c2) A[p * ld + v] = sb * rpu + cb * rpv;
+                if (r2) {
+                    A[q * ld + u] = cb * rqu - sb * rqv;
+                    if (c2) A[q * ld + v] = sb * rqu + cb * rqv;
+                }
+                if (ka == kb && sa != 0.0) {  // the annihilated pair
+                    A[p * ld + q] = 0.0;
+                    A[q * ld + p] = 0.0;
+                }
+            }
+            for (int e = tid; e < half * g; e += kGramThreads) {
+                const int k = e / g, i = e - k * g;
+                const double sn = cs[k][1];
+                if (sn != 0.0) {
+                    const double c = cs[k][0];
+                    const int p = pp[k], q = qq[k];
+                    const double u = R[p * ld + i], v = R[q * ld + i];
+                    R[p * ld + i] = c * u - sn * v;
+                    R[q * ld + i] = sn * u + c * v;
+                }
+            }
+            __syncthreads();
+        }
+        const int any = rotated;
+        __syncthreads();
+        if (!any) break;
+    }
+    __syncthreads();
+    // decreasing order by eigenvalue (ties by index): rank of each row
+    for (int i = tid; i < g; i += kGramThreads) {
+        const double li = A[i * ld + i];
+        int rk = 0;
+        for (int j = 0; j < g; ++j) {
+            const double lj = A[j * ld + j];
+            rk += (lj > li) || (lj == li && j < i);
+        }
+        rank[i] = rk;
+        P.s[rk] = sqrt(fmax(li, 0.0));
+    }
+    __syncthreads();
+    for (int e = tid; e < g * g; e += kGramThreads) {
+        const int i = e / g, j = e - i * g;
+        P.rt[(int64_t)rank[i] * P.ldr + j] = R[i * ld + j];
+    }
+    if (tid == 0 && P.sweeps) *P.sweeps = sweep;
+}
+
 }  // namespace syn
+
+extern "C" int syn_jacobi_gram_f64(const syn_jacobi_item *items, int n_items, int64_t nv_max, double tol,
+                                   int max_sweeps, void *stream) {
+    using namespace syn;
+    if (n_items < 0 || nv_max < 1 || nv_max > kGramMax || max_sweeps < 0 || !(tol >= 0.0)) {
+        set_error("syn_jacobi_gram_f64: bad arguments (nv_max=%lld, limit %d)", (long long)nv_max, kGramMax);
+        return SYN_EINVAL;
+    }
+    if (n_items == 0) return SYN_OK;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)jacobi_gram_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(2 * kGramMax * (kGramMax + 1) * sizeof(double)));
+        attr_set = true;
+    }
+    const size_t shm = 2 * (size_t)nv_max * (size_t)(nv_max + 1) * sizeof(double);
+    ProfScope ps(kTagJacobi, 0.0, (hipStream_t)stream);
+    hipLaunchKernelGGL(jacobi_gram_kernel, dim3(n_items), dim3(kGramThreads), shm, (hipStream_t)stream, items,
+                       (int)nv_max, tol, max_sweeps);
+    SYN_LAUNCH_CHECK("syn_jacobi_gram_f64 launch");
+    return SYN_OK;
+}
 
 extern "C" int syn_jacobi_f64(const syn_jacobi_item *items, int n_items, double tol, int max_sweeps,
                               void *stream) {

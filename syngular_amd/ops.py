@@ -340,6 +340,41 @@ def jacobi_many(mats, tol=None, max_sweeps=JACOBI_SWEEPS, gram=False):
     return outs
 
 
+GRAM_LDS_MAX = 64
+
+
+def jacobi_gram_many(mats, tol=None, max_sweeps=JACOBI_SWEEPS):
+    """Two-sided Jacobi eigen-decomposition of symmetric Gram matrices
+    (nv x nv, nv <= 64) in LDS.  Returns per problem (rt (nv x nv), s (nv,))
+    with rt rows the eigenvectors by decreasing eigenvalue and s the square
+    roots of the eigenvalues (the singular values of C for G = C^T C).  The
+    inputs are not modified."""
+    if tol is None:
+        tol = jacobi_tol(GRAM_LDS_MAX)
+    items = np.zeros(len(mats), dtype=_lib.JACOBI_ITEM)
+    outs = []
+    nv_max = 1
+    for i, G in enumerate(mats):
+        nv, ln = G.shape
+        if nv != ln or nv > GRAM_LDS_MAX:
+            raise ValueError(f"jacobi_gram_many takes square matrices up to {GRAM_LDS_MAX}, got {tuple(G.shape)}")
+        px, ldx, tr = as_matrix(G)
+        if tr:
+            raise ValueError("jacobi_gram_many needs a row-major matrix")
+        rt = torch.empty((nv, nv), dtype=F64, device=G.device)
+        s = torch.empty(nv, dtype=F64, device=G.device)
+        outs.append((rt, s))
+        nv_max = max(nv_max, nv)
+        it = items[i]
+        it["x"], it["ldx"], it["nv"], it["len"] = px, ldx, nv, nv
+        it["rt"], it["ldr"], it["s"], it["sweeps"] = _ptr(rt), nv, _ptr(s), 0
+        it["gram"] = 1
+    if len(mats):
+        dev = _upload(items)
+        call("syn_jacobi_gram_f64", _ptr(dev), len(mats), nv_max, float(tol), int(max_sweeps), _stream())
+    return outs
+
+
 def truncate_many(problems):
     """problems: list of (s sorted desc, chi_max or None, cutoff).  Returns
     (keep int32 (n,), discarded f64 (n,)) device tensors."""

@@ -41,7 +41,8 @@ class RoundingReport:
 # MFMA GEMM work); smaller ones the one-work-group Householder kernel.
 CHOLQR_MIN_COLS = 96
 # Right-sweep Gram matrices up to this size are diagonalised in LDS by one
-# work-group; larger ones by the block Jacobi spread over many work-groups.
+# work-group (two-sided Jacobi); larger ones by the block one-sided Jacobi
+# spread over many work-groups.
 LDS_JACOBI_MAX = 64
 
 
@@ -89,7 +90,7 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None):
         else:
             G = ops.gemm(C, C.t())       # (m x m) = C C^T
         if G.shape[0] <= LDS_JACOBI_MAX:
-            rt, s, _ = ops.jacobi_many([G], gram=True)[0]
+            rt, s = ops.jacobi_gram_many([G])[0]
         else:
             rt, s, _ = ops.jacobi_blocked(G, gram=True)
         keep, disc = ops.truncate_many([(s, chi_max, cutoff)])

@@ -204,6 +204,21 @@ class Emulator:
                 self.buf(r["sweeps"], 1, np.int32, "jacobi sweeps")[0] = 1
         return 0
 
+    def syn_jacobi_gram_f64(self, items, n, nv_max, tol, max_sweeps, stream):
+        from syngular_amd import _lib
+        for r in self.table(items, n, _lib.JACOBI_ITEM):
+            nv = int(r["nv"])
+            if nv > nv_max or nv > 64 or int(r["len"]) != nv:
+                raise BoundsError(f"gram jacobi item of size {nv} (nv_max {nv_max})")
+            X = self.mat(r["x"], nv, nv, int(r["ldx"]), 0, "gram x")
+            R = self.mat(r["rt"], nv, nv, int(r["ldr"]), 0, "gram rt")
+            S = self.buf(r["s"], nv, what="gram s")
+            w, V = np.linalg.eigh(0.5 * (X + X.T))
+            order = np.argsort(-w, kind="stable")
+            R[...] = V[:, order].T
+            S[:] = np.sqrt(w[order].clip(min=0))
+        return 0
+
     def syn_jacobi_blocked_many_f64(self, items_dev, items_host, n, work, counters, tol, max_sweeps, sweeps, stream):
         from syngular_amd import _lib
         self.buf(items_dev, n * _lib.BJAC_ITEM.itemsize, np.uint8, "bjac device table")
@@ -219,6 +234,49 @@ class Emulator:
                              int(r["gram"]))
         if sweeps is not None and hasattr(sweeps, "_obj"):
             sweeps._obj.value = 1
+        return 0
+
+    def syn_diagmax_f64(self, items, n, stream):
+        from syngular_amd import _lib
+        for r in self.table(items, n, _lib.DIAGMAX_ITEM):
+            nn, ld = int(r["n"]), int(r["ld"])
+            G = self.mat(r["g"], nn, nn, ld, 0, "diagmax g")
+            self.buf(r["out"], 1, what="diagmax out")[0] = float(np.max(np.diag(G)))
+            self.buf(r["dropped"], 1, np.int32, "diagmax dropped")[0] = 0
+        return 0
+
+    def syn_chol_leaf_f64(self, items, n, stream):
+        """Right-looking Cholesky with the pivot-drop rule, then inv(L~)."""
+        from syngular_amd import _lib
+        for r in self.table(items, n, _lib.CHOL_ITEM):
+            nn = int(r["n"])
+            if not 1 <= nn <= 64:
+                raise BoundsError(f"chol leaf of size {nn}")
+            G = self.mat(r["g"], nn, nn, int(r["ldg"]), 0, "chol g")
+            Lo = self.mat(r["l"], nn, nn, int(r["ldl"]), 0, "chol l")
+            Xo = self.mat(r["x"], nn, nn, int(r["ldx"]), 0, "chol x")
+            thr = float(r["tolrel"]) * float(self.buf(r["maxdiag"], 1, what="chol maxdiag")[0])
+            a = np.tril(G) + np.tril(G, -1).T
+            L = np.zeros((nn, nn))
+            keep = np.ones(nn, bool)
+            for j in range(nn):
+                d = a[j, j]
+                if d > thr:
+                    sd = np.sqrt(d)
+                    L[j, j] = sd
+                    L[j + 1:, j] = a[j + 1:, j] / sd
+                    a[j + 1:, j + 1:] -= np.outer(a[j + 1:, j], a[j + 1:, j]) / d
+                else:
+                    keep[j] = False
+            Lt = L.copy()
+            for i in np.nonzero(~keep)[0]:
+                Lt[i, :] = 0.0
+                Lt[i, i] = 1.0
+            Z = np.linalg.inv(Lt)
+            Z[:, ~keep] = 0.0
+            Lo[...] = L
+            Xo[...] = np.tril(Z)
+            self.buf(r["dropped"], 1, np.int32, "chol dropped")[0] += int((~keep).sum())
         return 0
 
     def syn_truncate_f64(self, items, n, stream):

@@ -31,7 +31,7 @@ def _mpo(rng, n, D, d=2):
     return [rng.standard_normal((w[k], d, d, w[k + 1])) / np.sqrt(D * d) for k in range(n)]
 
 
-SPEC = [(3, 2, 2), (5, 4, 3), (7, 8, 4), (4, 16, 2), (9, 6, 4), (14, 40, 3), (2, 5, 4), (8, 12, 1), (1, 3, 2)]
+SPEC = [(3, 2, 2), (5, 4, 3), (7, 8, 4), (4, 16, 2), (9, 6, 4), (14, 40, 3), (2, 5, 4), (8, 12, 1), (1, 3, 2), (10, 100, 2)]
 
 
 def test_emulated_batched_round_matches_oracle(emu):
@@ -42,7 +42,8 @@ def test_emulated_batched_round_matches_oracle(emu):
     recs = []
     Z = B.apply_and_round(X, O, np.array([j[2] for j in jobs]), 0.0, records=recs)
     out = Z.to_numpy()
-    assert "syn_jacobi_blocked_many_f64" in emu.calls and "syn_qr_f64" in emu.calls and "syn_eye_f64" in emu.calls
+    for fn in ("syn_jacobi_blocked_many_f64", "syn_qr_f64", "syn_eye_f64", "syn_chol_leaf_f64", "syn_jacobi_gram_f64"):
+        assert fn in emu.calls
     for q, (xs, os_, c) in enumerate(jobs):
         ref, sv, kept, disc = ornd.apply_and_round(os_, xs, chi_max=c)
         assert [o.shape for o in out[q]] == [r.shape for r in ref]
@@ -143,3 +144,34 @@ def test_emulated_server_longest_job_leaves_first(emu):
             mpo = W.gate_mpo_cores(j["n"], l % 2, W.layer_gates(j["seed"], j["id"], l, j["n"]))
             cores = ornd.apply_and_round(mpo, cores, chi_max=j["chi_max"])[0]
         assert np.abs(oref.dense_mps(r.cores) - oref.dense_mps(cores)).max() <= TOL
+
+
+def test_emulated_cholqr2_table(emu):
+    """Batched CholeskyQR2 over uneven sizes (leaf, one split, nested splits,
+    a rank-deficient matrix): Q orthonormal on the kept columns, Q R = A."""
+    rng = np.random.default_rng(5)
+    shapes = [(80, 40), (300, 130), (700, 200), (600, 257), (400, 96)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    mats[4][:, 50:] = mats[4][:, :46] @ rng.standard_normal((46, 46))   # rank 50
+    dev = [B._h2d(torch_from(a)) for a in mats]
+    m = np.array([s[0] for s in shapes])
+    n = np.array([s[1] for s in shapes])
+    hold = []
+    Q, R = B.cholqr2_table(np.array([t.data_ptr() for t in dev], np.uint64), m, n, n, hold)
+    for q, (mm, nn) in enumerate(shapes):
+        Qh = emu.buf(Q[q], mm * nn).reshape(mm, nn)
+        Rh = emu.buf(R[q], nn * nn).reshape(nn, nn)
+        assert np.abs(np.tril(Rh, -1)).max() == 0.0
+        assert np.abs(Qh @ Rh - mats[q]).max() <= 1e-10 * np.abs(mats[q]).max()
+        keep = np.abs(Qh).sum(axis=0) > 0
+        if q < 4:
+            assert keep.all()
+        else:
+            assert keep.sum() == 50
+        Qk = Qh[:, keep]
+        assert np.abs(Qk.T @ Qk - np.eye(keep.sum())).max() <= 1e-12
+
+
+def torch_from(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a))

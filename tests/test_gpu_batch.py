@@ -30,7 +30,7 @@ def _random_mpo(rng, n, D, d=2):
 
 def _mixed_jobs(seed=0):
     rng = np.random.default_rng(seed)
-    spec = [(3, 2, 2), (5, 4, 3), (7, 8, 4), (4, 16, 2), (9, 6, 4), (14, 40, 3), (2, 5, 4), (8, 12, 1)]
+    spec = [(3, 2, 2), (5, 4, 3), (7, 8, 4), (4, 16, 2), (9, 6, 4), (14, 40, 3), (2, 5, 4), (8, 12, 1), (10, 100, 2)]
     jobs = []
     for n, chi, D in spec:
         jobs.append((_random_chain(rng, n, chi), _random_mpo(rng, n, D), chi))

@@ -264,3 +264,33 @@ def test_gemm_structural_zero_hints(tri, n):
     if tri & 1:
         got, ref = np.tril(got), np.tril(ref)
     np.testing.assert_allclose(got, ref, atol=1e-12 * n)
+
+
+@pytest.mark.parametrize("g,rank", [(1, 1), (2, 2), (7, 7), (33, 20), (64, 64), (64, 10)])
+def test_jacobi_gram_two_sided(g, rank):
+    """Two-sided LDS Jacobi of C^T C: eigenvalues = squared singular values of
+    C, rows of rt orthonormal eigenvectors sorted decreasingly, input kept."""
+    from syngular_amd import ops
+
+    case(f"jacobi gram two-sided g={g} rank={rank}")
+    rng = np.random.default_rng(g * 100 + rank)
+    C = rng.standard_normal((g + 5, rank)) @ rng.standard_normal((rank, g))
+    G0 = C.T @ C
+    G = T(G0)
+    (rt, s), = ops.jacobi_gram_many([G])
+    s = s.cpu().numpy()
+    R = rt.cpu().numpy()
+    ref = np.linalg.svd(C, compute_uv=False)
+    assert np.all(np.diff(s) <= 0)
+    np.testing.assert_allclose(s[:rank], ref[:rank], rtol=1e-10, atol=1e-12 * ref[0])
+    assert np.all(s[rank:] <= 1e-6 * ref[0])
+    np.testing.assert_allclose(R @ R.T, np.eye(g), atol=1e-12)
+    np.testing.assert_allclose(R.T @ np.diag(s ** 2) @ R, G0, atol=1e-11 * ref[0] ** 2)
+    np.testing.assert_array_equal(G.cpu().numpy(), G0)
+
+
+def test_jacobi_gram_rejects_oversize():
+    from syngular_amd import ops
+
+    with pytest.raises(ValueError):
+        ops.jacobi_gram_many([T(np.eye(65))])
