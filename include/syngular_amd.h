@@ -45,7 +45,7 @@ int syn_version(void);
 int syn_device_count(int *count);
 /* ABI self-check: writes sizeof() of the item structs below, in declaration
  * order (gemm, apply, mpo_mpo, dsum, qr, jacobi, trunc, rowscale, bjac,
- * eye, copy, diagmax, chol), into out[0..n-1]; returns the number of structs. */
+ * eye, copy, diagmax, chol, gbj), into out[0..n-1]; returns the number of structs. */
 int syn_struct_sizes(int64_t *out, int n);
 
 /* ------------------------------------------------------------------ */
@@ -232,6 +232,37 @@ int syn_jacobi_blocked_many_f64(syn_bjac_item *items, const syn_bjac_item *items
 /* work size in doubles for problems (nv[i], len[i]) */
 int64_t syn_jacobi_blocked_many_workspace(const int64_t *nv, const int64_t *len, int n_items);
 
+/* Two-sided block Jacobi eigen-decomposition of many symmetric Gram     */
+/* matrices of uneven size (nv > 64), for the right sweep of the SVD      */
+/* rounding.  Per problem: g (nv x nv, ldg) is diagonalised IN PLACE;      */
+/* on exit rt (nv x nv, ldr) holds the eigenvectors as rows sorted by     */
+/* decreasing eigenvalue (G_in = rt^T diag(lambda) rt) and s[i] =          */
+/* sqrt(max(lambda_i, 0)).  Blocks of 16 rows; every step is one launch   */
+/* of pair solves (LDS, one wave each) and one launch of MFMA tile         */
+/* updates covering all problems still rotating.  `items` is a DEVICE     */
+/* table of n_items entries and `items_host` the same table on the host;  */
+/* w, solve_start, tile_start, counter and active are filled by the       */
+/* library.  work: syn_jacobi_gram_blocked_workspace(nv, n) doubles;      */
+/* counters: n_items ints; *sweeps (host, may be null) gets the sweeps.   */
+/* No reference call site (north-star addition: SVD rounding).           */
+typedef struct {
+    double *g;
+    int64_t ldg;
+    double *rt;
+    int64_t ldr;
+    double *s;
+    double *w;
+    int64_t nv;
+    int64_t solve_start, tile_start;
+    int32_t *counter;
+    int32_t active;
+} syn_gbj_item;
+int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_host, int n_items, double *work,
+                                int32_t *counters, double tol, int max_sweeps, int *sweeps, void *stream);
+int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items);
+/* inner Jacobi sweeps per pair solve (1..8, default 1) */
+void syn_tune_gram_inner_sweeps(int sweeps);
+
 /* Batched copy: dst[0..n) = src[0..n) for every item (compaction). */
 typedef struct {
     const double *src;
@@ -334,7 +365,8 @@ int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_
 /* launches, milliseconds and algorithmic flops (arrays of ntags) and    */
 /* returns the number of tags.  Tags: 0/1 GEMM by value 64/128 tiles,     */
 /* 2/3 grouped GEMM 64/128 tiles, 4 block-Jacobi step, 5 Cholesky        */
-/* diagonal block, 6 Householder QR, 7 LDS Jacobi, 8 MPO x MPS apply.    */
+/* diagonal block, 6 Householder QR, 7 LDS Jacobi, 8 MPO x MPS apply,    */
+/* 9/10 Gram block-Jacobi pair solve / tile update.                      */
 /* syn_prof_note_flops sets the count attached to the next grouped GEMM */
 /* or apply launch (their tables live on the device).                    */
 /* ------------------------------------------------------------------ */

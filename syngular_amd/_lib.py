@@ -61,8 +61,12 @@ CHOL_ITEM = np.dtype(
     [("g", _P), ("ldg", _I), ("l", _P), ("ldl", _I), ("x", _P), ("ldx", _I), ("n", _I), ("maxdiag", _P),
      ("tolrel", np.float64), ("dropped", _P)], align=True)
 
+GBJ_ITEM = np.dtype(
+    [("g", _P), ("ldg", _I), ("rt", _P), ("ldr", _I), ("s", _P), ("w", _P), ("nv", _I), ("solve_start", _I),
+     ("tile_start", _I), ("counter", _P), ("active", np.int32)], align=True)
+
 ITEM_DTYPES = [GEMM_ITEM, APPLY_ITEM, MPO_MPO_ITEM, DSUM_ITEM, QR_ITEM, JACOBI_ITEM, TRUNC_ITEM, ROWSCALE_ITEM,
-               BJAC_ITEM, EYE_ITEM, COPY_ITEM, DIAGMAX_ITEM, CHOL_ITEM]
+               BJAC_ITEM, EYE_ITEM, COPY_ITEM, DIAGMAX_ITEM, CHOL_ITEM, GBJ_ITEM]
 
 # every symbol the header declares, with its ctypes signature
 _VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -92,6 +96,9 @@ SIGNATURES = {
     "syn_eye_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_copy_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_diagmax_f64": (_I32, [_VP, _I32, _VP]),
+    "syn_jacobi_gram_blocked_f64": (_I32, [_VP, _VP, _I32, _VP, _VP, _D, _I32, ctypes.POINTER(ctypes.c_int), _VP]),
+    "syn_jacobi_gram_blocked_workspace": (_I64, [_VP, _I32]),
+    "syn_tune_gram_inner_sweeps": (None, [_I32]),
     "syn_chol_leaf_f64": (_I32, [_VP, _I32, _VP]),
     "syn_prof_begin": (_I32, []),
     "syn_prof_note_flops": (_I32, [_D]),
@@ -112,6 +119,8 @@ PROF_TAGS = {
     6: "qr_kernel",
     7: "jacobi_kernel",
     8: "apply_kernel",
+    9: "gram_solve_kernel",
+    10: "gram_update_kernel",
 }
 
 

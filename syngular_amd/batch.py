@@ -310,6 +310,26 @@ def cholqr2_table(A, m, n, lda, hold):
     return Q, R
 
 
+def jacobi_gram_blocked_table(x, g, rt, s):
+    """Two-sided block Jacobi of many symmetric g x g Gram matrices (g > 64),
+    one batched call; x is diagonalised in place."""
+    cnt = len(x)
+    if cnt == 0:
+        return 0
+    it = np.zeros(cnt, dtype=_lib.GBJ_ITEM)
+    it["g"], it["ldg"], it["rt"], it["ldr"], it["s"], it["nv"] = x, g, rt, g, s, g
+    g = np.ascontiguousarray(g, dtype=np.int64)
+    lib = _lib.load()
+    wsz = int(lib.syn_jacobi_gram_blocked_workspace(g.ctypes.data_as(ctypes.c_void_p), cnt))
+    work = _empty(wsz)
+    counters = _empty(cnt, torch.int32, zero=True)
+    dev_items = _empty(it.nbytes, torch.uint8)
+    sweeps = ctypes.c_int(0)
+    call("syn_jacobi_gram_blocked_f64", dev_items.data_ptr(), it.ctypes.data_as(ctypes.c_void_p), cnt,
+         work.data_ptr(), counters.data_ptr(), TOL_BLOCKED, ops.JACOBI_SWEEPS, ctypes.byref(sweeps), ops._stream())
+    return sweeps.value
+
+
 def truncate_table(s, g, chi_max, cutoff, keep, disc):
     it = np.zeros(len(s), dtype=_lib.TRUNC_ITEM)
     it["s"], it["n"], it["chi_max"], it["cutoff"] = s, g, chi_max, cutoff
@@ -557,7 +577,7 @@ class ChainBatch:
             if small.any():
                 jacobi_table(gp[small], g[small], rtp[small], sp[small])
             if (~small).any():
-                jacobi_blocked_table(gp[~small], g[~small], rtp[~small], sp[~small])
+                jacobi_gram_blocked_table(gp[~small], g[~small], rtp[~small], sp[~small])
             keep = _empty(len(a), torch.int32, zero=True)
             disc = _empty(len(a), F64, zero=True)
             kb, db = np.uint64(keep.data_ptr()), np.uint64(disc.data_ptr())

@@ -225,6 +225,17 @@ __global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jac
         A[i * ld + j] = P.x[i * P.ldx + j];
         R[i * ld + j] = (i == j) ? 1.0 : 0.0;
     }
+    __syncthreads();
+    // absolute rotation floor eps * max_i G_ii: entries below it are at the
+    // rounding level of forming G (a rank-deficient G has a null space of
+    // such entries, which a purely relative test would keep rotating)
+    __shared__ double floor_abs;
+    if (tid < 64) {
+        double m = 0.0;
+        for (int i = tid; i < g; i += 64) m = fmax(m, fabs(A[i * ld + i]));
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        if (tid == 0) floor_abs = __DBL_EPSILON__ * m;
+    }
     const int np = g + (g & 1), half = np / 2;
     int sweep = 0;
     for (; sweep < max_sweeps && g > 1; ++sweep) {
@@ -244,7 +255,7 @@ __global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jac
                 double c = 1.0, sn = 0.0;
                 if (q < g) {
                     const double apq = A[p * ld + q], app = A[p * ld + p], aqq = A[q * ld + q];
-                    if (fabs(apq) > tol * sqrt(fabs(app) * fabs(aqq))) {
+                    if (fabs(apq) > fmax(tol * sqrt(fabs(app) * fabs(aqq)), floor_abs)) {
                         const double zeta = (aqq - app) / (2.0 * apq);
                         const double t = copysign(1.0, zeta) / (fabs(zeta) + hypot(1.0, zeta));
                         c = 1.0 / sqrt(fma(t, t, 1.0));

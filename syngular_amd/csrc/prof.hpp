@@ -15,6 +15,8 @@ enum ProfTag {
     kTagQr = 6,
     kTagJacobi = 7,
     kTagApply = 8,
+    kTagGramSolve = 9,
+    kTagGramUpdate = 10,
 };
 
 bool prof_on();

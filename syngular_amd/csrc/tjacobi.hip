@@ -1,0 +1,508 @@
+// Two-sided block Jacobi eigen-decomposition of symmetric Gram matrices.
+//
+// For the right sweep of the SVD rounding the matrix to decompose is a Gram
+// matrix G = C^T C (or C C^T) of size g = 65 .. a few thousand.  The rows and
+// columns of G are split into blocks of kTB; a sweep pairs every two blocks
+// once in the round-robin order, and in one step the pairs are disjoint:
+//
+//   solve   one small work-group per pair: the kTP x kTP diagonal tile of the
+//           pair (rows and columns of both blocks) is loaded into LDS and
+//           rotated by cyclic two-sided Jacobi (the scheme of the LDS Gram
+//           kernel), accumulating the orthogonal W of the pair;
+//   update  one work-group per tile: G_ab <- W_a^T G_ab W_b for every pair of
+//           pairs (a, b) and rt_a <- W_a^T rt_a for every 64-column slab of
+//           the accumulated eigenvector rows, on the f64 MFMA.
+//
+// Both launches of a step spread over many work-groups (the update over
+// (g/kTP)^2 + (g/kTP)(g/64) tiles), and several problems share every launch.
+// A pair whose tile has no |G_ik| > max(tol sqrt(|G_ii G_kk|), eps max_j G_jj)
+// is not rotated; the sweep loop stops after a sweep in which no pair was
+// rotated, so on exit every off-diagonal entry of G meets that test, G is
+// diagonal to it and G_in = rt^T diag(G_out) rt.  This is the two-sided
+// block Jacobi method (Hansen; Van Loan; Becka, Oksa & Vajtersic) in its
+// parallel round-robin form.
+#include "common.hpp"
+#include "prof.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace syn {
+
+constexpr int kTB = 16;              // rows per block
+constexpr int kTP = 2 * kTB;         // rows of a pair = tile edge
+constexpr int kTSolveThreads = 256;  // one thread per 2 x 2 block of a round
+constexpr int kTUpdThreads = 256;    // four waves per tile update
+constexpr int kTCol = 64;            // columns of an rt slab
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void tj_pair(int64_t step, int64_t w, int64_t np, int64_t &a, int64_t &b) {
+    if (w == 0) {
+        a = step;
+        b = np - 1;
+    } else {
+        a = (step + w) % (np - 1);
+        b = (step - w + np - 1) % (np - 1);
+    }
+}
+
+// Matrix row of entry i (0 <= i < kTP) of pair w at a step, -1 if padding.
+__device__ __forceinline__ int tj_row(int64_t step, int64_t w, int64_t np, int64_t nblk, int64_t nv, int i) {
+    int64_t a, b;
+    tj_pair(step, w, np, a, b);
+    const int64_t blk = i < kTB ? a : b;
+    const int64_t r = blk * kTB + (i % kTB);
+    return (blk < nblk && r < nv) ? (int)r : -1;
+}
+
+// Plane k of round r in the round-robin order over the kTP indices of a
+// pair tile, as (smaller, larger) index.
+__device__ __forceinline__ void plane32(int r, int k, int &p, int &q) {
+    constexpr int m = kTP - 1;
+    int a, b;
+    if (k == 0) {
+        a = r;
+        b = m;
+    } else {
+        a = (r + k) % m;
+        b = (r - k + m) % m;
+    }
+    p = a < b ? a : b;
+    q = a < b ? b : a;
+}
+
+struct TjDims {
+    int64_t nblk, np, half, ncs;
+};
+
+__device__ __forceinline__ TjDims tj_dims(int64_t nv) {
+    TjDims d;
+    d.nblk = (nv + kTB - 1) / kTB;
+    d.np = d.nblk + (d.nblk & 1);
+    d.half = d.np / 2;
+    d.ncs = (nv + kTCol - 1) / kTCol;
+    return d;
+}
+
+template <bool SOLVE>
+__device__ __forceinline__ int tj_find(const syn_gbj_item *items, int n, int64_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t st = SOLVE ? items[mid].solve_start : items[mid].tile_start;
+        if (st <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int *tj_flags(const syn_gbj_item &P, int64_t half) {
+    return (int *)(P.w + half * kTP * kTP);
+}
+
+// Absolute rotation floor eps * max_i G_ii (set by the init kernel): Gram
+// entries below it are at the rounding level of forming G, so rotating them
+// changes nothing that the Gram route can resolve, and in a rank-deficient G
+// (a null space at noise level) a purely relative test would never settle.
+__device__ __forceinline__ double *tj_floor(const syn_gbj_item &P, int64_t half) {
+    return P.w + half * kTP * kTP + (half + 1) / 2;
+}
+
+__global__ __launch_bounds__(kTSolveThreads) void tj_solve_kernel(const syn_gbj_item *__restrict__ items, int n,
+                                                                 int64_t step, double tol, int inner_sweeps) {
+    __shared__ double S[kTP][kTP + 1];
+    __shared__ double W[kTP][kTP + 1];
+    __shared__ double cs[kTP / 2][2];
+    __shared__ int rows[kTP];
+    __shared__ int do_rot;
+    const int64_t bid = blockIdx.x;
+    const syn_gbj_item &P = items[tj_find<true>(items, n, bid)];
+    const TjDims D = tj_dims(P.nv);
+    const int64_t pw = bid - P.solve_start;
+    const int64_t st = step % (D.np - 1);
+    const int tid = threadIdx.x;
+    if (tid < kTP) rows[tid] = tj_row(st, pw, D.np, D.nblk, P.nv, tid);
+    if (tid == 0) do_rot = 0;
+    __syncthreads();
+    for (int e = tid; e < kTP * kTP; e += kTSolveThreads) {
+        const int i = e / kTP, k = e % kTP, ri = rows[i], rk = rows[k];
+        S[i][k] = (ri >= 0 && rk >= 0) ? P.g[(int64_t)ri * P.ldg + rk] : 0.0;
+        W[i][k] = (i == k) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const double floor_abs = *tj_floor(P, D.half);
+    for (int e = tid; e < kTP * kTP; e += kTSolveThreads) {
+        const int i = e / kTP, k = e % kTP;
+        if (i < k && fabs(S[i][k]) > fmax(tol * sqrt(fabs(S[i][i]) * fabs(S[k][k])), floor_abs)) do_rot = 1;
+    }
+    __syncthreads();
+    int *flags = tj_flags(P, D.half);
+    if (!do_rot) {
+        if (tid == 0) flags[pw] = 0;
+        return;
+    }
+    // thread (ka, kb) = (tid / 16, tid % 16) owns the 2 x 2 block of planes
+    // ka (rows) and kb (columns) of each round, and row wi of the columns of
+    // planes wk and wk + 8 of W
+    static_assert(kTSolveThreads == (kTP / 2) * (kTP / 2), "one thread per 2 x 2 block");
+    const int ka = tid / (kTP / 2), kb = tid % (kTP / 2);
+    const int wk = tid / kTP, wi = tid % kTP;  // W: planes wk and wk + 8, row wi
+    __shared__ int any_rot;
+    for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
+        if (tid == 0) any_rot = 0;
+        __syncthreads();
+        for (int r = 0; r < kTP - 1; ++r) {
+            int p, q, u, v;
+            plane32(r, ka, p, q);
+            plane32(r, kb, u, v);
+            if (tid < kTP / 2) {   // here ka == 0 and kb == tid: plane tid is (u, v)
+                const int p = u, q = v;
+                const double apq = S[p][q], app = S[p][p], aqq = S[q][q];
+                double c = 1.0, sn = 0.0;
+                if (fabs(apq) > fmax(tol * sqrt(fabs(app) * fabs(aqq)), floor_abs)) {
+                    const double zeta = (aqq - app) / (2.0 * apq);
+                    const double t = copysign(1.0, zeta) / (fabs(zeta) + hypot(1.0, zeta));
+                    c = 1.0 / sqrt(fma(t, t, 1.0));
+                    sn = c * t;
+                    any_rot = 1;
+                }
+                cs[tid][0] = c;
+                cs[tid][1] = sn;
+            }
+            __syncthreads();
+            {
+                const double ca = cs[ka][0], sa = cs[ka][1], cb = cs[kb][0], sb = cs[kb][1];
+                if (sa != 0.0 || sb != 0.0) {
+                    const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
+                    const double rpu = ca * spu - sa * squ, rpv = ca * spv - sa * sqv;
+                    const double rqu = sa * spu + ca * squ, rqv = sa * spv + ca * sqv;
+                    S[p][u] = cb * rpu - sb * rpv;
+                    S[p][v] = sb * rpu + cb * rpv;
+                    S[q][u] = cb * rqu - sb * rqv;
+                    S[q][v] = sb * rqu + cb * rqv;
+                    if (ka == kb) {  // the annihilated entry of this plane
+                        S[p][q] = 0.0;
+                        S[q][p] = 0.0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {  // W <- W J (columns of plane wk + 8h)
+                const int k = wk + 8 * h;
+                const double sn = cs[k][1];
+                if (sn != 0.0) {
+                    const double c = cs[k][0];
+                    int pp, qq;
+                    plane32(r, k, pp, qq);
+                    const double x = W[wi][pp], y = W[wi][qq];
+                    W[wi][pp] = c * x - sn * y;
+                    W[wi][qq] = sn * x + c * y;
+                }
+            }
+            __syncthreads();
+        }
+        // any_rot was last written before the final barrier of the sweep
+        const int any = any_rot;
+        __syncthreads();
+        if (!any) break;
+    }
+    double *Wout = P.w + pw * kTP * kTP;
+    for (int e = tid; e < kTP * kTP; e += kTSolveThreads) Wout[e] = W[e / kTP][e % kTP];
+    if (tid == 0) {
+        flags[pw] = 1;
+        atomicAdd(P.counter, 1);
+    }
+}
+
+// MFMA f64 16x16x4 layout: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
+// D reg r of lane l = D[l/16 + 4r][l%16].
+__global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_item *__restrict__ items, int n,
+                                                                  int64_t step) {
+    __shared__ double Wa[kTP][kTP + 1];
+    __shared__ double Wb[kTP][kTP + 1];
+    __shared__ double X[kTP][kTCol + 1];   // G tile (kTP columns used) or rt slab
+    __shared__ double Tt[kTP][kTP + 1];
+    __shared__ int ra[kTP], rb[kTP];
+    const int64_t bid = blockIdx.x;
+    const syn_gbj_item &P = items[tj_find<false>(items, n, bid)];
+    const TjDims D = tj_dims(P.nv);
+    const int64_t st = step % (D.np - 1);
+    const int64_t t = bid - P.tile_start;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int *flags = tj_flags(P, D.half);
+    if (t < D.half * D.half) {
+        // ---- G tile (a, b) -------------------------------------------------
+        const int64_t a = t / D.half, b = t % D.half;
+        const int fa = flags[a], fb = flags[b];
+        if (!fa && !fb) return;
+        if (tid < kTP) {
+            ra[tid] = tj_row(st, a, D.np, D.nblk, P.nv, tid);
+            rb[tid] = tj_row(st, b, D.np, D.nblk, P.nv, tid);
+        }
+        __syncthreads();
+        const double *wa = P.w + a * kTP * kTP, *wb = P.w + b * kTP * kTP;
+        for (int e = tid; e < kTP * kTP; e += kTUpdThreads) {
+            const int i = e / kTP, k = e % kTP;
+            Wa[i][k] = fa ? wa[e] : (i == k ? 1.0 : 0.0);
+            Wb[i][k] = fb ? wb[e] : (i == k ? 1.0 : 0.0);
+            const int r = ra[i], c = rb[k];
+            X[i][k] = (r >= 0 && c >= 0) ? P.g[(int64_t)r * P.ldg + c] : 0.0;
+        }
+        __syncthreads();
+        const int ti = w >> 1, tj = w & 1;
+        // T = Wa^T X
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < kTP; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc, 0, 0,
+                                                        0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Tt[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
+        __syncthreads();
+        // G_ab = T Wb
+        acc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < kTP; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tt[ti * 16 + li][k0 + lk], Wb[k0 + lk][tj * 16 + li], acc, 0, 0,
+                                                        0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = ti * 16 + lk + 4 * r, k = tj * 16 + li;
+            const int gr = ra[i], gc = rb[k];
+            if (gr >= 0 && gc >= 0) P.g[(int64_t)gr * P.ldg + gc] = acc[r];
+        }
+        return;
+    }
+    // ---- rt slab (a, columns [c0, c0 + kTCol)) ------------------------------
+    const int64_t u = t - D.half * D.half;
+    const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
+    if (!flags[a]) return;
+    if (tid < kTP) ra[tid] = tj_row(st, a, D.np, D.nblk, P.nv, tid);
+    __syncthreads();
+    const double *wa = P.w + a * kTP * kTP;
+    for (int e = tid; e < kTP * kTP; e += kTUpdThreads) Wa[e / kTP][e % kTP] = wa[e];
+    for (int e = tid; e < kTP * kTCol; e += kTUpdThreads) {
+        const int i = e / kTCol, j = e % kTCol, r = ra[i];
+        X[i][j] = (r >= 0 && c0 + j < P.nv) ? P.rt[(int64_t)r * P.ldr + c0 + j] : 0.0;
+    }
+    __syncthreads();
+    // rt_a = Wa^T X: 2 x 4 output tiles of 16 x 16, two per wave
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int tile = w * 2 + h, ti = tile >> 2, tj = tile & 3;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < kTP; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc, 0, 0,
+                                                        0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = ti * 16 + lk + 4 * r, j = tj * 16 + li;
+            const int gr = ra[i];
+            if (gr >= 0 && c0 + j < P.nv) P.rt[(int64_t)gr * P.ldr + c0 + j] = acc[r];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void tj_init_kernel(const syn_gbj_item *__restrict__ items) {
+    __shared__ double red[4];
+    const syn_gbj_item &P = items[blockIdx.x];
+    const int64_t nel = P.nv * P.nv;
+    for (int64_t e = threadIdx.x; e < nel; e += 256) {
+        const int64_t i = e / P.nv, j = e - i * P.nv;
+        P.rt[i * P.ldr + j] = (i == j) ? 1.0 : 0.0;
+    }
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < P.nv; i += 256) m = fmax(m, fabs(P.g[i * P.ldg + i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *tj_floor(P, tj_dims(P.nv).half) = __DBL_EPSILON__ * fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void tj_diag_kernel(const syn_gbj_item *__restrict__ items) {
+    const syn_gbj_item &P = items[blockIdx.x];
+    for (int64_t i = threadIdx.x; i < P.nv; i += 256) P.s[i] = sqrt(fmax(P.g[i * P.ldg + i], 0.0));
+}
+
+// rows of rt and entries of s in the order perm (per problem), through tmp
+__global__ __launch_bounds__(256) void tj_permute_kernel(const syn_gbj_item *__restrict__ items,
+                                                         const int64_t *__restrict__ perm,
+                                                         const int64_t *__restrict__ poff, double *tmp,
+                                                         const int64_t *__restrict__ toff, int pass) {
+    const syn_gbj_item &P = items[blockIdx.x];
+    const int64_t *pm = perm + poff[blockIdx.x];
+    double *tr = tmp + toff[blockIdx.x];
+    double *ts = tr + P.nv * P.nv;
+    const int64_t nr = P.nv * P.nv;
+    for (int64_t e = threadIdx.x; e < nr + P.nv; e += 256) {
+        if (e < nr) {
+            const int64_t i = e / P.nv, j = e - i * P.nv;
+            if (pass == 0)
+                tr[e] = P.rt[pm[i] * P.ldr + j];
+            else
+                P.rt[i * P.ldr + j] = tr[e];
+        } else {
+            const int64_t i = e - nr;
+            if (pass == 0)
+                ts[i] = P.s[pm[i]];
+            else
+                P.s[i] = ts[i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void tj_zero_kernel(int32_t *c, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) c[i] = 0;
+}
+
+static int64_t tj_half(int64_t nv) {
+    const int64_t nblk = (nv + kTB - 1) / kTB;
+    return (nblk + (nblk & 1)) / 2;
+}
+
+// per-problem workspace: W tiles, flags, then the sort scratch (rt copy, s)
+static int64_t tj_work(int64_t nv) {
+    const int64_t half = tj_half(nv);
+    return half * kTP * kTP + (half + 1) / 2 + 1 + nv * nv + nv + 4;
+}
+
+static int g_tj_inner = 1;
+
+}  // namespace syn
+
+extern "C" void syn_tune_gram_inner_sweeps(int sweeps) { syn::g_tj_inner = (sweeps >= 1 && sweeps <= 8) ? sweeps : 1; }
+
+extern "C" int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items) {
+    int64_t tot = 0;
+    for (int i = 0; i < n_items; ++i) tot += syn::tj_work(nv[i]) + 2 * nv[i];
+    return tot + 2 * (int64_t)n_items + 16;
+}
+
+extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_host, int n_items,
+                                           double *work, int32_t *counters, double tol, int max_sweeps, int *sweeps,
+                                           void *stream) {
+    using namespace syn;
+    if (n_items < 0 || !items || (n_items && (!items_host || !work || !counters)) || max_sweeps < 0 ||
+        !(tol >= 0.0)) {
+        set_error("syn_jacobi_gram_blocked_f64: bad arguments");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0) return SYN_OK;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<syn_gbj_item> all(items_host, items_host + n_items);
+    std::vector<int64_t> woff(n_items);
+    int64_t tot_w = 0;
+    for (int i = 0; i < n_items; ++i) {
+        if (all[i].nv <= 0 || all[i].ldg < all[i].nv || all[i].ldr < all[i].nv || !all[i].g || !all[i].rt ||
+            !all[i].s) {
+            set_error("syn_jacobi_gram_blocked_f64: bad problem %d (nv=%lld)", i, (long long)all[i].nv);
+            return SYN_EINVAL;
+        }
+        woff[i] = tot_w;
+        all[i].w = work + tot_w;
+        tot_w += tj_work(all[i].nv);
+        all[i].counter = counters + i;
+        all[i].active = all[i].nv > 1;
+        all[i].solve_start = all[i].tile_start = 0;
+    }
+    // uploads go through the stream and are waited for, so the host vectors
+    // may change afterwards and earlier kernels reading the table are done
+    auto upload = [&](const void *src, void *dst, size_t bytes, const char *what) -> int {
+        if (int rc = hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what)) return rc;
+        return hip_check(hipStreamSynchronize(st), what);
+    };
+    if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
+    hipLaunchKernelGGL(tj_init_kernel, dim3(n_items), dim3(256), 0, st, items);
+    SYN_LAUNCH_CHECK("gram init");
+
+    std::vector<int32_t> hc(n_items);
+    std::vector<int> idx;
+    for (int i = 0; i < n_items; ++i)
+        if (all[i].active) idx.push_back(i);
+    int used = 0;
+    for (; used < max_sweeps && !idx.empty(); ++used) {
+        std::vector<syn_gbj_item> act;
+        int64_t ss = 0, ts = 0, max_np = 2;
+        for (int i : idx) {
+            syn_gbj_item it = all[i];
+            const int64_t half = tj_half(it.nv), ncs = (it.nv + kTCol - 1) / kTCol;
+            it.solve_start = ss;
+            it.tile_start = ts;
+            ss += half;
+            ts += half * half + half * ncs;
+            if (2 * half > max_np) max_np = 2 * half;
+            act.push_back(it);
+        }
+        if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
+        hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, counters, n_items);
+        for (int64_t step = 0; step < max_np - 1; ++step) {
+            {
+                ProfScope ps(kTagGramSolve, 0.0, st);
+                hipLaunchKernelGGL(tj_solve_kernel, dim3((unsigned)ss), dim3(kTSolveThreads), 0, st, items,
+                                   (int)act.size(), step, tol, g_tj_inner);
+            }
+            {
+                ProfScope ps(kTagGramUpdate, 0.0, st);
+                hipLaunchKernelGGL(tj_update_kernel, dim3((unsigned)ts), dim3(kTUpdThreads), 0, st, items,
+                                   (int)act.size(), step);
+            }
+        }
+        SYN_LAUNCH_CHECK("gram step");
+        if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                              st),
+                               "gram counters"))
+            return rc;
+        if (int rc = hip_check(hipStreamSynchronize(st), "gram sync")) return rc;
+        std::vector<int> next;
+        for (int i : idx)
+            if (hc[i] != 0) next.push_back(i);
+        idx.swap(next);
+        if (idx.empty()) {
+            ++used;
+            break;
+        }
+    }
+    if (sweeps) *sweeps = used;
+
+    // eigenvalues, host argsort per problem, permutation of rt rows and s
+    if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
+    hipLaunchKernelGGL(tj_diag_kernel, dim3(n_items), dim3(256), 0, st, items);
+    SYN_LAUNCH_CHECK("gram diag");
+    int64_t tot_nv = 0;
+    std::vector<int64_t> poff(n_items), toff(n_items);
+    for (int i = 0; i < n_items; ++i) {
+        poff[i] = tot_nv;
+        tot_nv += all[i].nv;
+        toff[i] = woff[i] + tj_half(all[i].nv) * kTP * kTP + (tj_half(all[i].nv) + 1) / 2 + 1;
+    }
+    std::vector<double> hs(tot_nv);
+    for (int i = 0; i < n_items; ++i)
+        if (int rc = hip_check(hipMemcpyAsync(hs.data() + poff[i], all[i].s, all[i].nv * sizeof(double),
+                                              hipMemcpyDeviceToHost, st),
+                               "gram s copy"))
+            return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "gram s sync")) return rc;
+    std::vector<int64_t> hp(tot_nv);
+    for (int i = 0; i < n_items; ++i) {
+        int64_t *p = hp.data() + poff[i];
+        const double *sv = hs.data() + poff[i];
+        std::iota(p, p + all[i].nv, 0);
+        std::stable_sort(p, p + all[i].nv, [&](int64_t a, int64_t b) { return sv[a] > sv[b]; });
+    }
+    int64_t *dperm = (int64_t *)(work + tot_w);
+    int64_t *dpoff = dperm + tot_nv;
+    int64_t *dtoff = dpoff + n_items;
+    if (int rc = upload(hp.data(), dperm, tot_nv * sizeof(int64_t), "gram perm")) return rc;
+    if (int rc = upload(poff.data(), dpoff, n_items * sizeof(int64_t), "gram poff")) return rc;
+    if (int rc = upload(toff.data(), dtoff, n_items * sizeof(int64_t), "gram toff")) return rc;
+    hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 0);
+    hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 1);
+    SYN_LAUNCH_CHECK("gram permute");
+    return hip_check(hipStreamSynchronize(st), "gram final sync");
+}

@@ -454,3 +454,30 @@ def jacobi_blocked(X, gram=False, tol=None, max_sweeps=JACOBI_SWEEPS):
     call("syn_jacobi_blocked_f64", px, nv, ln, ldx, _ptr(rt), nv, _ptr(s), _ptr(work), float(tol),
          int(max_sweeps), int(gram), ctypes.byref(sweeps), _stream())
     return rt, s, sweeps.value
+
+
+def jacobi_gram_blocked(G, tol=None, max_sweeps=JACOBI_SWEEPS):
+    """Two-sided block Jacobi eigen-decomposition of one symmetric Gram matrix
+    (row-major nv x nv, diagonalised in place).  Returns (rt, s, sweeps) with
+    rt rows the eigenvectors by decreasing eigenvalue and s = sqrt(lambda)."""
+    nv, ln = G.shape
+    if nv != ln:
+        raise ValueError("jacobi_gram_blocked needs a square matrix")
+    if tol is None:
+        tol = jacobi_tol(nv)
+    px, ldx, tr = as_matrix(G)
+    if tr:
+        raise ValueError("jacobi_gram_blocked needs a row-major matrix")
+    rt = torch.empty((nv, nv), dtype=F64, device=G.device)
+    s = torch.empty(nv, dtype=F64, device=G.device)
+    it = np.zeros(1, dtype=_lib.GBJ_ITEM)
+    it["g"], it["ldg"], it["rt"], it["ldr"], it["s"], it["nv"] = px, ldx, _ptr(rt), nv, _ptr(s), nv
+    nva = np.array([nv], dtype=np.int64)
+    work = torch.empty(int(_lib.load().syn_jacobi_gram_blocked_workspace(nva.ctypes.data_as(ctypes.c_void_p), 1)),
+                       dtype=F64, device=G.device)
+    counters = torch.zeros(1, dtype=torch.int32, device=G.device)
+    dev_items = torch.empty(it.nbytes, dtype=torch.uint8, device=G.device)
+    sweeps = ctypes.c_int(0)
+    call("syn_jacobi_gram_blocked_f64", _ptr(dev_items), it.ctypes.data_as(ctypes.c_void_p), 1, _ptr(work),
+         _ptr(counters), float(tol), int(max_sweeps), ctypes.byref(sweeps), _stream())
+    return rt, s, sweeps.value

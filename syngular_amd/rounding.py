@@ -92,7 +92,7 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None):
         if G.shape[0] <= LDS_JACOBI_MAX:
             rt, s = ops.jacobi_gram_many([G])[0]
         else:
-            rt, s, _ = ops.jacobi_blocked(G, gram=True)
+            rt, s, _ = ops.jacobi_gram_blocked(G)
         keep, disc = ops.truncate_many([(s, chi_max, cutoff)])
         cc = int(keep.item())
         prev = cores[k - 1]

@@ -279,6 +279,28 @@ class Emulator:
             self.buf(r["dropped"], 1, np.int32, "chol dropped")[0] += int((~keep).sum())
         return 0
 
+    def syn_jacobi_gram_blocked_f64(self, items_dev, items_host, n, work, counters, tol, max_sweeps, sweeps, stream):
+        from syngular_amd import _lib
+        self.buf(items_dev, n * _lib.GBJ_ITEM.itemsize, np.uint8, "gram device table")
+        addr = items_host.value if isinstance(items_host, ctypes.c_void_p) else int(items_host)
+        it = np.frombuffer(bytes((ctypes.c_char * (n * _lib.GBJ_ITEM.itemsize)).from_address(addr)),
+                           dtype=_lib.GBJ_ITEM)
+        self.buf(work, int(_gram_workspace(it["nv"])), what="gram work")
+        self.buf(counters, n, np.int32, "gram counters")
+        for r in it:
+            nv = int(r["nv"])
+            G = self.mat(r["g"], nv, nv, int(r["ldg"]), 0, "gram g")
+            R = self.mat(r["rt"], nv, nv, int(r["ldr"]), 0, "gram rt")
+            S = self.buf(r["s"], nv, what="gram s")
+            w, V = np.linalg.eigh(0.5 * (G + G.T))
+            order = np.argsort(-w, kind="stable")
+            R[...] = V[:, order].T
+            S[:] = np.sqrt(w[order].clip(min=0))
+            G[...] = np.diag(w)
+        if sweeps is not None and hasattr(sweeps, "_obj"):
+            sweeps._obj.value = 1
+        return 0
+
     def syn_truncate_f64(self, items, n, stream):
         from syngular_amd import _lib
         for r in self.table(items, n, _lib.TRUNC_ITEM):
@@ -294,7 +316,20 @@ def _workspace(nv, ln):
     return int((nv * ln + nv * nv + nv + 2 * nv + 2).sum() + 2 * len(nv) + 16)
 
 
+def _gram_workspace(nv):
+    tot = 0
+    for v in np.asarray(nv, np.int64):
+        nblk = (int(v) + 15) // 16
+        half = (nblk + (nblk & 1)) // 2
+        tot += half * 32 * 32 + (half + 1) // 2 + 1 + int(v) * int(v) + int(v) + 4 + 2 * int(v)
+    return tot + 2 * len(nv) + 16
+
+
 class _FakeLib:
+    def syn_jacobi_gram_blocked_workspace(self, nv_ptr, n):
+        nv = np.ctypeslib.as_array(ctypes.cast(nv_ptr, ctypes.POINTER(ctypes.c_int64)), (n,))
+        return _gram_workspace(nv)
+
     def syn_jacobi_blocked_many_workspace(self, nv_ptr, len_ptr, n):
         nv = np.ctypeslib.as_array(ctypes.cast(nv_ptr, ctypes.POINTER(ctypes.c_int64)), (n,))
         ln = np.ctypeslib.as_array(ctypes.cast(len_ptr, ctypes.POINTER(ctypes.c_int64)), (n,))

@@ -168,3 +168,28 @@ def test_served_job_matches_oracle_circuit():
             mpo = W.gate_mpo_cores(j["n"], l % 2, W.layer_gates(j["seed"], j["id"], l, j["n"]))
             cores = ornd.apply_and_round(mpo, cores, chi_max=j["chi_max"], cutoff=j["cutoff"])[0]
         _check_against_oracle(r.cores, cores, j["n"])
+
+
+def test_gram_blocked_many_uneven():
+    rng = np.random.default_rng(19)
+    sizes = [65, 96, 130, 200, 33 * 16 + 5]
+    mats = []
+    for g in sizes:
+        C = rng.standard_normal((g + 20, g))
+        mats.append(torch.from_numpy(C.T @ C).to(ops.device()))
+    gt, gp = B.arena([g * g for g in sizes])
+    B.copy_table(np.array([m.data_ptr() for m in mats], np.uint64), gp, np.array([g * g for g in sizes]))
+    rtt, rtp = B.arena([g * g for g in sizes])
+    st, sp = B.arena(sizes)
+    B.jacobi_gram_blocked_table(gp, np.array(sizes), rtp, sp)
+    torch.cuda.synchronize()
+    hs, hr = st.cpu().numpy(), rtt.cpu().numpy()
+    for q, g in enumerate(sizes):
+        o = int((sp[q] - np.uint64(st.data_ptr())) // np.uint64(8))
+        orr = int((rtp[q] - np.uint64(rtt.data_ptr())) // np.uint64(8))
+        s = hs[o:o + g]
+        R = hr[orr:orr + g * g].reshape(g, g)
+        G0 = mats[q].cpu().numpy()
+        ref = np.sqrt(np.linalg.eigvalsh(G0)[::-1].clip(min=0))
+        assert np.abs(s - ref).max() <= 1e-12 * ref[0]
+        assert np.abs(R.T @ np.diag(s ** 2) @ R - G0).max() <= 1e-11 * ref[0] ** 2

@@ -294,3 +294,24 @@ def test_jacobi_gram_rejects_oversize():
 
     with pytest.raises(ValueError):
         ops.jacobi_gram_many([T(np.eye(65))])
+
+
+@pytest.mark.parametrize("g,rank", [(65, 65), (100, 100), (257, 257), (300, 120), (512, 512)])
+def test_jacobi_gram_blocked(g, rank):
+    """Two-sided block Jacobi of a Gram matrix: eigenvalues = squared
+    singular values, orthonormal sorted eigenvector rows, reconstruction."""
+    from syngular_amd import ops
+
+    case(f"jacobi gram blocked g={g} rank={rank}")
+    rng = np.random.default_rng(g + rank)
+    C = rng.standard_normal((g + 7, rank)) @ rng.standard_normal((rank, g))
+    G0 = C.T @ C
+    rt, s, sweeps = ops.jacobi_gram_blocked(T(G0))
+    s = s.cpu().numpy()
+    R = rt.cpu().numpy()
+    ref = np.linalg.svd(C, compute_uv=False)
+    assert np.all(np.diff(s) <= 0) and 0 < sweeps < 60
+    np.testing.assert_allclose(s[:rank], ref[:rank], rtol=1e-9, atol=1e-12 * ref[0])
+    assert np.all(s[rank:] <= 1e-6 * ref[0])
+    np.testing.assert_allclose(R @ R.T, np.eye(g), atol=1e-12)
+    np.testing.assert_allclose(R.T @ np.diag(s ** 2) @ R, G0, atol=1e-11 * ref[0] ** 2)
