@@ -1,21 +1,30 @@
 #!/usr/bin/env python3
 """apply+round benchmark of the matrix-product core on one MI355X.
 
-Workload (one "step"): the MPO x MPS product of an N=32-site chain
-(d = 2, MPS bond chi = 256, MPO bond D_w = 16, product bond 4096) followed by
-TT-rounding back to chi = 256 (left CholeskyQR2 sweep, right Jacobi-SVD
-sweep), plain FP64, inputs resident in HBM.  Value = sites/s = 32 * steps /
-elapsed.
+Headline workload (one "step", the JSON `value`): the MPO x MPS product of
+an N=32-site chain (d = 2, MPS bond chi = 256, MPO bond D_w = 16, product bond
+4096) followed by TT-rounding back to chi = 256 (left CholeskyQR2 sweep,
+right Gram + Jacobi SVD sweep), plain FP64, inputs resident in HBM.
+Value = sites/s = 32 * steps / elapsed.
 
-Besides the JSON contract line the script reports:
+Beside the JSON contract fields the line carries:
   roofline      the kernel with the largest share of GPU time in the timed
-                region, its algorithmic rate (HIP events recorded by the
+                chain steps, its algorithmic rate (HIP events recorded by the
                 library on the launch stream) against the MI355X peak;
   cpu_baseline  the NumPy FP64 oracle on a bounded sample of the same chain
                 (its last three sites), timed on this host;
-  parity        errors of the GPU result against the oracle on a reduced
-                chain, and the Pythagoras identity ||OX||^2 - ||Z||^2 =
-                sum of discarded weights at full size.
+  parity        errors against the oracle on a reduced chain and the
+                Pythagoras identity at full size;
+  forms         the many-chain workloads run by the grouped engine:
+                batch: 4096 independent chains (N ~ U[8,48], chi ~ U{8..128})
+                       with one layer of Haar 2-site gates each, rounded back
+                       to chi, all in lock-step grouped launches;
+                serve: a Poisson stream of brickwork-circuit jobs (N ~
+                       U[16,128], layers ~ U[4,32], chi_max ~ U{64..512},
+                       cutoff 1e-10) under the continuous-batching server;
+                each beside the per-chain FP64 form (one chain at a time
+                through the same kernels) on the same jobs, with p50/p99
+                latency and the error against the oracle.
 """
 from __future__ import annotations
 
@@ -44,8 +53,16 @@ def parse():
     ap.add_argument("--chi", type=int, default=256)
     ap.add_argument("--bond-mpo", type=int, default=16)
     ap.add_argument("--phys", type=int, default=2)
+    ap.add_argument("--batch-jobs", type=int, default=4096)
+    ap.add_argument("--serve-jobs", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--plain-seconds", type=float, default=20.0,
+                    help="time budget of each per-chain comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--chain-only", action="store_true", help="only the headline chain (for profiling)")
     return ap.parse_args()
 
 
@@ -56,6 +73,8 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               f"(asked for --gpus {args.gpus}, WORLD_SIZE={world}).", file=sys.stderr)
         sys.exit(2)
+    if args.chain_only:
+        args.no_batch = args.no_serve = args.no_cpu_baseline = args.no_parity = True
 
     import numpy as np
     import torch
@@ -92,38 +111,23 @@ def main():
     sites_per_s = n * args.steps / elapsed
     log(f"timed {args.steps} steps: {elapsed:.3f} s, {ms_per_step:.1f} ms/step, {sites_per_s:.2f} sites/s")
 
-    # ---- roofline of the dominant kernel ---------------------------------
-    total_ms = sum(v[1] for v in prof.values())
-    dom, (dl, dms, dfl) = max(prof.items(), key=lambda kv: kv[1][1])
-    if dom == "apply_kernel":
-        achieved = dfl / (dms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
-    else:
-        achieved = dfl / (dms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAKS["fp64_mfma_tflops"],
-                "unit": "TFLOP/s"}
-    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
-    roof["traffic"] = None
-    roof["kernel"] = dom
-    roof["launches_per_step"] = dl / args.steps
-    roof["avg_launch_us"] = round(1e3 * dms / dl, 2)
-    roof["share_of_gpu_time"] = round(dms / total_ms, 3) if total_ms else None
-    kept = list(rep.kept)
-    textbook = W.apply_round_flops(*W.chain_shapes(n, chi, dw), d, kept)
-    pipeline = {"textbook_flops_per_step": textbook,
-                "textbook_tflops": round(textbook / (ms_per_step * 1e-3) / 1e12, 3),
-                "kernels": {k: {"launches": v[0], "ms": round(v[1], 2),
-                                "rate": round(v[2] / (v[1] * 1e-3) / (1e9 if k == "apply_kernel" else 1e12), 3)}
-                            for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}}
-    log(f"dominant kernel {dom}: {roof['achieved']} {roof['unit']} ({roof['frac']:.1%} of peak)")
+    roof, pipeline = roofline(prof, args.steps, ms_per_step, W, n, chi, dw, d, rep)
+    log(f"dominant kernel {roof['kernel']}: {roof['achieved']} {roof['unit']} ({roof['frac']:.1%} of peak)")
 
-    # ---- parity ----------------------------------------------------------
     parity = None
     if not args.no_parity:
         parity = parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep)
         log(f"parity: {parity}")
+    del Z, rep
 
-    # ---- CPU baseline (oracle on the chain's last three sites) -----------
+    forms = {}
+    if not args.no_batch:
+        forms["batch"] = batch_form(np, torch, args)
+        log(f"batch form: {forms['batch']}")
+    if not args.no_serve:
+        forms["serve"] = serve_form(np, torch, args)
+        log(f"serve form: {forms['serve']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(np, xs, os_, chi)
@@ -141,7 +145,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Gaussian cores)",
+        "data": "synthetic (seeded Gaussian cores, Haar-random orthogonal 2-site gates)",
         "config": {"workload": f"MPO x MPS apply + SVD rounding, chain N={n} d={d} chi={chi} D_w={dw} "
                                f"(product bond {chi * dw}), plain FP64",
                    "form": "plain-fp64", "sites": n, "phys_dim": d, "chi": chi, "mpo_bond": dw,
@@ -150,8 +154,42 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "pipeline": pipeline,
+        "forms": forms or None,
     }
     print(json.dumps(line), flush=True)
+
+
+def roofline(prof, steps, ms_per_step, W, n, chi, dw, d, rep):
+    """Dominant kernel of the timed chain steps: algorithmic work (flops for
+    the GEMMs, bytes for the memory-bound kernels) over its event time."""
+    total_ms = sum(v[1] for v in prof.values())
+    dom, (dl, dms, dwork) = max(prof.items(), key=lambda kv: kv[1][1])
+    if dom in BYTE_KERNELS:
+        achieved = dwork / (dms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
+    else:
+        achieved = dwork / (dms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAKS["fp64_mfma_tflops"],
+                "unit": "TFLOP/s"}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    roof["traffic"] = None
+    roof["kernel"] = dom
+    roof["launches_per_step"] = dl / steps
+    roof["avg_launch_us"] = round(1e3 * dms / dl, 2)
+    roof["share_of_gpu_time"] = round(dms / total_ms, 3) if total_ms else None
+    kept = list(rep.kept)
+    textbook = W.apply_round_flops(*W.chain_shapes(n, chi, dw), d, kept)
+    pipeline = {"textbook_flops_per_step": textbook,
+                "textbook_tflops": round(textbook / (ms_per_step * 1e-3) / 1e12, 3),
+                "kernels": {k: {"launches": v[0], "ms": round(v[1], 2),
+                                "rate": round(v[2] / (v[1] * 1e-3) / (1e9 if k in BYTE_KERNELS else 1e12), 3),
+                                "rate_unit": "GB/s" if k in BYTE_KERNELS else "TFLOP/s"}
+                            for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}}
+    return roof, pipeline
+
+
+# kernels whose profiler work figure is bytes moved (the rest: flops)
+BYTE_KERNELS = ("apply_kernel", "gram_update_kernel")
 
 
 def parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep):
@@ -191,6 +229,136 @@ def parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep):
     out["full_pythagoras_rel_err"] = abs(full2 - z2 - wsum) / full2
     out["full_discarded_weight_rel"] = wsum / full2
     return out
+
+
+def batch_form(np, torch, args):
+    """4096 chains, one gate layer each, rounded back to their chi: the
+    grouped engine (all chains in lock-step launches) against the per-chain
+    form on a sample of the same jobs; oracle error on a few jobs."""
+    from syngular_amd import batch as B
+    from syngular_amd import ops, rounding
+    from syngular_amd import workloads as W
+
+    t0 = time.perf_counter()
+    jobs = W.batching_jobs(args.batch_jobs, seed=3)
+    nsites = sum(j["n"] for j in jobs)
+    log(f"batch: {len(jobs)} jobs, {nsites} sites, generated in {time.perf_counter() - t0:.1f} s")
+    X = B.ChainBatch.from_numpy([j["cores"] for j in jobs])
+    Op = B.gate_layer([j["n"] for j in jobs], np.zeros(len(jobs), np.int64),
+                      np.concatenate([j["gates"] for j in jobs]))
+    chi = np.array([j["chi"] for j in jobs])
+    torch.cuda.synchronize()
+    B.apply_and_round(X, Op, chi)          # warm-up
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        Z = B.apply_and_round(X, Op, chi)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    batched = nsites / min(times)
+    # per-chain form on a strided sample of the same jobs
+    dev = ops.device()
+    sample = jobs[:: max(1, len(jobs) // 256)]
+    done_sites, plain_time, used = 0, 0.0, 0
+    for j in sample:
+        # inputs go to the device before the clock starts, as for the batch
+        Xs = [torch.from_numpy(c).to(dev) for c in j["cores"]]
+        Os = [torch.from_numpy(c).to(dev) for c in W.gate_mpo_cores(j["n"], 0, j["gates"])]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rounding.apply_and_round(Os, Xs, chi_max=j["chi"])
+        torch.cuda.synchronize()
+        plain_time += time.perf_counter() - t1
+        done_sites += j["n"]
+        used += 1
+        if plain_time > args.plain_seconds:
+            break
+    plain = done_sites / plain_time
+    # oracle error on a few jobs (every 512th)
+    from oracle import reference as oref
+    from oracle import rounding as ornd
+    errs = []
+    check = list(range(0, len(jobs), 512))[:8]
+    outs = Z.to_numpy(check)
+    for q, c in zip(check, outs):
+        j = jobs[q]
+        ref = ornd.apply_and_round(W.gate_mpo_cores(j["n"], 0, j["gates"]), j["cores"], chi_max=j["chi"])[0]
+        n2r, ovr = ornd.norm2(ref), oref.overlap(ref, j["cores"])
+        scale = np.sqrt(n2r * ornd.norm2(j["cores"]))
+        errs.append((abs(ornd.norm2(c) - n2r) / n2r, abs(oref.overlap(c, j["cores"]) - ovr) / scale))
+    return {"workload": f"{len(jobs)} chains, N~U[8,48], d=2, chi~U{{8,16,32,64,128}}, one layer of Haar "
+                        f"O(4) 2-site gates (MPO bond 4) on pairs (0,1),(2,3),..., rounded to chi; plain FP64",
+            "sites": nsites, "sites_per_s": round(batched, 1), "seconds": round(min(times), 3),
+            "plain_fp64_per_chain_sites_per_s": round(plain, 1),
+            "plain_sample": f"{done_sites} sites of {used} strided jobs, one chain at a time",
+            "speedup_vs_plain": round(batched / plain, 2),
+            "oracle_jobs": len(errs), "norm2_rel_err_max": max(e[0] for e in errs),
+            "overlap_rel_err_max": max(e[1] for e in errs)}
+
+
+def serve_form(np, torch, args):
+    """A Poisson stream through the continuous-batching server against the
+    per-chain form on the same jobs (time-boxed), with latency percentiles
+    and the error against the oracle on the cheapest jobs."""
+    from syngular_amd import ops, rounding, serving
+    from syngular_amd import workloads as W
+
+    jobs = W.serving_jobs(args.serve_jobs, args.serve_rate, seed=4)
+    srv = serving.Server(serving.DeviceExecutor(), hbm_budget=200 * 2 ** 30, keep_states=False)
+    # warm-up on a tiny stream (kernel loads, allocator)
+    serving.Server(serving.DeviceExecutor(), hbm_budget=2 ** 34).run(
+        W.serving_jobs(2, 1e6, seed=99, n_range=(16, 20), layer_range=(2, 2), chis=(64,)))
+    torch.cuda.synchronize()
+    rep = srv.run(jobs)
+    lat = rep.latency_percentiles()
+    pyth = max(abs(r.norm2 + r.discarded - 1.0) for r in rep.results)
+    # per-chain form: the same jobs one chain and one layer at a time
+    dev = ops.device()
+    done, t0 = 0, time.perf_counter()
+    for j in jobs:
+        cores = [torch.from_numpy(np.eye(2)[0].reshape(1, 2, 1).copy()).to(dev) for _ in range(j["n"])]
+        for layer in range(j["layers"]):
+            mpo = [torch.from_numpy(c).to(dev)
+                   for c in W.gate_mpo_cores(j["n"], layer % 2, W.layer_gates(j["seed"], j["id"], layer, j["n"]))]
+            cores = rounding.apply_and_round(mpo, cores, chi_max=j["chi_max"], cutoff=j["cutoff"])
+            torch.cuda.synchronize()
+            done += j["n"]
+            if time.perf_counter() - t0 > args.plain_seconds:
+                break
+        if time.perf_counter() - t0 > args.plain_seconds:
+            break
+    plain = done / (time.perf_counter() - t0)
+    # oracle on the two cheapest jobs (n * layers), full circuits
+    from oracle import rounding as ornd
+    cheap = sorted(jobs, key=lambda j: j["n"] * j["layers"] * j["chi_max"])[:2]
+    srv2 = serving.Server(serving.DeviceExecutor(), hbm_budget=2 ** 36, keep_states=True)
+    got = {r.id: r for r in srv2.run([dict(j, arrival=0.0) for j in cheap]).results}
+    errs = []
+    for j in cheap:
+        cores = [np.eye(2)[0].reshape(1, 2, 1) for _ in range(j["n"])]
+        for layer in range(j["layers"]):
+            mpo = W.gate_mpo_cores(j["n"], layer % 2, W.layer_gates(j["seed"], j["id"], layer, j["n"]))
+            cores = ornd.apply_and_round(mpo, cores, chi_max=j["chi_max"], cutoff=j["cutoff"])[0]
+        from oracle import reference as oref
+        g = got[j["id"]].cores
+        n2r = ornd.norm2(cores)
+        errs.append((abs(ornd.norm2(g) - n2r) / n2r, abs(oref.overlap(g, cores) - n2r) / n2r))
+    return {"workload": f"Poisson stream of {len(jobs)} jobs at {args.serve_rate}/s: N~U[16,128], layers~U[4,32] "
+                        f"brickwork Haar O(4) gates (MPO bond 4), chi_max~U{{64,128,256,512}}, cutoff 1e-10, "
+                        f"from |0...0>; plain FP64",
+            "site_layers": rep.site_layers, "makespan_s": round(rep.makespan, 2),
+            "sites_per_s": round(rep.sites_per_s, 1), "p50_latency_s": round(lat["p50"], 2),
+            "p99_latency_s": round(lat["p99"], 2), "layer_steps": rep.steps, "peak_resident": rep.peak_resident,
+            "admitted_while_running": rep.admitted_during_run,
+            "plain_fp64_per_chain_sites_per_s": round(plain, 1),
+            "plain_sample": f"{done} site-layers of the same jobs in order, one chain and layer at a time, "
+                            f"time-boxed to {args.plain_seconds:.0f} s",
+            "speedup_vs_plain": round(rep.sites_per_s / plain, 2),
+            "norm_pythagoras_err_max": pyth,
+            "oracle_jobs": [f"N={j['n']} layers={j['layers']} chi_max={j['chi_max']}" for j in cheap],
+            "norm2_rel_err_max": max(e[0] for e in errs),
+            "overlap_rel_err_max": max(e[1] for e in errs)}
 
 
 def cpu_baseline(np, xs, os_, chi):

@@ -23,6 +23,7 @@ bool g_on = false;
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_pool;
 double g_pending_flops = 0.0;
+double g_extra[SYN_PROF_NTAGS] = {};   // work added without events (totals per call)
 
 hipEvent_t take_event() {
     if (!g_pool.empty()) {
@@ -37,6 +38,12 @@ hipEvent_t take_event() {
 }  // namespace
 
 bool prof_on() { return g_on; }
+
+void prof_add_work(int tag, double work) {
+    if (!g_on || tag < 0 || tag >= SYN_PROF_NTAGS) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_extra[tag] += work;
+}
 
 ProfScope::ProfScope(int tag, double flops, hipStream_t s) : tag_(tag), flops_(flops), s_(s), a_(nullptr) {
     if (!g_on) return;
@@ -67,6 +74,7 @@ extern "C" int syn_prof_begin(void) {
     }
     syn::g_recs.clear();
     syn::g_pending_flops = 0.0;
+    for (double &e : syn::g_extra) e = 0.0;
     syn::g_on = true;
     return SYN_OK;
 }
@@ -86,6 +94,7 @@ extern "C" int syn_prof_end(int64_t *launches, double *ms, double *flops, int nt
         ms[t] = 0.0;
         flops[t] = 0.0;
     }
+    for (int t = 0; t < ntags && t < SYN_PROF_NTAGS; ++t) flops[t] += g_extra[t];
     for (auto &r : g_recs) {
         if (int rc = hip_check(hipEventSynchronize(r.b), "prof event sync")) return rc;
         float el = 0.0f;

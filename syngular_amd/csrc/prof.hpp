@@ -21,6 +21,10 @@ enum ProfTag {
 
 bool prof_on();
 
+// Adds work (flops or bytes) to a tag's total without events: for kernels
+// whose work is only known on the device and summed up after the call.
+void prof_add_work(int tag, double work);
+
 // Records an event before (constructor) and after (destructor) the launch
 // issued in its scope.  flops < 0 takes the value given by the last
 // syn_prof_note_flops call.

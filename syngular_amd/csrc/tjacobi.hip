@@ -220,8 +220,12 @@ __global__ __launch_bounds__(kTSolveThreads) void tj_solve_kernel(const syn_gbj_
 
 // MFMA f64 16x16x4 layout: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
 // D reg r of lane l = D[l/16 + 4r][l%16].
+// bytes moved by one executed tile: a G tile read and written, or an rt slab
+constexpr unsigned long long kTileBytes = 2ull * kTP * kTP * sizeof(double);
+constexpr unsigned long long kSlabBytes = 2ull * kTP * kTCol * sizeof(double);
+
 __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_item *__restrict__ items, int n,
-                                                                  int64_t step) {
+                                                                  int64_t step, unsigned long long *bytes) {
     __shared__ double Wa[kTP][kTP + 1];
     __shared__ double Wb[kTP][kTP + 1];
     __shared__ double X[kTP][kTCol + 1];   // G tile (kTP columns used) or rt slab
@@ -239,6 +243,7 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
         const int64_t a = t / D.half, b = t % D.half;
         const int fa = flags[a], fb = flags[b];
         if (!fa && !fb) return;
+        if (tid == 0) atomicAdd(bytes, kTileBytes);
         if (tid < kTP) {
             ra[tid] = tj_row(st, a, D.np, D.nblk, P.nv, tid);
             rb[tid] = tj_row(st, b, D.np, D.nblk, P.nv, tid);
@@ -281,6 +286,7 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
     const int64_t u = t - D.half * D.half;
     const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
     if (!flags[a]) return;
+    if (tid == 0) atomicAdd(bytes, kSlabBytes);
     if (tid < kTP) ra[tid] = tj_row(st, a, D.np, D.nblk, P.nv, tid);
     __syncthreads();
     const double *wa = P.w + a * kTP * kTP;
@@ -420,6 +426,12 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
     if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
     hipLaunchKernelGGL(tj_init_kernel, dim3(n_items), dim3(256), 0, st, items);
     SYN_LAUNCH_CHECK("gram init");
+    // byte counter of the update launches, after the sort scratch
+    int64_t tot_nv0 = 0;
+    for (int i = 0; i < n_items; ++i) tot_nv0 += all[i].nv;
+    unsigned long long *dbytes = (unsigned long long *)(work + tot_w + tot_nv0 + 2 * (int64_t)n_items);
+    hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, (int32_t *)dbytes, 2);
+    SYN_LAUNCH_CHECK("gram byte counter");
 
     std::vector<int32_t> hc(n_items);
     std::vector<int> idx;
@@ -450,7 +462,7 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
             {
                 ProfScope ps(kTagGramUpdate, 0.0, st);
                 hipLaunchKernelGGL(tj_update_kernel, dim3((unsigned)ts), dim3(kTUpdThreads), 0, st, items,
-                                   (int)act.size(), step);
+                                   (int)act.size(), step, dbytes);
             }
         }
         SYN_LAUNCH_CHECK("gram step");
@@ -504,5 +516,12 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
     hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 0);
     hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 1);
     SYN_LAUNCH_CHECK("gram permute");
+    if (prof_on()) {
+        unsigned long long hb = 0;
+        if (int rc = hip_check(hipMemcpyAsync(&hb, dbytes, sizeof(hb), hipMemcpyDeviceToHost, st), "gram bytes"))
+            return rc;
+        if (int rc = hip_check(hipStreamSynchronize(st), "gram bytes sync")) return rc;
+        prof_add_work(kTagGramUpdate, (double)hb);
+    }
     return hip_check(hipStreamSynchronize(st), "gram final sync");
 }
