@@ -540,7 +540,8 @@ class ChainBatch:
             dn, bn = self.phys[a, k + 1], bond[a, k + 2]
             ncols = dn * bn
             nt, npk = arena(new_r * ncols)
-            gemm_table(S, ptr[a, k + 1], npk, new_r, ncols, r, lds, ncols, ncols)
+            # S from a QR is upper triangular (tri 4); S = L of a wide unfolding is not
+            gemm_table(S, ptr[a, k + 1], npk, new_r, ncols, r, lds, ncols, ncols, tri=np.where(wide, 0, 4))
             hold += [qt, nt]
             ptr[a, k] = q
             ptr[a, k + 1] = npk

@@ -3,14 +3,16 @@
 The reference's live ``>>`` is QR truncation (tensor/matrix_product_state.py:432-468);
 this module adds the textbook TT-rounding (Oseledets 2011, Alg. 2):
 
-1. left sweep: QR of each left unfolding (Householder, one work-group per
-   matrix) and R absorbed into the next core with the MFMA GEMM; an unfolding
-   with no more rows than columns needs no factorisation (Q = I, R = L);
+1. left sweep: QR of each left unfolding (CholeskyQR2 on the MFMA GEMM for
+   96 columns or more, else Householder with one work-group per matrix) and
+   R absorbed into the next core with the GEMM; an unfolding with no more
+   rows than columns needs no factorisation (Q = I, R = L);
 2. right sweep: at bond k the right unfolding C (m x n) of core k is
    decomposed through the Gram matrix of its short side (G = C^T C or C C^T,
-   MFMA GEMM) by the one-sided Jacobi kernel, the kept rank is chosen on the
-   device (relative discarded weight <= cutoff, then <= chi_max, >= 1), and
-   U*S is absorbed into core k-1 with the GEMM.
+   MFMA GEMM) by two-sided Jacobi (in LDS up to 64, block Jacobi beyond),
+   the kept rank is chosen on the device (relative discarded weight <=
+   cutoff, then <= chi_max, >= 1), and U*S is absorbed into core k-1 with
+   the GEMM.
 
 Singular values come from the Gram matrix, so a singular value s_i carries an
 absolute error of about eps * s_max^2 / s_i; all retained values of the
@@ -57,14 +59,16 @@ def left_orthonormalize(cores, own=False):
         m = L.shape[0]
         if m <= r:
             # a wide unfolding: Q = I_m and R = L, no factorisation needed
-            Q, S = torch.eye(m, dtype=F64, device=L.device), L
+            Q, S, tri = torch.eye(m, dtype=F64, device=L.device), L, 0
         elif r >= CHOLQR_MIN_COLS:
             Q, S = ops.cholqr2(L)
+            tri = 4                      # S = R is upper triangular
         else:
             Q, S = _qr(L)
+            tri = 4
         nxt = cores[k + 1]
         cores[k] = Q.reshape(l, *mid, Q.shape[1])
-        W = ops.gemm(S, nxt.reshape(nxt.shape[0], -1))
+        W = ops.gemm(S, nxt.reshape(nxt.shape[0], -1), tri=tri)
         cores[k + 1] = W.reshape(S.shape[0], *nxt.shape[1:])
     return cores
 

@@ -351,3 +351,56 @@ def test_apply_and_round_exact_when_untruncated():
     y = np.einsum("abcd,abcdefgh->efgh", x, o)
     np.testing.assert_allclose(R.dense_mps(cores), y, atol=1e-11)
     assert all(d < 1e-20 for d in disc[1:])
+
+
+# --------------------------------------------------------------------------
+# non-strict compress (left and right modes) and MPO element access
+# --------------------------------------------------------------------------
+def dense_qr_truncate_right(sites, dim):
+    """Right-mode QR truncation from the ORIGINAL cores, dense: at bond k
+    (right to left) the right block is projected onto the span of its first
+    ``dim`` rows, then core k-1 is attached."""
+    B = sites[-1].reshape(sites[-1].shape[0], -1)
+    for k in range(len(sites) - 1, 0, -1):
+        B = (proj(B.T[:, :dim]) @ B.T).T
+        prv = sites[k - 1]
+        B = (prv.reshape(-1, prv.shape[-1]) @ B).reshape(prv.shape[0], -1)
+    return B
+
+
+def test_mps_compress_non_strict_left_and_right():
+    """Non-strict compress = orthonormalize (tensor unchanged, pinned above)
+    then the QR truncation whose dense effect is the projector closed form."""
+    rng = np.random.default_rng(21)
+    sites = random_sites(rng, (3, 4, 4, 3), (3, 6, 3))
+    for mode, dense in (("left", dense_qr_truncate), ("right", dense_qr_truncate_right)):
+        A = R.MPS.from_sites([s.copy() for s in sites])
+        if mode == "left":
+            A.left_orthonormalization()
+        else:
+            A.right_orthonormalization()
+        expect = dense([s.copy() for s in A.sites], 2).reshape(-1)
+        A.compress(2, mode=mode, strict=False)
+        np.testing.assert_allclose(A.to_tensor().reshape(-1), expect, atol=1e-12)
+        assert all(2 in (s.shape[0], s.shape[-1]) for s in A.sites)
+
+
+def test_mpo_compress_non_strict_and_retrieve():
+    rng = np.random.default_rng(22)
+    w = [1, 3, 5, 1]
+    cores = [rng.standard_normal((w[k], 2, 3, w[k + 1])) for k in range(3)]
+    M = R.MPO.from_sites([c.copy() for c in cores])
+    t = M.to_tensor()
+    # element access = product of the sliced core matrices
+    for idx in [((0, 1, 1), (2, 0, 1)), ((1, 0, 0), (0, 2, 2))]:
+        v = np.ones((1, 1))
+        for k in range(3):
+            v = v @ cores[k][:, idx[0][k], idx[1][k], :]
+        assert abs(M[idx] - v[0, 0]) < 1e-12
+        assert abs(t[idx[0] + idx[1]] - v[0, 0]) < 1e-12
+    M.left_orthonormalization()
+    lo = [c.reshape(c.shape[0], -1, c.shape[-1]).copy() for c in M.sites]
+    expect = dense_qr_truncate(lo, 2).reshape(-1)
+    M.compress(2, strict=False)
+    got = R.dense_mps([c.reshape(c.shape[0], -1, c.shape[-1]) for c in M.sites]).reshape(-1)
+    np.testing.assert_allclose(got, expect, atol=1e-12)
