@@ -14,6 +14,12 @@ granularity:
 3. retire the jobs whose last layer is done (their latency is completion
    minus arrival) and free their charge.
 
+With lanes (``chi_lanes``: by chi_max class) each lane runs this loop for its
+own jobs in its own host thread and HIP stream, under the shared budget: a
+layer step lasts as long as its heaviest job, so light jobs do not wait for
+heavy ones, and their launches fill the compute units that a heavy job's
+latency-bound steps leave idle.
+
 So requests that arrive while others are on the device join at the next layer
 boundary, their work shares every launch with the jobs already running, and a
 job's result does not depend on which others share its launches (each item of
@@ -24,6 +30,7 @@ tests drive the scheduler on the CPU with a stand-in that records calls.
 """
 from __future__ import annotations
 
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -77,58 +84,115 @@ class ServeReport:
         return {f"p{q}": float(np.percentile(lat, q)) for q in qs}
 
 
+class _Shared:
+    """Admission state shared by the lanes: HBM charge and slots in use."""
+
+    def __init__(self, budget, max_slots):
+        self.budget, self.max_slots = budget, max_slots
+        self.used = self.resident = self.peak_bytes = self.peak_resident = 0
+        self.lock = threading.Lock()
+
+    def try_admit(self, nbytes):
+        with self.lock:
+            if self.used + nbytes > self.budget or self.resident + 1 > self.max_slots:
+                return False
+            self.used += nbytes
+            self.resident += 1
+            self.peak_bytes = max(self.peak_bytes, self.used)
+            self.peak_resident = max(self.peak_resident, self.resident)
+            return True
+
+    def release(self, nbytes):
+        with self.lock:
+            self.used -= nbytes
+            self.resident -= 1
+
+
 class Server:
+    """Continuous batching over one or more lanes.
+
+    executor: an executor (one lane), or with ``lanes`` a callable
+    lane key -> executor (one executor, and for the device one HIP stream,
+    per lane).  lanes: None, or a callable job -> lane key; jobs of a lane
+    share its layer steps, lanes run concurrently in their own host threads
+    and share the HBM budget and the slot count."""
+
     def __init__(self, executor, hbm_budget, max_slots=4096, clock=time.perf_counter, sleep=time.sleep,
-                 keep_states=False):
+                 keep_states=False, lanes=None, poll=1e-3):
         self.ex = executor
         self.budget = int(hbm_budget)
         self.max_slots = int(max_slots)
         self.clock = clock
         self.sleep = sleep
         self.keep_states = keep_states
+        self.lanes = lanes
+        self.poll = poll
 
     def run(self, jobs):
         """Serve `jobs` (dicts with id, arrival, n, layers, chi_max, cutoff,
         seed); arrival times are seconds after the call starts."""
-        pending = sorted(jobs, key=lambda j: (j["arrival"], j["id"]))
-        for j in pending:
+        for j in jobs:
             if job_bytes(j["n"], j["chi_max"]) > self.budget:
                 raise ValueError(f"job {j['id']} needs {job_bytes(j['n'], j['chi_max'])} B, more than the "
                                  f"whole HBM budget {self.budget} B")
+        shared = _Shared(self.budget, self.max_slots)
+        results = {}
         t0 = self.clock()
+        if self.lanes is None:
+            stats = [self._lane(self.ex, list(jobs), shared, results, t0)]
+        else:
+            groups = {}
+            for j in jobs:
+                groups.setdefault(self.lanes(j), []).append(j)
+            stats, errors = [], []
+
+            def work(key, group):
+                try:
+                    stats.append(self._lane(self.ex(key), group, shared, results, t0))
+                except BaseException as e:     # re-raised in the calling thread
+                    errors.append(e)
+            threads = [threading.Thread(target=work, args=(k, g), daemon=True) for k, g in sorted(groups.items())]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+            if errors:
+                raise errors[0]
+        makespan = self.clock() - t0
+        trace = sorted(x for st in stats for x in st["trace"])
+        return ServeReport([results[j["id"]] for j in jobs], makespan, sum(st["steps"] for st in stats),
+                           shared.peak_resident, shared.peak_bytes, sum(st["site_layers"] for st in stats),
+                           sum(st["late"] for st in stats), trace)
+
+    def _lane(self, ex, jobs, shared, results, t0):
+        pending = sorted(jobs, key=lambda j: (j["arrival"], j["id"]))
         resident = []          # job dicts, same order as the executor state
         done_layers = []       # layers done per resident job
-        results = {}
-        state = self.ex.empty()
-        used = 0
-        steps = peak_res = peak_bytes = site_layers = late = 0
+        state = ex.empty()
+        steps = site_layers = late = 0
         trace = []
         pi = 0
         while pi < len(pending) or resident:
             now = self.clock() - t0
             new = []
-            while (pi < len(pending) and pending[pi]["arrival"] <= now and len(resident) + len(new) < self.max_slots
-                   and used + job_bytes(pending[pi]["n"], pending[pi]["chi_max"]) <= self.budget):
+            while (pi < len(pending) and pending[pi]["arrival"] <= now
+                   and shared.try_admit(job_bytes(pending[pi]["n"], pending[pi]["chi_max"]))):
                 j = pending[pi]
-                used += job_bytes(j["n"], j["chi_max"])
                 results[j["id"]] = JobResult(j["id"], j["arrival"], admitted=now)
                 new.append(j)
                 pi += 1
             if new:
                 if resident:
                     late += len(new)
-                state = self.ex.merge(state, self.ex.init(new))
+                state = ex.merge(state, ex.init(new))
                 resident += new
                 done_layers += [0] * len(new)
             if not resident:
-                # idle until the next arrival
+                # idle until the next arrival, or until another lane frees room
                 wait = pending[pi]["arrival"] - (self.clock() - t0)
-                if wait > 0:
-                    self.sleep(wait)
+                self.sleep(wait if wait > 0 else self.poll)
                 continue
-            peak_res = max(peak_res, len(resident))
-            peak_bytes = max(peak_bytes, used)
-            state, stats = self.ex.layer(state, resident, done_layers)
+            state, stats = ex.layer(state, resident, done_layers)
             steps += 1
             for i, j in enumerate(resident):
                 r = results[j["id"]]
@@ -138,7 +202,7 @@ class Server:
             fin = [i for i, j in enumerate(resident) if done_layers[i] >= j["layers"]]
             trace.append((self.clock() - t0, len(resident), len(new), len(fin)))
             if fin:
-                out = self.ex.harvest(state, fin, keep_states=self.keep_states)
+                out = ex.harvest(state, fin, keep_states=self.keep_states)
                 tnow = self.clock() - t0
                 for q, i in enumerate(fin):
                     j = resident[i]
@@ -149,30 +213,50 @@ class Server:
                     r.norm2 = out["norm2"][q]
                     if self.keep_states:
                         r.cores = out["cores"][q]
-                    used -= job_bytes(j["n"], j["chi_max"])
-                keepi = [i for i in range(len(resident)) if i not in set(fin)]
-                state = self.ex.select(state, keepi)
+                    shared.release(job_bytes(j["n"], j["chi_max"]))
+                fs = set(fin)
+                keepi = [i for i in range(len(resident)) if i not in fs]
+                state = ex.select(state, keepi)
                 resident = [resident[i] for i in keepi]
                 done_layers = [done_layers[i] for i in keepi]
-        makespan = self.clock() - t0
-        return ServeReport([results[j["id"]] for j in jobs], makespan, steps, peak_res, peak_bytes, site_layers,
-                           late, trace)
+        return {"steps": steps, "site_layers": site_layers, "late": late, "trace": trace}
+
+
+def chi_lanes(job):
+    """Lane key by chi_max class: jobs of similar bond share layer steps."""
+    c = job["chi_max"]
+    return 0 if c <= 128 else (1 if c <= 256 else 2)
 
 
 class DeviceExecutor:
-    """Runs the layers with the grouped gfx950 kernels (syngular_amd.batch)."""
+    """Runs the layers with the grouped gfx950 kernels (syngular_amd.batch).
+    With a stream, every launch of this executor goes to that HIP stream (one
+    per lane, so lanes overlap on the device)."""
 
-    def __init__(self, d=2):
+    def __init__(self, d=2, stream=None):
         from . import batch, ops
         self.B = batch
         self.ops = ops
         self.d = d
+        self.stream = stream
+
+    def _ctx(self):
+        import contextlib
+        import torch
+        return torch.cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
+
+    @classmethod
+    def lanes(cls, d=2):
+        """Factory for Server(lanes=...): one executor and one stream per lane."""
+        import torch
+        return lambda key: cls(d, torch.cuda.Stream())
 
     def empty(self):
         return self.B.concat([])
 
     def init(self, jobs):
-        return self.B.product_states([j["n"] for j in jobs], self.d)
+        with self._ctx():
+            return self.B.product_states([j["n"] for j in jobs], self.d)
 
     def merge(self, a, b):
         return self.B.concat([a, b])
@@ -181,6 +265,10 @@ class DeviceExecutor:
         return state.subset(idx)
 
     def layer(self, state, jobs, done_layers):
+        with self._ctx():
+            return self._layer(state, jobs, done_layers)
+
+    def _layer(self, state, jobs, done_layers):
         from .workloads import layer_gates
         offs = np.array([l % 2 for l in done_layers], dtype=np.int64)
         gates = [layer_gates(j["seed"], j["id"], l, j["n"]) for j, l in zip(jobs, done_layers)]
@@ -201,6 +289,10 @@ class DeviceExecutor:
         return new, {"discarded": disc}
 
     def harvest(self, state, idx, keep_states=False):
+        with self._ctx():
+            return self._harvest(state, idx, keep_states)
+
+    def _harvest(self, state, idx, keep_states=False):
         sub = state.subset(idx)
         n2 = self.B.overlap(sub, sub).cpu().numpy()
         out = {"norm2": [float(v) for v in n2]}

@@ -193,3 +193,21 @@ def test_gram_blocked_many_uneven():
         ref = np.sqrt(np.linalg.eigvalsh(G0)[::-1].clip(min=0))
         assert np.abs(s - ref).max() <= 1e-12 * ref[0]
         assert np.abs(R.T @ np.diag(s ** 2) @ R - G0).max() <= 1e-11 * ref[0] ** 2
+
+
+def test_served_lanes_alone_equals_in_mix():
+    """Lanes (one host thread and HIP stream each) give every job the same
+    result as when it runs alone."""
+    jobs = W.serving_jobs(6, 1e6, 8, n_range=(6, 12), layer_range=(2, 5), chis=(4, 8, 16), cutoff=1e-10)
+    lanes = lambda j: 0 if j["chi_max"] <= 8 else 1
+    rep = serving.Server(serving.DeviceExecutor.lanes(), hbm_budget=10 ** 11, keep_states=True,
+                         lanes=lanes).run(jobs)
+    torch.cuda.synchronize()
+    mix = {r.id: r for r in rep.results}
+    for j in jobs:
+        solo = serving.Server(serving.DeviceExecutor(), hbm_budget=10 ** 11, keep_states=True).run(
+            [dict(j, arrival=0.0)])
+        a, m = solo.results[0], mix[j["id"]]
+        da, dm = oref.dense_mps(a.cores), oref.dense_mps(m.cores)
+        assert np.abs(da - dm).max() <= 1e-12
+        assert abs(m.norm2 + m.discarded - 1.0) <= 1e-10

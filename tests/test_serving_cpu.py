@@ -157,3 +157,61 @@ def test_job_larger_than_budget_is_rejected():
 
 def test_job_bytes_formula():
     assert serving.job_bytes(100, 256) == serving.WORKING_SET_FACTOR * 8 * 100 * 2 * 256 * 256
+
+
+class SleepExecutor:
+    """Stand-in lane executor on the real clock: a layer costs `cost(job)`
+    seconds for its heaviest job (lock-step within the lane)."""
+
+    def __init__(self, log, lock):
+        self.log, self.lock = log, lock
+
+    def empty(self):
+        return []
+
+    def init(self, jobs):
+        return [j["id"] for j in jobs]
+
+    def merge(self, a, b):
+        return a + b
+
+    def select(self, state, idx):
+        return [state[i] for i in idx]
+
+    def layer(self, state, jobs, done):
+        import time as _t
+        _t.sleep(max(0.002 * (j["chi_max"] // 64) for j in jobs))
+        with self.lock:
+            self.log.extend((j["id"], l) for j, l in zip(jobs, done))
+        return state, {"discarded": [0.0] * len(state)}
+
+    def harvest(self, state, idx, keep_states=False):
+        return {"norm2": [1.0] * len(idx)}
+
+
+def test_lanes_decouple_light_jobs_from_heavy_ones():
+    import threading
+    log, lock = [], threading.Lock()
+    jobs = [_job(0, 0.0, layers=10, chi=512)] + [_job(i, 0.0, layers=3, chi=64) for i in range(1, 6)]
+    one = serving.job_bytes(16, 512)
+    srv = serving.Server(lambda key: SleepExecutor(log, lock), hbm_budget=2 * one, lanes=serving.chi_lanes)
+    rep = srv.run(jobs)
+    r = {x.id: x for x in rep.results}
+    # every job ran its layers in order
+    for j in jobs:
+        assert [l for i, l in log if i == j["id"]] == list(range(j["layers"]))
+    # light jobs (3 layers of ~2 ms) do not wait for the heavy one (10 x ~16 ms)
+    assert max(r[i].finished for i in range(1, 6)) < r[0].finished
+    assert rep.peak_bytes <= 2 * one
+    assert rep.site_layers == 16 * (10 + 5 * 3)
+    assert serving.chi_lanes({"chi_max": 64}) == 0 and serving.chi_lanes({"chi_max": 512}) == 2
+
+
+def test_lane_errors_propagate():
+    class Boom(SleepExecutor):
+        def layer(self, state, jobs, done):
+            raise RuntimeError("boom")
+    import threading
+    srv = serving.Server(lambda key: Boom([], threading.Lock()), hbm_budget=10 ** 12, lanes=serving.chi_lanes)
+    with pytest.raises(RuntimeError, match="boom"):
+        srv.run([_job(0, 0.0, chi=64), _job(1, 0.0, chi=512)])
