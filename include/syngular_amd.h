@@ -237,7 +237,7 @@ int64_t syn_jacobi_blocked_many_workspace(const int64_t *nv, const int64_t *len,
 /* rounding.  Per problem: g (nv x nv, ldg) is diagonalised IN PLACE;      */
 /* on exit rt (nv x nv, ldr) holds the eigenvectors as rows sorted by     */
 /* decreasing eigenvalue (G_in = rt^T diag(lambda) rt) and s[i] =          */
-/* sqrt(max(lambda_i, 0)).  Blocks of 16 rows; every step is one launch   */
+/* sqrt(max(lambda_i, 0)).  Blocks of 16 or 32 rows; every step is one launch */
 /* of pair solves (LDS, one wave each) and one launch of MFMA tile         */
 /* updates covering all problems still rotating.  `items` is a DEVICE     */
 /* table of n_items entries and `items_host` the same table on the host;  */
@@ -262,6 +262,9 @@ int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_h
 int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items);
 /* inner Jacobi sweeps per pair solve (1..8, default 1) */
 void syn_tune_gram_inner_sweeps(int sweeps);
+/* rows per block (16 or 32; 0 = 32 when the largest problem of a call has */
+/* at least 384 rows, else 16) */
+void syn_tune_gram_block_rows(int rows);
 
 /* Batched copy: dst[0..n) = src[0..n) for every item (compaction). */
 typedef struct {

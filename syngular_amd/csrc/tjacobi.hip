@@ -30,9 +30,9 @@
 
 namespace syn {
 
-constexpr int kTB = 16;              // rows per block
-constexpr int kTP = 2 * kTB;         // rows of a pair = tile edge
-constexpr int kTSolveThreads = 256;  // one thread per 2 x 2 block of a round
+// Blocks of TB rows (16 or 32, chosen per call); a pair tile has TP = 2 TB
+// rows.  The pair solve uses one thread per 2 x 2 block of a round, (TP/2)^2
+// threads; the tile update four waves.
 constexpr int kTUpdThreads = 256;    // four waves per tile update
 constexpr int kTCol = 64;            // columns of an rt slab
 
@@ -48,19 +48,21 @@ __device__ __forceinline__ void tj_pair(int64_t step, int64_t w, int64_t np, int
     }
 }
 
-// Matrix row of entry i (0 <= i < kTP) of pair w at a step, -1 if padding.
+// Matrix row of entry i (0 <= i < 2 TB) of pair w at a step, -1 if padding.
+template <int TB>
 __device__ __forceinline__ int tj_row(int64_t step, int64_t w, int64_t np, int64_t nblk, int64_t nv, int i) {
     int64_t a, b;
     tj_pair(step, w, np, a, b);
-    const int64_t blk = i < kTB ? a : b;
-    const int64_t r = blk * kTB + (i % kTB);
+    const int64_t blk = i < TB ? a : b;
+    const int64_t r = blk * TB + (i % TB);
     return (blk < nblk && r < nv) ? (int)r : -1;
 }
 
-// Plane k of round r in the round-robin order over the kTP indices of a
+// Plane k of round r in the round-robin order over the TP indices of a
 // pair tile, as (smaller, larger) index.
-__device__ __forceinline__ void plane32(int r, int k, int &p, int &q) {
-    constexpr int m = kTP - 1;
+template <int TP>
+__device__ __forceinline__ void plane_rr(int r, int k, int &p, int &q) {
+    constexpr int m = TP - 1;
     int a, b;
     if (k == 0) {
         a = r;
@@ -73,13 +75,41 @@ __device__ __forceinline__ void plane32(int r, int k, int &p, int &q) {
     q = a < b ? b : a;
 }
 
+// Rotation test of plane (p, q): |G_pq| > max(tol sqrt(|G_pp G_qq|), floor),
+// in squares (no square root on the critical path of a round).
+__device__ __forceinline__ bool rotates(double apq, double app, double aqq, double tol, double floor_abs) {
+    const double a2 = apq * apq;
+    return a2 > tol * tol * fabs(app * aqq) && fabs(apq) > floor_abs;
+}
+
+// Jacobi rotation annihilating G_pq (Rutishauser's angle, Golub & Van Loan
+// 8.5.2): with d = G_qq - G_pp and e = 2 G_pq, tau = d / e and
+// t = sign(tau) / (|tau| + sqrt(1 + tau^2)) = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)),
+// c = 1 / sqrt(1 + t^2), s = c t.  The scaled form needs one division and one
+// square root for t; operands far from 1 (|d|, |e| outside 1e-150 .. 1e150)
+// take the tau form, which cannot overflow.
+__device__ __forceinline__ void rotation(double apq, double app, double aqq, double &c, double &s) {
+    const double d = aqq - app, e = 2.0 * apq;
+    const double ad = fabs(d), ae = fabs(e), m = fmax(ad, ae);
+    double t;
+    if (m < 1e150 && m > 1e-150) {
+        t = copysign(ae, d * e) / (ad + sqrt(fma(d, d, e * e)));
+    } else {
+        const double tau = d / e, at = fabs(tau);
+        t = copysign(1.0, tau) / (at < 1e150 ? at + sqrt(fma(at, at, 1.0)) : 2.0 * at);
+    }
+    c = 1.0 / sqrt(fma(t, t, 1.0));
+    s = c * t;
+}
+
 struct TjDims {
     int64_t nblk, np, half, ncs;
 };
 
-__device__ __forceinline__ TjDims tj_dims(int64_t nv) {
+template <int TB>
+__host__ __device__ __forceinline__ TjDims tj_dims(int64_t nv) {
     TjDims d;
-    d.nblk = (nv + kTB - 1) / kTB;
+    d.nblk = (nv + TB - 1) / TB;
     d.np = d.nblk + (d.nblk & 1);
     d.half = d.np / 2;
     d.ncs = (nv + kTCol - 1) / kTCol;
@@ -100,74 +130,85 @@ __device__ __forceinline__ int tj_find(const syn_gbj_item *items, int n, int64_t
     return lo;
 }
 
+// Per-problem scratch layout (doubles): W tiles (half * TP^2), the rotation
+// flags of the pairs (half ints), the rotation floor, then the sort scratch.
+template <int TB>
 __device__ __forceinline__ int *tj_flags(const syn_gbj_item &P, int64_t half) {
-    return (int *)(P.w + half * kTP * kTP);
+    return (int *)(P.w + half * (2 * TB) * (2 * TB));
 }
 
 // Absolute rotation floor eps * max_i G_ii (set by the init kernel): Gram
 // entries below it are at the rounding level of forming G, so rotating them
 // changes nothing that the Gram route can resolve, and in a rank-deficient G
 // (a null space at noise level) a purely relative test would never settle.
-__device__ __forceinline__ double *tj_floor(const syn_gbj_item &P, int64_t half) {
-    return P.w + half * kTP * kTP + (half + 1) / 2;
+template <int TB>
+__host__ __device__ __forceinline__ int64_t tj_floor_off(int64_t half) {
+    return half * (2 * TB) * (2 * TB) + (half + 1) / 2;
 }
 
-__global__ __launch_bounds__(kTSolveThreads) void tj_solve_kernel(const syn_gbj_item *__restrict__ items, int n,
-                                                                 int64_t step, double tol, int inner_sweeps) {
-    __shared__ double S[kTP][kTP + 1];
-    __shared__ double W[kTP][kTP + 1];
-    __shared__ double cs[kTP / 2][2];
-    __shared__ int rows[kTP];
-    __shared__ int do_rot;
+template <int TB>
+__global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *__restrict__ items, int n,
+                                                           int64_t step, double tol, int inner_sweeps) {
+    constexpr int TP = 2 * TB, NT = TB * TB;
+    __shared__ double S[TP][TP + 1];
+    __shared__ double W[TP][TP + 1];
+    __shared__ double cs[TP / 2][2];
+    __shared__ int rows[TP];
+    __shared__ int do_rot, any_rot;
     const int64_t bid = blockIdx.x;
     const syn_gbj_item &P = items[tj_find<true>(items, n, bid)];
-    const TjDims D = tj_dims(P.nv);
+    const TjDims D = tj_dims<TB>(P.nv);
     const int64_t pw = bid - P.solve_start;
     const int64_t st = step % (D.np - 1);
     const int tid = threadIdx.x;
-    if (tid < kTP) rows[tid] = tj_row(st, pw, D.np, D.nblk, P.nv, tid);
+    if (tid < TP) rows[tid] = tj_row<TB>(st, pw, D.np, D.nblk, P.nv, tid);
     if (tid == 0) do_rot = 0;
     __syncthreads();
-    for (int e = tid; e < kTP * kTP; e += kTSolveThreads) {
-        const int i = e / kTP, k = e % kTP, ri = rows[i], rk = rows[k];
+    for (int e = tid; e < TP * TP; e += NT) {
+        const int i = e / TP, k = e % TP, ri = rows[i], rk = rows[k];
         S[i][k] = (ri >= 0 && rk >= 0) ? P.g[(int64_t)ri * P.ldg + rk] : 0.0;
         W[i][k] = (i == k) ? 1.0 : 0.0;
     }
     __syncthreads();
-    const double floor_abs = *tj_floor(P, D.half);
-    for (int e = tid; e < kTP * kTP; e += kTSolveThreads) {
-        const int i = e / kTP, k = e % kTP;
-        if (i < k && fabs(S[i][k]) > fmax(tol * sqrt(fabs(S[i][i]) * fabs(S[k][k])), floor_abs)) do_rot = 1;
+    const double floor_abs = P.w[tj_floor_off<TB>(D.half)];
+    for (int e = tid; e < TP * TP; e += NT) {
+        const int i = e / TP, k = e % TP;
+        if (i < k && rotates(S[i][k], S[i][i], S[k][k], tol, floor_abs)) do_rot = 1;
     }
     __syncthreads();
-    int *flags = tj_flags(P, D.half);
+    int *flags = tj_flags<TB>(P, D.half);
     if (!do_rot) {
         if (tid == 0) flags[pw] = 0;
         return;
     }
-    // thread (ka, kb) = (tid / 16, tid % 16) owns the 2 x 2 block of planes
-    // ka (rows) and kb (columns) of each round, and row wi of the columns of
-    // planes wk and wk + 8 of W
-    static_assert(kTSolveThreads == (kTP / 2) * (kTP / 2), "one thread per 2 x 2 block");
-    const int ka = tid / (kTP / 2), kb = tid % (kTP / 2);
-    const int wk = tid / kTP, wi = tid % kTP;  // W: planes wk and wk + 8, row wi
-    __shared__ int any_rot;
+    // thread (ka, kb) = (tid / (TP/2), tid % (TP/2)) owns the 2 x 2 block of
+    // planes ka (rows) and kb (columns) of each round, and row wi of the
+    // columns of planes wk and wk + TP/4 of W
+    const int ka = tid / (TP / 2), kb = tid % (TP / 2);
+    const int wk = tid / TP, wi = tid % TP;
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
         if (tid == 0) any_rot = 0;
         __syncthreads();
-        for (int r = 0; r < kTP - 1; ++r) {
+        for (int r = 0; r < TP - 1; ++r) {
             int p, q, u, v;
-            plane32(r, ka, p, q);
-            plane32(r, kb, u, v);
-            if (tid < kTP / 2) {   // here ka == 0 and kb == tid: plane tid is (u, v)
-                const int p = u, q = v;
-                const double apq = S[p][q], app = S[p][p], aqq = S[q][q];
+            plane_rr<TP>(r, ka, p, q);
+            plane_rr<TP>(r, kb, u, v);
+            // this round's operands were final at the last barrier and phase 1
+            // only reads S: load them now, so their latency overlaps phase 1
+            const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
+            int wp[2], wq[2];
+            double wx[2], wy[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                plane_rr<TP>(r, wk + h * (TP / 4), wp[h], wq[h]);
+                wx[h] = W[wi][wp[h]];
+                wy[h] = W[wi][wq[h]];
+            }
+            if (tid < TP / 2) {   // here ka == 0 and kb == tid: plane tid is (u, v)
+                const double apq = S[u][v], app = S[u][u], aqq = S[v][v];
                 double c = 1.0, sn = 0.0;
-                if (fabs(apq) > fmax(tol * sqrt(fabs(app) * fabs(aqq)), floor_abs)) {
-                    const double zeta = (aqq - app) / (2.0 * apq);
-                    const double t = copysign(1.0, zeta) / (fabs(zeta) + hypot(1.0, zeta));
-                    c = 1.0 / sqrt(fma(t, t, 1.0));
-                    sn = c * t;
+                if (rotates(apq, app, aqq, tol, floor_abs)) {
+                    rotation(apq, app, aqq, c, sn);
                     any_rot = 1;
                 }
                 cs[tid][0] = c;
@@ -177,7 +218,6 @@ __global__ __launch_bounds__(kTSolveThreads) void tj_solve_kernel(const syn_gbj_
             {
                 const double ca = cs[ka][0], sa = cs[ka][1], cb = cs[kb][0], sb = cs[kb][1];
                 if (sa != 0.0 || sb != 0.0) {
-                    const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
                     const double rpu = ca * spu - sa * squ, rpv = ca * spv - sa * sqv;
                     const double rqu = sa * spu + ca * squ, rqv = sa * spv + ca * sqv;
                     S[p][u] = cb * rpu - sb * rpv;
@@ -191,16 +231,13 @@ __global__ __launch_bounds__(kTSolveThreads) void tj_solve_kernel(const syn_gbj_
                 }
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {  // W <- W J (columns of plane wk + 8h)
-                const int k = wk + 8 * h;
+            for (int h = 0; h < 2; ++h) {  // W <- W J (columns of plane wk + h TP/4)
+                const int k = wk + h * (TP / 4);
                 const double sn = cs[k][1];
                 if (sn != 0.0) {
                     const double c = cs[k][0];
-                    int pp, qq;
-                    plane32(r, k, pp, qq);
-                    const double x = W[wi][pp], y = W[wi][qq];
-                    W[wi][pp] = c * x - sn * y;
-                    W[wi][qq] = sn * x + c * y;
+                    W[wi][wp[h]] = c * wx[h] - sn * wy[h];
+                    W[wi][wq[h]] = sn * wx[h] + c * wy[h];
                 }
             }
             __syncthreads();
@@ -210,8 +247,8 @@ __global__ __launch_bounds__(kTSolveThreads) void tj_solve_kernel(const syn_gbj_
         __syncthreads();
         if (!any) break;
     }
-    double *Wout = P.w + pw * kTP * kTP;
-    for (int e = tid; e < kTP * kTP; e += kTSolveThreads) Wout[e] = W[e / kTP][e % kTP];
+    double *Wout = P.w + pw * TP * TP;
+    for (int e = tid; e < TP * TP; e += NT) Wout[e] = W[e / TP][e % TP];
     if (tid == 0) {
         flags[pw] = 1;
         atomicAdd(P.counter, 1);
@@ -220,65 +257,80 @@ __global__ __launch_bounds__(kTSolveThreads) void tj_solve_kernel(const syn_gbj_
 
 // MFMA f64 16x16x4 layout: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
 // D reg r of lane l = D[l/16 + 4r][l%16].
-// bytes moved by one executed tile: a G tile read and written, or an rt slab
-constexpr unsigned long long kTileBytes = 2ull * kTP * kTP * sizeof(double);
-constexpr unsigned long long kSlabBytes = 2ull * kTP * kTCol * sizeof(double);
-
+template <int TB>
 __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_item *__restrict__ items, int n,
                                                                   int64_t step, unsigned long long *bytes) {
-    __shared__ double Wa[kTP][kTP + 1];
-    __shared__ double Wb[kTP][kTP + 1];
-    __shared__ double X[kTP][kTCol + 1];   // G tile (kTP columns used) or rt slab
-    __shared__ double Tt[kTP][kTP + 1];
-    __shared__ int ra[kTP], rb[kTP];
+    constexpr int TP = 2 * TB, NS = TP / 16;            // 16 x 16 sub-tiles per edge
+    constexpr unsigned long long kTileBytes = (unsigned long long)TP * TP * sizeof(double);
+    constexpr unsigned long long kSlabBytes = 2ull * TP * kTCol * sizeof(double);
+    __shared__ double Wa[TP][TP + 1];
+    __shared__ double Wb[TP][TP + 1];
+    __shared__ double X[TP][kTCol + 1];   // G tile (TP columns used) or rt slab
+    __shared__ double Tt[TP][TP + 1];
+    __shared__ int ra[TP], rb[TP];
+    static_assert(TP <= kTCol, "a G tile must fit the slab buffer");
     const int64_t bid = blockIdx.x;
     const syn_gbj_item &P = items[tj_find<false>(items, n, bid)];
-    const TjDims D = tj_dims(P.nv);
+    const TjDims D = tj_dims<TB>(P.nv);
     const int64_t st = step % (D.np - 1);
     const int64_t t = bid - P.tile_start;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
-    const int *flags = tj_flags(P, D.half);
+    const int *flags = tj_flags<TB>(P, D.half);
     if (t < D.half * D.half) {
         // ---- G tile (a, b) -------------------------------------------------
+        // G stays symmetric: the work-group of (a, b), a <= b, also writes
+        // (b, a) as the transpose; those of a > b have nothing to do
         const int64_t a = t / D.half, b = t % D.half;
+        if (a > b) return;
         const int fa = flags[a], fb = flags[b];
         if (!fa && !fb) return;
-        if (tid == 0) atomicAdd(bytes, kTileBytes);
-        if (tid < kTP) {
-            ra[tid] = tj_row(st, a, D.np, D.nblk, P.nv, tid);
-            rb[tid] = tj_row(st, b, D.np, D.nblk, P.nv, tid);
+        if (tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);  // read one tile, write one or two
+        if (tid < TP) {
+            ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
+            rb[tid] = tj_row<TB>(st, b, D.np, D.nblk, P.nv, tid);
         }
         __syncthreads();
-        const double *wa = P.w + a * kTP * kTP, *wb = P.w + b * kTP * kTP;
-        for (int e = tid; e < kTP * kTP; e += kTUpdThreads) {
-            const int i = e / kTP, k = e % kTP;
+        const double *wa = P.w + a * TP * TP, *wb = P.w + b * TP * TP;
+        for (int e = tid; e < TP * TP; e += kTUpdThreads) {
+            const int i = e / TP, k = e % TP;
             Wa[i][k] = fa ? wa[e] : (i == k ? 1.0 : 0.0);
             Wb[i][k] = fb ? wb[e] : (i == k ? 1.0 : 0.0);
             const int r = ra[i], c = rb[k];
             X[i][k] = (r >= 0 && c >= 0) ? P.g[(int64_t)r * P.ldg + c] : 0.0;
         }
         __syncthreads();
-        const int ti = w >> 1, tj = w & 1;
-        // T = Wa^T X
-        v4d acc = {0.0, 0.0, 0.0, 0.0};
+        // T = Wa^T X, sub-tiles dealt to the waves
+        for (int q = w; q < NS * NS; q += kTUpdThreads / 64) {
+            const int ti = q / NS, tj = q % NS;
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k0 = 0; k0 < kTP; k0 += 4)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc, 0, 0,
-                                                        0);
+            for (int k0 = 0; k0 < TP; k0 += 4)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc,
+                                                            0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Tt[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
+            for (int r = 0; r < 4; ++r) Tt[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
+        }
         __syncthreads();
-        // G_ab = T Wb
-        acc = v4d{0.0, 0.0, 0.0, 0.0};
+        // G_ab = T Wb, into X (the input tile is no longer needed)
+        for (int q = w; q < NS * NS; q += kTUpdThreads / 64) {
+            const int ti = q / NS, tj = q % NS;
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k0 = 0; k0 < kTP; k0 += 4)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tt[ti * 16 + li][k0 + lk], Wb[k0 + lk][tj * 16 + li], acc, 0, 0,
-                                                        0);
+            for (int k0 = 0; k0 < TP; k0 += 4)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tt[ti * 16 + li][k0 + lk], Wb[k0 + lk][tj * 16 + li], acc,
+                                                            0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = ti * 16 + lk + 4 * r, k = tj * 16 + li;
+            for (int r = 0; r < 4; ++r) X[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
+        }
+        __syncthreads();
+        for (int e = tid; e < TP * TP; e += kTUpdThreads) {
+            const int i = e / TP, k = e % TP;
             const int gr = ra[i], gc = rb[k];
-            if (gr >= 0 && gc >= 0) P.g[(int64_t)gr * P.ldg + gc] = acc[r];
+            if (gr >= 0 && gc >= 0) P.g[(int64_t)gr * P.ldg + gc] = X[i][k];
+            if (a != b) {
+                const int tr = rb[i], tc = ra[k];
+                if (tr >= 0 && tc >= 0) P.g[(int64_t)tr * P.ldg + tc] = X[k][i];
+            }
         }
         return;
     }
@@ -287,22 +339,21 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
     const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
     if (!flags[a]) return;
     if (tid == 0) atomicAdd(bytes, kSlabBytes);
-    if (tid < kTP) ra[tid] = tj_row(st, a, D.np, D.nblk, P.nv, tid);
+    if (tid < TP) ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
     __syncthreads();
-    const double *wa = P.w + a * kTP * kTP;
-    for (int e = tid; e < kTP * kTP; e += kTUpdThreads) Wa[e / kTP][e % kTP] = wa[e];
-    for (int e = tid; e < kTP * kTCol; e += kTUpdThreads) {
+    const double *wa = P.w + a * TP * TP;
+    for (int e = tid; e < TP * TP; e += kTUpdThreads) Wa[e / TP][e % TP] = wa[e];
+    for (int e = tid; e < TP * kTCol; e += kTUpdThreads) {
         const int i = e / kTCol, j = e % kTCol, r = ra[i];
         X[i][j] = (r >= 0 && c0 + j < P.nv) ? P.rt[(int64_t)r * P.ldr + c0 + j] : 0.0;
     }
     __syncthreads();
-    // rt_a = Wa^T X: 2 x 4 output tiles of 16 x 16, two per wave
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int tile = w * 2 + h, ti = tile >> 2, tj = tile & 3;
+    // rt_a = Wa^T X: NS x 4 sub-tiles of 16 x 16 dealt to the waves
+    for (int q = w; q < NS * (kTCol / 16); q += kTUpdThreads / 64) {
+        const int ti = q / (kTCol / 16), tj = q % (kTCol / 16);
         v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k0 = 0; k0 < kTP; k0 += 4)
+        for (int k0 = 0; k0 < TP; k0 += 4)
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc, 0, 0,
                                                         0);
 #pragma unroll
@@ -314,6 +365,7 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
     }
 }
 
+template <int TB>
 __global__ __launch_bounds__(256) void tj_init_kernel(const syn_gbj_item *__restrict__ items) {
     __shared__ double red[4];
     const syn_gbj_item &P = items[blockIdx.x];
@@ -328,7 +380,8 @@ __global__ __launch_bounds__(256) void tj_init_kernel(const syn_gbj_item *__rest
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
-        *tj_floor(P, tj_dims(P.nv).half) = __DBL_EPSILON__ * fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        P.w[tj_floor_off<TB>(tj_dims<TB>(P.nv).half)] =
+            __DBL_EPSILON__ * fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
 __global__ __launch_bounds__(256) void tj_diag_kernel(const syn_gbj_item *__restrict__ items) {
@@ -367,52 +420,31 @@ __global__ __launch_bounds__(256) void tj_zero_kernel(int32_t *c, int n) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) c[i] = 0;
 }
 
-static int64_t tj_half(int64_t nv) {
-    const int64_t nblk = (nv + kTB - 1) / kTB;
-    return (nblk + (nblk & 1)) / 2;
+// per-problem workspace: W tiles, flags, floor, then the sort scratch (rt copy, s)
+template <int TB>
+static int64_t tj_work(int64_t nv) {
+    const int64_t half = tj_dims<TB>(nv).half;
+    return tj_floor_off<TB>(half) + 1 + nv * nv + nv + 4;
 }
 
-// per-problem workspace: W tiles, flags, then the sort scratch (rt copy, s)
-static int64_t tj_work(int64_t nv) {
-    const int64_t half = tj_half(nv);
-    return half * kTP * kTP + (half + 1) / 2 + 1 + nv * nv + nv + 4;
+static int64_t tj_work_max(int64_t nv) {
+    const int64_t a = tj_work<16>(nv), b = tj_work<32>(nv);
+    return a > b ? a : b;
 }
 
 static int g_tj_inner = 1;
+static int g_tj_block = 0;   // 0: 32-row blocks when the largest problem has >= 384 rows, else 16
 
-}  // namespace syn
-
-extern "C" void syn_tune_gram_inner_sweeps(int sweeps) { syn::g_tj_inner = (sweeps >= 1 && sweeps <= 8) ? sweeps : 1; }
-
-extern "C" int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items) {
-    int64_t tot = 0;
-    for (int i = 0; i < n_items; ++i) tot += syn::tj_work(nv[i]) + 2 * nv[i];
-    return tot + 2 * (int64_t)n_items + 16;
-}
-
-extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_host, int n_items,
-                                           double *work, int32_t *counters, double tol, int max_sweeps, int *sweeps,
-                                           void *stream) {
-    using namespace syn;
-    if (n_items < 0 || !items || (n_items && (!items_host || !work || !counters)) || max_sweeps < 0 ||
-        !(tol >= 0.0)) {
-        set_error("syn_jacobi_gram_blocked_f64: bad arguments");
-        return SYN_EINVAL;
-    }
-    if (n_items == 0) return SYN_OK;
-    hipStream_t st = (hipStream_t)stream;
-    std::vector<syn_gbj_item> all(items_host, items_host + n_items);
+template <int TB>
+static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
+                        int32_t *counters, double tol, int max_sweeps, int *sweeps, hipStream_t st) {
+    constexpr int TP = 2 * TB;
     std::vector<int64_t> woff(n_items);
     int64_t tot_w = 0;
     for (int i = 0; i < n_items; ++i) {
-        if (all[i].nv <= 0 || all[i].ldg < all[i].nv || all[i].ldr < all[i].nv || !all[i].g || !all[i].rt ||
-            !all[i].s) {
-            set_error("syn_jacobi_gram_blocked_f64: bad problem %d (nv=%lld)", i, (long long)all[i].nv);
-            return SYN_EINVAL;
-        }
         woff[i] = tot_w;
         all[i].w = work + tot_w;
-        tot_w += tj_work(all[i].nv);
+        tot_w += tj_work<TB>(all[i].nv);
         all[i].counter = counters + i;
         all[i].active = all[i].nv > 1;
         all[i].solve_start = all[i].tile_start = 0;
@@ -424,12 +456,12 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
         return hip_check(hipStreamSynchronize(st), what);
     };
     if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
-    hipLaunchKernelGGL(tj_init_kernel, dim3(n_items), dim3(256), 0, st, items);
+    hipLaunchKernelGGL(tj_init_kernel<TB>, dim3(n_items), dim3(256), 0, st, items);
     SYN_LAUNCH_CHECK("gram init");
     // byte counter of the update launches, after the sort scratch
-    int64_t tot_nv0 = 0;
-    for (int i = 0; i < n_items; ++i) tot_nv0 += all[i].nv;
-    unsigned long long *dbytes = (unsigned long long *)(work + tot_w + tot_nv0 + 2 * (int64_t)n_items);
+    int64_t tot_nv = 0;
+    for (int i = 0; i < n_items; ++i) tot_nv += all[i].nv;
+    unsigned long long *dbytes = (unsigned long long *)(work + tot_w + tot_nv + 2 * (int64_t)n_items);
     hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, (int32_t *)dbytes, 2);
     SYN_LAUNCH_CHECK("gram byte counter");
 
@@ -443,12 +475,12 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
         int64_t ss = 0, ts = 0, max_np = 2;
         for (int i : idx) {
             syn_gbj_item it = all[i];
-            const int64_t half = tj_half(it.nv), ncs = (it.nv + kTCol - 1) / kTCol;
+            const TjDims D = tj_dims<TB>(it.nv);
             it.solve_start = ss;
             it.tile_start = ts;
-            ss += half;
-            ts += half * half + half * ncs;
-            if (2 * half > max_np) max_np = 2 * half;
+            ss += D.half;
+            ts += D.half * D.half + D.half * D.ncs;
+            if (D.np > max_np) max_np = D.np;
             act.push_back(it);
         }
         if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
@@ -456,12 +488,12 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
         for (int64_t step = 0; step < max_np - 1; ++step) {
             {
                 ProfScope ps(kTagGramSolve, 0.0, st);
-                hipLaunchKernelGGL(tj_solve_kernel, dim3((unsigned)ss), dim3(kTSolveThreads), 0, st, items,
+                hipLaunchKernelGGL(tj_solve_kernel<TB>, dim3((unsigned)ss), dim3(TB * TB), 0, st, items,
                                    (int)act.size(), step, tol, g_tj_inner);
             }
             {
                 ProfScope ps(kTagGramUpdate, 0.0, st);
-                hipLaunchKernelGGL(tj_update_kernel, dim3((unsigned)ts), dim3(kTUpdThreads), 0, st, items,
+                hipLaunchKernelGGL(tj_update_kernel<TB>, dim3((unsigned)ts), dim3(kTUpdThreads), 0, st, items,
                                    (int)act.size(), step, dbytes);
             }
         }
@@ -486,12 +518,12 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
     if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
     hipLaunchKernelGGL(tj_diag_kernel, dim3(n_items), dim3(256), 0, st, items);
     SYN_LAUNCH_CHECK("gram diag");
-    int64_t tot_nv = 0;
     std::vector<int64_t> poff(n_items), toff(n_items);
+    int64_t acc_nv = 0;
     for (int i = 0; i < n_items; ++i) {
-        poff[i] = tot_nv;
-        tot_nv += all[i].nv;
-        toff[i] = woff[i] + tj_half(all[i].nv) * kTP * kTP + (tj_half(all[i].nv) + 1) / 2 + 1;
+        poff[i] = acc_nv;
+        acc_nv += all[i].nv;
+        toff[i] = woff[i] + tj_floor_off<TB>(tj_dims<TB>(all[i].nv).half) + 1;
     }
     std::vector<double> hs(tot_nv);
     for (int i = 0; i < n_items; ++i)
@@ -523,5 +555,44 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
         if (int rc = hip_check(hipStreamSynchronize(st), "gram bytes sync")) return rc;
         prof_add_work(kTagGramUpdate, (double)hb);
     }
+    (void)TP;
     return hip_check(hipStreamSynchronize(st), "gram final sync");
+}
+
+}  // namespace syn
+
+extern "C" void syn_tune_gram_inner_sweeps(int sweeps) { syn::g_tj_inner = (sweeps >= 1 && sweeps <= 8) ? sweeps : 1; }
+
+extern "C" void syn_tune_gram_block_rows(int rows) { syn::g_tj_block = (rows == 16 || rows == 32) ? rows : 0; }
+
+extern "C" int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items) {
+    int64_t tot = 0;
+    for (int i = 0; i < n_items; ++i) tot += syn::tj_work_max(nv[i]) + 2 * nv[i];
+    return tot + 2 * (int64_t)n_items + 16;
+}
+
+extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_host, int n_items,
+                                           double *work, int32_t *counters, double tol, int max_sweeps, int *sweeps,
+                                           void *stream) {
+    using namespace syn;
+    if (n_items < 0 || !items || (n_items && (!items_host || !work || !counters)) || max_sweeps < 0 ||
+        !(tol >= 0.0)) {
+        set_error("syn_jacobi_gram_blocked_f64: bad arguments");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0) return SYN_OK;
+    std::vector<syn_gbj_item> all(items_host, items_host + n_items);
+    int64_t max_nv = 0;
+    for (int i = 0; i < n_items; ++i) {
+        if (all[i].nv <= 0 || all[i].ldg < all[i].nv || all[i].ldr < all[i].nv || !all[i].g || !all[i].rt ||
+            !all[i].s) {
+            set_error("syn_jacobi_gram_blocked_f64: bad problem %d (nv=%lld)", i, (long long)all[i].nv);
+            return SYN_EINVAL;
+        }
+        if (all[i].nv > max_nv) max_nv = all[i].nv;
+    }
+    const int tb = g_tj_block ? g_tj_block : (max_nv >= 384 ? 32 : 16);
+    hipStream_t st = (hipStream_t)stream;
+    if (tb == 32) return gram_blocked<32>(items, all, n_items, work, counters, tol, max_sweeps, sweeps, st);
+    return gram_blocked<16>(items, all, n_items, work, counters, tol, max_sweeps, sweeps, st);
 }

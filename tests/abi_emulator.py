@@ -319,9 +319,12 @@ def _workspace(nv, ln):
 def _gram_workspace(nv):
     tot = 0
     for v in np.asarray(nv, np.int64):
-        nblk = (int(v) + 15) // 16
-        half = (nblk + (nblk & 1)) // 2
-        tot += half * 32 * 32 + (half + 1) // 2 + 1 + int(v) * int(v) + int(v) + 4 + 2 * int(v)
+        work = []
+        for tb in (16, 32):
+            nblk = (int(v) + tb - 1) // tb
+            half = (nblk + (nblk & 1)) // 2
+            work.append(half * 4 * tb * tb + (half + 1) // 2 + 1 + int(v) * int(v) + int(v) + 4)
+        tot += max(work) + 2 * int(v)
     return tot + 2 * len(nv) + 16
 
 

@@ -296,17 +296,22 @@ def test_jacobi_gram_rejects_oversize():
         ops.jacobi_gram_many([T(np.eye(65))])
 
 
+@pytest.mark.parametrize("block", [16, 32])
 @pytest.mark.parametrize("g,rank", [(65, 65), (100, 100), (257, 257), (300, 120), (512, 512)])
-def test_jacobi_gram_blocked(g, rank):
+def test_jacobi_gram_blocked(g, rank, block):
     """Two-sided block Jacobi of a Gram matrix: eigenvalues = squared
     singular values, orthonormal sorted eigenvector rows, reconstruction."""
-    from syngular_amd import ops
+    from syngular_amd import _lib, ops
 
-    case(f"jacobi gram blocked g={g} rank={rank}")
+    case(f"jacobi gram blocked g={g} rank={rank} block={block}")
     rng = np.random.default_rng(g + rank)
     C = rng.standard_normal((g + 7, rank)) @ rng.standard_normal((rank, g))
     G0 = C.T @ C
-    rt, s, sweeps = ops.jacobi_gram_blocked(T(G0))
+    _lib.load().syn_tune_gram_block_rows(block)
+    try:
+        rt, s, sweeps = ops.jacobi_gram_blocked(T(G0))
+    finally:
+        _lib.load().syn_tune_gram_block_rows(0)
     s = s.cpu().numpy()
     R = rt.cpu().numpy()
     ref = np.linalg.svd(C, compute_uv=False)
