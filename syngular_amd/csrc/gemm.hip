@@ -7,7 +7,10 @@
 // 16 lanes that feed one MFMA row read 16 consecutive doubles; rows are padded
 // by one double to spread the transposed stores over the banks.  The K loop is
 // double-buffered: the next global tile is loaded into registers while the
-// current LDS tile feeds the MFMAs.
+// current LDS tile feeds the MFMAs.  Each operand orientation is a separate
+// instance (no per-element branches), which keeps a 128 x 128 tile within the
+// register budget of two work-groups per compute unit, so one work-group's
+// barrier and load waits are covered by the other's MFMAs.
 //
 // MFMA f64 16x16x4 operand layout (lane l of 64):
 //   A: A[i = l%16][k = l/16]     B: B[k = l/16][j = l%16]
@@ -21,14 +24,20 @@ namespace syn {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <int BM, int BN, int BK, int WM, int WN>
-__device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t) {
+struct GemmSmem {
+    double As[2][BK][BM + 1];
+    double Bs[2][BK][BN + 1];
+};
+
+template <int BM, int BN, int BK, int WM, int WN, bool TA, bool TB>
+__device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t t, GemmSmem<BM, BN, BK, WM, WN> &sm) {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NT = NW * 64;
     constexpr int TI = WM / 16, TJ = WN / 16;
     constexpr int EA = BM * BK / NT, EB = BK * BN / NT;
     static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "tile/threads mismatch");
-    __shared__ double As[2][BK][BM + 1];
-    __shared__ double Bs[2][BK][BN + 1];
+    auto &As = sm.As;
+    auto &Bs = sm.Bs;
 
     const int64_t row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
     const int tri = P.tri;
@@ -41,7 +50,7 @@ __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t)
     if (tri & 4) kmin = row0;
     if (tri & 16) kmin = kmin > col0 ? kmin : col0;
     kmin = (kmin / BK) * BK;
-    const int ta = P.transA, tb = P.transB;
+    constexpr bool ta = TA, tb = TB;
     const double *__restrict__ A = (const double *)P.A;
     const double *__restrict__ B = (const double *)P.B;
 
@@ -54,17 +63,23 @@ __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t)
         for (int s = 0; s < EA; ++s) {
             int e = tid + NT * s;
             int r, c;
-            if (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
+            if constexpr (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
             int64_t gi = row0 + r, gk = k0 + c;
-            ra[s] = (gi < m && gk < k) ? (ta ? A[gk * lda + gi] : A[gi * lda + gk]) : 0.0;
+            if constexpr (ta)
+                ra[s] = (gi < m && gk < k) ? A[gk * lda + gi] : 0.0;
+            else
+                ra[s] = (gi < m && gk < k) ? A[gi * lda + gk] : 0.0;
         }
 #pragma unroll
         for (int s = 0; s < EB; ++s) {
             int e = tid + NT * s;
             int j, c;
-            if (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
+            if constexpr (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
             int64_t gj = col0 + j, gk = k0 + c;
-            rb[s] = (gj < n && gk < k) ? (tb ? B[gj * ldb + gk] : B[gk * ldb + gj]) : 0.0;
+            if constexpr (tb)
+                rb[s] = (gj < n && gk < k) ? B[gj * ldb + gk] : 0.0;
+            else
+                rb[s] = (gj < n && gk < k) ? B[gk * ldb + gj] : 0.0;
         }
     };
     auto store = [&](int buf) {
@@ -72,14 +87,14 @@ __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t)
         for (int s = 0; s < EA; ++s) {
             int e = tid + NT * s;
             int r, c;
-            if (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
+            if constexpr (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
             As[buf][c][r] = ra[s];
         }
 #pragma unroll
         for (int s = 0; s < EB; ++s) {
             int e = tid + NT * s;
             int j, c;
-            if (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
+            if constexpr (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
             Bs[buf][c][j] = rb[s];
         }
     };
@@ -98,7 +113,7 @@ __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t)
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < nk) load(kmin + (kt + 1) * BK);
-#pragma unroll
+#pragma unroll 1
         for (int kk = 0; kk < BK / 4; ++kk) {
             double a[TI], b[TJ];
 #pragma unroll
@@ -134,18 +149,60 @@ __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t)
             }
 }
 
+// The operand orientation is fixed at compile time in each instance, so the
+// loads carry no per-element branches and the register budget fits two
+// work-groups per compute unit.
 template <int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_grouped(
+__device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t) {
+    __shared__ GemmSmem<BM, BN, BK, WM, WN> sm;
+    if (!P.transA) {
+        if (!P.transB)
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, false>(P, t, sm);
+        else
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, true>(P, t, sm);
+    } else {
+        if (!P.transB)
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, false>(P, t, sm);
+        else
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, true>(P, t, sm);
+    }
+}
+
+// Tile order of a single problem.  Work-groups are dealt to the XCDs round
+// robin (id % 8), and each XCD has its own L2.  XCD x gets whole tile rows
+// (or columns), zig-zag paired within every 16: rows x and 15 - x, then 16 + x
+// and 31 - x, ...  A whole row reuses its A panel in one L2, and when the
+// work of a tile grows or shrinks along the rows (a triangular hint) each XCD
+// gets the same sum of long and short rows.  Columns are dealt instead when
+// the hint makes the work depend on the column (op(B) triangular).  Shapes
+// that do not split into such groups keep contiguous ranges per XCD.
+__device__ __forceinline__ int64_t tile_of_block(int64_t bid, int64_t total, int64_t tiles_n, int tri) {
+    const int64_t tiles_m = total / tiles_n;
+    const bool by_col = (tri & (8 | 16)) && !(tri & (1 | 2 | 4));
+    const int64_t lines = by_col ? tiles_n : tiles_m, len = by_col ? tiles_m : tiles_n;
+    if (lines % (2 * kXcds) != 0 || total < 2 * kXcds) return xcd_remap(bid, total);
+    const int64_t x = bid % kXcds, j = bid / kXcds, u = j / len, g = u / 2;
+    const int64_t line = 2 * kXcds * g + ((u & 1) ? 2 * kXcds - 1 - x : x), pos = j % len;
+    return by_col ? pos * tiles_n + line : line * tiles_n + pos;
+}
+
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_grouped(
     const syn_gemm_item *__restrict__ items, int n_items, int64_t total) {
+    if (n_items == 1) {
+        gemm_f64_tile<BM, BN, BK, WM, WN>(items[0],
+                                          tile_of_block(blockIdx.x, total, items[0].tiles_n, items[0].tri));
+        return;
+    }
     const int64_t bid = xcd_remap(blockIdx.x, total);
     const syn_gemm_item &P = items[find_item(items, n_items, bid)];
     gemm_f64_tile<BM, BN, BK, WM, WN>(P, bid - P.block_start);
 }
 
 template <int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_single(syn_gemm_item P) {
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_single(syn_gemm_item P) {
     const int64_t total = (int64_t)gridDim.x;
-    gemm_f64_tile<BM, BN, BK, WM, WN>(P, xcd_remap(blockIdx.x, total));
+    gemm_f64_tile<BM, BN, BK, WM, WN>(P, tile_of_block(blockIdx.x, total, P.tiles_n, P.tri));
 }
 
 // One GEMM launched with its descriptor passed by value (no device table):
