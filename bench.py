@@ -65,6 +65,12 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--chain-only", action="store_true", help="only the headline chain (for profiling)")
+    ap.add_argument("--cholqr-probe", action="store_true",
+                    help="only the copy probe and one CholeskyQR pass on an 8192 x 4096 unfolding (the chain's "
+                         "left-sweep shape, whose GEMMs are the dominant kernel), for short counter runs")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB device copy (syn_copy_f64) three times: the known-size "
+                         "reference that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run")
     return ap.parse_args()
 
 
@@ -85,6 +91,14 @@ def main():
     from syngular_amd import workloads as W
 
     dev = ops.device()
+    if args.copy_probe or args.cholqr_probe:
+        copy_probe(np, torch)
+    if args.cholqr_probe:
+        A = torch.randn(8192, 4096, dtype=torch.float64, device=dev) / 64.0
+        ops.cholqr(A)
+        torch.cuda.synchronize()
+        log("cholqr probe: one CholeskyQR pass on 8192 x 4096")
+        return
     n, d, chi, dw = args.sites, args.phys, args.chi, args.bond_mpo
     xs, os_ = W.chain_inputs(n, d, chi, dw, seed=2024)
     X = [torch.from_numpy(x).to(dev) for x in xs]
@@ -174,7 +188,7 @@ def roofline(prof, steps, ms_per_step, W, n, chi, dw, d, rep):
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAKS["fp64_mfma_tflops"],
                 "unit": "TFLOP/s"}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(dom)
     roof["kernel"] = dom
     roof["launches_per_step"] = dl / steps
     roof["avg_launch_us"] = round(1e3 * dms / dl, 2)
@@ -188,6 +202,20 @@ def roofline(prof, steps, ms_per_step, W, n, chi, dw, d, rep):
                                 "rate_unit": "GB/s" if k in BYTE_KERNELS else "TFLOP/s"}
                             for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}}
     return roof, pipeline
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a known-size copy), or None
+    when no counter run covers it.  The counters cannot be read from inside
+    this process; profiles/gemm_f64_single_128_traffic.md has the commands."""
+    path = os.path.join(ROOT, "profiles", "gemm_f64_single_128_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return rec["bytes_per_launch"] if rec.get("kernel") == kernel else None
 
 
 # kernels whose profiler work figure is bytes moved (the rest: flops)
@@ -369,6 +397,20 @@ def serve_form(np, torch, args):
             "oracle_jobs": [f"N={j['n']} layers={j['layers']} chi_max={j['chi_max']}" for j in cheap],
             "norm2_rel_err_max": max(e[0] for e in errs),
             "overlap_rel_err_max": max(e[1] for e in errs)}
+
+
+def copy_probe(np, torch):
+    """Three launches of the library's copy kernel over 256 MiB (reads 256 MiB,
+    writes 256 MiB each): the reference of known traffic for counter runs."""
+    from syngular_amd import batch as B
+    n = 32 * 2 ** 20                       # doubles = 256 MiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        B.copy_table(np.array([src.data_ptr()], np.uint64), np.array([dst.data_ptr()], np.uint64),
+                     np.array([n], np.int64))
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x syn_copy_f64 of {n * 8} bytes")
 
 
 def cpu_baseline(np, xs, os_, chi):

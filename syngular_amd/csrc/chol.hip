@@ -24,14 +24,6 @@ namespace syn {
 
 constexpr int kNb = 64;
 
-__global__ __launch_bounds__(256) void fill_kernel(double *x, int64_t rows, int64_t cols, int64_t ld, double v) {
-    const int64_t nel = rows * cols;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nel; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e / cols, j = e - i * cols;
-        x[i * ld + j] = v;
-    }
-}
-
 __global__ __launch_bounds__(256) void maxdiag_kernel(const double *g, int64_t n, int64_t ld, double *out,
                                                       int *dropped) {
     __shared__ double red[4];
@@ -206,14 +198,13 @@ __global__ __launch_bounds__(256) void diagmax_many_kernel(const syn_diagmax_ite
     }
 }
 
-static int fill(double *x, int64_t rows, int64_t cols, int64_t ld, double v, hipStream_t s) {
-    const int64_t nel = rows * cols;
-    if (nel <= 0) return SYN_OK;
-    int64_t blocks = (nel + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, rows, cols, ld, v);
-    SYN_LAUNCH_CHECK("fill");
-    return SYN_OK;
+// Zero an m x n block (leading dimension ld) on the stream.  These routines
+// are not meant for graph capture, so a memset is fine here.
+static int zero(double *x, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return SYN_OK;
+    return hip_check(hipMemset2DAsync(x, (size_t)ld * sizeof(double), 0, (size_t)cols * sizeof(double),
+                                      (size_t)rows, s),
+                     "zero fill");
 }
 
 // Blocked Cholesky of G (destroyed) into L; Z holds the inverted diagonal
@@ -221,7 +212,7 @@ static int fill(double *x, int64_t rows, int64_t cols, int64_t ld, double v, hip
 // dropped pivots.
 static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, double *Z, double *scal, int *dropped,
                  double tolrel, hipStream_t s) {
-    int rc = fill(L, n, n, ldl, 0.0, s);
+    int rc = zero(L, n, n, ldl, s);
     if (rc) return rc;
     hipLaunchKernelGGL(maxdiag_kernel, dim3(1), dim3(256), 0, s, G, n, ldg, scal, dropped);
     SYN_LAUNCH_CHECK("maxdiag");
@@ -323,8 +314,8 @@ extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, doub
     // G = A^T A, lower triangle only (the Cholesky reads nothing else)
     int rc = gemm_f64_launch(make_gemm(A, lda, 1, A, lda, 0, G, n, n, n, m, 1.0, 0.0, 1), s);
     if (rc) return rc;
-    if ((rc = fill(L, n, n, ldl, 0.0, s))) return rc;
-    if ((rc = fill(X, n, n, n, 0.0, s))) return rc;
+    if ((rc = zero(L, n, n, ldl, s))) return rc;
+    if ((rc = zero(X, n, n, n, s))) return rc;
     hipLaunchKernelGGL(maxdiag_kernel, dim3(1), dim3(256), 0, s, G, n, n, scal, dropped);
     SYN_LAUNCH_CHECK("maxdiag");
     if ((rc = chol_inv(G, n, L, ldl, X, n, n, T, scal, dropped, tolrel, s))) return rc;
