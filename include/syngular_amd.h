@@ -262,8 +262,8 @@ int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_h
 int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items);
 /* inner Jacobi sweeps per pair solve (1..8, default 1) */
 void syn_tune_gram_inner_sweeps(int sweeps);
-/* rows per block (16 or 32; 0 = 32 when the largest problem of a call has */
-/* at least 384 rows, else 16) */
+/* rows per block (16 or 32; 0 = automatic, currently 16: 32-row blocks halve */
+/* the steps but double the rounds of each pair solve, and measured no faster) */
 void syn_tune_gram_block_rows(int rows);
 
 /* Batched copy: dst[0..n) = src[0..n) for every item (compaction). */
@@ -296,6 +296,17 @@ int syn_eye_f64(const syn_eye_item *items, int n_items, int64_t total_blocks, vo
 /* (tensor/matrix_product_state.py:560, the left sweep of the rounding). */
 /* ------------------------------------------------------------------ */
 int64_t syn_cholqr_workspace(int64_t n);
+
+/* syn_cholqr_refine_f64: the second pass of CholeskyQR2 on Q1 (m x n,   */
+/* the first pass's Q, kept).  G = Q1^T Q1 = I + E; when max |E_ij| <=    */
+/* thresh, L = I + F and its inverse I - F with F = strict_lower(E) +     */
+/* diag(E)/2 (exact to O(n max|E|^2); no Cholesky), else the recursive    */
+/* Cholesky with the pivot-drop rule (tolrel).  Q = Q1 L^{-T} (m x n, ldq) */
+/* and L (n x n, lower) are written; *full (host, may be null) = 1 when   */
+/* the Cholesky ran.  work: syn_cholqr_workspace(n) doubles.               */
+int syn_cholqr_refine_f64(const double *q1, int64_t m, int64_t n, int64_t ldq1, double *q, int64_t ldq, double *l,
+                          int64_t ldl, double *work, double thresh, double tolrel, int *dropped, int *full,
+                          void *stream);
 
 /* Batched pieces of the same factorisation, for many problems of uneven  */
 /* size (the caller drives the recursion with syn_gemm_f64):              */

@@ -96,6 +96,8 @@ SIGNATURES = {
     "syn_eye_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_copy_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_diagmax_f64": (_I32, [_VP, _I32, _VP]),
+    "syn_cholqr_refine_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _D, _D, _VP,
+                                     ctypes.POINTER(ctypes.c_int), _VP]),
     "syn_jacobi_gram_blocked_f64": (_I32, [_VP, _VP, _I32, _VP, _VP, _D, _I32, ctypes.POINTER(ctypes.c_int), _VP]),
     "syn_jacobi_gram_blocked_workspace": (_I64, [_VP, _I32]),
     "syn_tune_gram_inner_sweeps": (None, [_I32]),

@@ -274,6 +274,46 @@ static int chol_inv(double *G, int64_t ldg, double *L, int64_t ldl, double *X, i
         make_gemm(X + n1 * ldx + n1, ldx, 0, T, n1, 0, X + n1 * ldx, ldx, n2, n1, n2, -1.0, 0.0, 2), s);
 }
 
+// max over the lower triangle of |G_ij - delta_ij|, grid-wide: every block
+// reduces its share and combines with an atomic max on the bit pattern (for
+// non-negative doubles the IEEE order is the unsigned integer order).  *out
+// must be zero on entry.
+__global__ __launch_bounds__(256) void offdiag_identity_kernel(const double *g, int64_t n, int64_t ld,
+                                                               unsigned long long *out) {
+    __shared__ double red[4];
+    double m = 0.0;
+    const int64_t nel = n * n;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nel; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e / n, j = e - i * n;
+        if (j <= i) m = fmax(m, fabs(g[i * ld + j] - (i == j ? 1.0 : 0.0)));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        atomicMax(out, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// First-order Cholesky of G = I + E: F = strict_lower(E) + diag(E)/2,
+// L = I + F, X = I - F (lower; zeros above the diagonal).
+__global__ __launch_bounds__(256) void near_identity_chol_kernel(const double *g, int64_t n, int64_t ldg, double *L,
+                                                                 int64_t ldl, double *X, int64_t ldx) {
+    const int64_t nel = n * n;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nel; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e / n, j = e - i * n;
+        double f = 0.0, d = 0.0;
+        if (j < i) f = g[i * ldg + j];
+        if (j == i) {
+            f = 0.5 * (g[i * ldg + i] - 1.0);
+            d = 1.0;
+        }
+        L[i * ldl + j] = d + f;
+        X[i * ldx + j] = d - f;
+    }
+}
+
 }  // namespace syn
 
 extern "C" int64_t syn_cholqr_workspace(int64_t n) {
@@ -347,4 +387,57 @@ extern "C" int syn_diagmax_f64(const syn_diagmax_item *items, int n_items, void 
     hipLaunchKernelGGL(diagmax_many_kernel, dim3(n_items), dim3(256), 0, (hipStream_t)stream, items);
     SYN_LAUNCH_CHECK("syn_diagmax_f64 launch");
     return SYN_OK;
+}
+
+extern "C" int syn_cholqr_refine_f64(const double *Q1, int64_t m, int64_t n, int64_t ldq1, double *Q, int64_t ldq,
+                                     double *L, int64_t ldl, double *work, double thresh, double tolrel,
+                                     int *dropped, int *full, void *stream) {
+    using namespace syn;
+    if (m <= 0 || n <= 0 || ldq1 < n || ldq < n || ldl < n || !Q1 || !Q || !L || !work || !dropped ||
+        !(thresh >= 0.0) || !(tolrel >= 0.0)) {
+        set_error("syn_cholqr_refine_f64: bad arguments (m=%lld n=%lld)", (long long)m, (long long)n);
+        return SYN_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nblk = (n + kNb - 1) / kNb;
+    const int64_t half = ((nblk + 1) / 2) * kNb;
+    double *G = work;
+    double *X = G + n * n;
+    double *T = X + n * n;
+    double *scal = T + half * half + nblk * kNb * kNb;
+    // G = Q1^T Q1 (lower triangle) and max |G - I| on it
+    int rc = gemm_f64_launch(make_gemm(Q1, ldq1, 1, Q1, ldq1, 0, G, n, n, n, m, 1.0, 0.0, 1), s);
+    if (rc) return rc;
+    if ((rc = hip_check(hipMemsetAsync(scal + 1, 0, sizeof(double), s), "refine emax reset"))) return rc;
+    {
+        int64_t blocks = (n * n + 4 * 256 - 1) / (4 * 256);
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL(offdiag_identity_kernel, dim3((unsigned)blocks), dim3(256), 0, s, G, n, n,
+                           (unsigned long long *)(scal + 1));
+        SYN_LAUNCH_CHECK("cholqr refine check");
+    }
+    double emax = 0.0;
+    if ((rc = hip_check(hipMemcpyAsync(&emax, scal + 1, sizeof(double), hipMemcpyDeviceToHost, s), "refine emax")))
+        return rc;
+    if ((rc = hip_check(hipStreamSynchronize(s), "refine sync"))) return rc;
+    if (emax <= thresh) {
+        // L = I + F, X = I - F: exact to O(n emax^2)
+        int64_t blocks = (n * n + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(near_identity_chol_kernel, dim3((unsigned)blocks), dim3(256), 0, s, G, n, n, L, ldl, X,
+                           n);
+        SYN_LAUNCH_CHECK("cholqr refine factor");
+        hipLaunchKernelGGL(maxdiag_kernel, dim3(1), dim3(256), 0, s, G, n, n, scal, dropped);   // zeroes *dropped
+        SYN_LAUNCH_CHECK("cholqr refine dropped");
+        if (full) *full = 0;
+    } else {
+        if ((rc = zero(L, n, n, ldl, s))) return rc;
+        if ((rc = zero(X, n, n, n, s))) return rc;
+        hipLaunchKernelGGL(maxdiag_kernel, dim3(1), dim3(256), 0, s, G, n, n, scal, dropped);
+        SYN_LAUNCH_CHECK("maxdiag");
+        if ((rc = chol_inv(G, n, L, ldl, X, n, n, T, scal, dropped, tolrel, s))) return rc;
+        if (full) *full = 1;
+    }
+    // Q = Q1 X^T  (op(B) = X^T upper)
+    return gemm_f64_launch(make_gemm(Q1, ldq1, 0, X, n, 1, Q, ldq, m, n, n, 1.0, 0.0, 8), s);
 }

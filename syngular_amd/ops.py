@@ -423,11 +423,36 @@ def cholqr(A, tolrel=None):
     return Q, L, dropped
 
 
+# Second CholeskyQR pass: when the first pass leaves max |Q1^T Q1 - I| below
+# this, the Cholesky of I + E is taken to first order (error O(n 1e-18)).
+CHOLQR_REFINE_THRESH = 1e-9
+
+
+def cholqr_refine(Q1, tolrel=None, thresh=CHOLQR_REFINE_THRESH):
+    """Second CholeskyQR pass on Q1 (not modified).  Returns (Q, L, full)
+    with Q1 = Q L^T; full tells whether the Cholesky factorisation ran."""
+    m, n = Q1.shape
+    if tolrel is None:
+        tolrel = cholqr_tol(m, n)
+    pa, lda, tr = as_matrix(Q1)
+    if tr:
+        raise ValueError("cholqr_refine needs a row-major matrix")
+    Q = torch.empty((m, n), dtype=F64, device=Q1.device)
+    L = torch.empty((n, n), dtype=F64, device=Q1.device)
+    work = torch.empty(int(_lib.load().syn_cholqr_workspace(n)), dtype=F64, device=Q1.device)
+    dropped = torch.zeros(1, dtype=torch.int32, device=Q1.device)
+    full = ctypes.c_int(0)
+    call("syn_cholqr_refine_f64", pa, m, n, lda, _ptr(Q), n, _ptr(L), n, _ptr(work), float(thresh), float(tolrel),
+         _ptr(dropped), ctypes.byref(full), _stream())
+    return Q, L, bool(full.value)
+
+
 def cholqr2(A, tolrel=None):
-    """Reduced QR of a tall row-major matrix by two CholeskyQR passes.
-    Returns (Q (m x n), R (n x n upper)) with A = Q R."""
+    """Reduced QR of a tall row-major matrix by two CholeskyQR passes (the
+    second through cholqr_refine).  Returns (Q (m x n), R (n x n upper))
+    with A = Q R."""
     Q1, L1, _ = cholqr(A, tolrel)
-    Q, L2, _ = cholqr(Q1, tolrel)
+    Q, L2, _ = cholqr_refine(Q1, tolrel)
     R = gemm(L2.t(), L1.t(), tri=4 | 8)  # R = R2 R1 = L2^T L1^T, both upper
     return Q, R
 

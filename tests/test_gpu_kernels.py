@@ -320,3 +320,27 @@ def test_jacobi_gram_blocked(g, rank, block):
     assert np.all(s[rank:] <= 1e-6 * ref[0])
     np.testing.assert_allclose(R @ R.T, np.eye(g), atol=1e-12)
     np.testing.assert_allclose(R.T @ np.diag(s ** 2) @ R, G0, atol=1e-11 * ref[0] ** 2)
+
+
+@pytest.mark.parametrize("cond,expect_full", [(10.0, False), (1e6, True)])
+def test_cholqr_refine_paths(cond, expect_full):
+    """Second CholeskyQR pass: the first-order factor of I + E when the first
+    pass is already near-orthonormal, the full Cholesky otherwise; both give
+    Q orthonormal and A = Q R."""
+    from syngular_amd import ops
+
+    case(f"cholqr refine cond={cond}")
+    rng = np.random.default_rng(31)
+    m, n = 900, 300
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (U * np.logspace(0, -np.log10(cond), n)) @ V.T
+    Q1, L1, _ = ops.cholqr(T(A))
+    Q, L2, full = ops.cholqr_refine(Q1)
+    assert full == expect_full
+    Qh = Q.cpu().numpy()
+    R = (L2.t().cpu().numpy() @ L1.t().cpu().numpy())
+    np.testing.assert_allclose(Qh.T @ Qh, np.eye(n), atol=1e-13)
+    np.testing.assert_allclose(Qh @ R, A, atol=1e-13)
+    Q2, R2 = ops.cholqr2(T(A))
+    np.testing.assert_allclose(Q2.cpu().numpy() @ R2.cpu().numpy(), A, atol=1e-13)
