@@ -389,7 +389,8 @@ __global__ __launch_bounds__(256) void tj_diag_kernel(const syn_gbj_item *__rest
     for (int64_t i = threadIdx.x; i < P.nv; i += 256) P.s[i] = sqrt(fmax(P.g[i * P.ldg + i], 0.0));
 }
 
-// rows of rt and entries of s in the order perm (per problem), through tmp
+// rows of rt and entries of s in the order perm (problem blockIdx.x, slice
+// blockIdx.y), through tmp; pass 0 gathers into tmp, pass 1 copies back
 __global__ __launch_bounds__(256) void tj_permute_kernel(const syn_gbj_item *__restrict__ items,
                                                          const int64_t *__restrict__ perm,
                                                          const int64_t *__restrict__ poff, double *tmp,
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void tj_permute_kernel(const syn_gbj_item *__r
     double *tr = tmp + toff[blockIdx.x];
     double *ts = tr + P.nv * P.nv;
     const int64_t nr = P.nv * P.nv;
-    for (int64_t e = threadIdx.x; e < nr + P.nv; e += 256) {
+    for (int64_t e = blockIdx.y * 256 + threadIdx.x; e < nr + P.nv; e += (int64_t)gridDim.y * 256) {
         if (e < nr) {
             const int64_t i = e / P.nv, j = e - i * P.nv;
             if (pass == 0)
@@ -545,8 +546,9 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     if (int rc = upload(hp.data(), dperm, tot_nv * sizeof(int64_t), "gram perm")) return rc;
     if (int rc = upload(poff.data(), dpoff, n_items * sizeof(int64_t), "gram poff")) return rc;
     if (int rc = upload(toff.data(), dtoff, n_items * sizeof(int64_t), "gram toff")) return rc;
-    hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 0);
-    hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 1);
+    // grid.y spreads each problem's rows over 64 work-groups
+    hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items, 64), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 0);
+    hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items, 64), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 1);
     SYN_LAUNCH_CHECK("gram permute");
     if (prof_on()) {
         unsigned long long hb = 0;
