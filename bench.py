@@ -15,13 +15,18 @@ Beside the JSON contract fields the line carries:
                 (its last three sites), timed on this host;
   parity        errors against the oracle on a reduced chain and the
                 Pythagoras identity at full size;
-  forms         the many-chain workloads run by the grouped engine:
+  forms         chain_mixed: the same chain in the mixed form (products on
+                       the FP32 matrix cores; Gram matrices, Cholesky, Jacobi
+                       and truncation in FP64) with its error vs the oracle;
+                the many-chain workloads run by the grouped engine:
                 batch: 4096 independent chains (N ~ U[8,48], chi ~ U{8..128})
                        with one layer of Haar 2-site gates each, rounded back
-                       to chi, all in lock-step grouped launches;
+                       to chi, all in lock-step grouped launches (FP64 and
+                       mixed);
                 serve: a Poisson stream of brickwork-circuit jobs (N ~
                        U[16,128], layers ~ U[4,32], chi_max ~ U{64..512},
-                       cutoff 1e-10) under the continuous-batching server;
+                       cutoff 1e-10) under the continuous-batching server
+                       (FP64 by default, --serve-precision mixed);
                 each beside the per-chain FP64 form (one chain at a time
                 through the same kernels) on the same jobs, with p50/p99
                 latency and the error against the oracle.
@@ -56,6 +61,10 @@ def parse():
     ap.add_argument("--batch-jobs", type=int, default=4096)
     ap.add_argument("--serve-jobs", type=int, default=12)
     ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="fp64",
+                    help="arithmetic of the served form (the per-chain comparison is always plain FP64); "
+                         "the mixed form gains no speed on this Jacobi-bound workload and its FP32 "
+                         "accumulation grows the overlap error over long circuits")
     ap.add_argument("--plain-seconds", type=float, default=20.0,
                     help="time budget of the per-chain comparison of the batch form")
     ap.add_argument("--serve-plain-seconds", type=float, default=60.0,
@@ -137,6 +146,9 @@ def main():
     del Z, rep
 
     forms = {}
+    if not args.chain_only:
+        forms["chain_mixed"] = chain_mixed_form(np, torch, args, O, X, sites_per_s)
+        log(f"chain mixed form: {forms['chain_mixed']}")
     if not args.no_batch:
         forms["batch"] = batch_form(np, torch, args)
         log(f"batch form: {forms['batch']}")
@@ -261,6 +273,42 @@ def parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep):
     return out
 
 
+def chain_mixed_form(np, torch, args, O, X, plain_sites_per_s):
+    """The headline chain in the mixed form (products on the FP32 matrix
+    cores; Gram matrices, Cholesky, Jacobi and truncation in FP64), with its
+    error against the FP64 oracle on the reduced chain."""
+    from oracle import reference as oref
+    from oracle import rounding as ornd
+    from syngular_amd import linalg, ops, rounding
+    from syngular_amd import workloads as W
+
+    n = len(X)
+    rounding.apply_and_round(O, X, chi_max=args.chi, precision="mixed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rounding.apply_and_round(O, X, chi_max=args.chi, precision="mixed")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    xs, os_ = W.chain_inputs(12, 2, 64, 8, seed=11)
+    dev = ops.device()
+    Xs = [torch.from_numpy(x).to(dev) for x in xs]
+    Os = [torch.from_numpy(o).to(dev) for o in os_]
+    r = rounding.RoundingReport()
+    cores = rounding.apply_and_round(Os, Xs, chi_max=64, report=r, precision="mixed")
+    ref, sv, kept, disc = ornd.apply_and_round(os_, xs, chi_max=64)
+    n2, n2r = float(linalg.overlap(cores, cores).item()), ornd.norm2(ref)
+    ov, ovr = float(linalg.overlap(cores, Xs).item()), oref.overlap(ref, xs)
+    err_s = max(float(np.max(np.abs(r.svals[k - 1].cpu().numpy()[: kept[k]] - sv[k][: kept[k]])) / sv[k][0])
+                for k in range(1, 12))
+    value = n * args.steps / dt
+    return {"sites_per_s": round(value, 3), "ms_per_step": round(1e3 * dt / args.steps, 2),
+            "speedup_vs_plain": round(value / plain_sites_per_s, 3),
+            "error_vs_fp64_oracle": {"reduced_chain": "N=12 d=2 chi=64 D_w=8", "norm2_rel_err": abs(n2 - n2r) / n2r,
+                                     "overlap_rel_err": abs(ov - ovr) / float(np.sqrt(n2r * ornd.norm2(xs))),
+                                     "svals_max_rel_err": err_s, "kept_match": r.kept == kept[1:]}}
+
+
 def batch_form(np, torch, args):
     """4096 chains, one gate layer each, rounded back to their chi: the
     grouped engine (all chains in lock-step launches) against the per-chain
@@ -280,13 +328,16 @@ def batch_form(np, torch, args):
     torch.cuda.synchronize()
     B.apply_and_round(X, Op, chi)          # warm-up
     torch.cuda.synchronize()
-    times = []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        Z = B.apply_and_round(X, Op, chi)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-    batched = nsites / min(times)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            Zp = B.apply_and_round(X, Op, chi, precision=prec)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        res[prec] = (nsites / min(times), min(times), Zp)
+    batched, Z = res["fp64"][0], res["fp64"][2]
     # per-chain form on a strided sample of the same jobs
     dev = ops.device()
     sample = jobs[:: max(1, len(jobs) // 256)]
@@ -308,15 +359,17 @@ def batch_form(np, torch, args):
     # oracle error on a few jobs (every 512th)
     from oracle import reference as oref
     from oracle import rounding as ornd
-    errs = []
+    errs, errm = [], []
     check = list(range(0, len(jobs), 512))[:8]
     outs = Z.to_numpy(check)
-    for q, c in zip(check, outs):
+    outm = res["mixed"][2].to_numpy(check)
+    for q, c, cm in zip(check, outs, outm):
         j = jobs[q]
         ref = ornd.apply_and_round(W.gate_mpo_cores(j["n"], 0, j["gates"]), j["cores"], chi_max=j["chi"])[0]
         n2r, ovr = ornd.norm2(ref), oref.overlap(ref, j["cores"])
         scale = np.sqrt(n2r * ornd.norm2(j["cores"]))
         errs.append((abs(ornd.norm2(c) - n2r) / n2r, abs(oref.overlap(c, j["cores"]) - ovr) / scale))
+        errm.append((abs(ornd.norm2(cm) - n2r) / n2r, abs(oref.overlap(cm, j["cores"]) - ovr) / scale))
     return {"workload": f"{len(jobs)} chains, N~U[8,48], d=2, chi~U{{8,16,32,64,128}}, one layer of Haar "
                         f"O(4) 2-site gates (MPO bond 4) on pairs (0,1),(2,3),..., rounded to chi; plain FP64",
             "sites": nsites, "sites_per_s": round(batched, 1), "seconds": round(min(times), 3),
@@ -324,7 +377,11 @@ def batch_form(np, torch, args):
             "plain_sample": f"{done_sites} sites of {used} strided jobs, one chain at a time",
             "speedup_vs_plain": round(batched / plain, 2),
             "oracle_jobs": len(errs), "norm2_rel_err_max": max(e[0] for e in errs),
-            "overlap_rel_err_max": max(e[1] for e in errs)}
+            "overlap_rel_err_max": max(e[1] for e in errs),
+            "mixed": {"sites_per_s": round(res["mixed"][0], 1), "seconds": round(res["mixed"][1], 3),
+                      "speedup_vs_plain": round(res["mixed"][0] / plain, 2),
+                      "norm2_rel_err_max": max(e[0] for e in errm),
+                      "overlap_rel_err_max": max(e[1] for e in errm)}}
 
 
 def serve_form(np, torch, args):
@@ -335,8 +392,9 @@ def serve_form(np, torch, args):
     from syngular_amd import workloads as W
 
     jobs = W.serving_jobs(args.serve_jobs, args.serve_rate, seed=4)
-    srv = serving.Server(serving.DeviceExecutor.lanes(), hbm_budget=200 * 2 ** 30, keep_states=False,
-                         lanes=serving.chi_lanes)
+    prec = args.serve_precision
+    srv = serving.Server(serving.DeviceExecutor.lanes(precision=prec), hbm_budget=200 * 2 ** 30,
+                         keep_states=False, lanes=serving.chi_lanes)
     # warm-up on a tiny stream (kernel loads, allocator)
     serving.Server(serving.DeviceExecutor(), hbm_budget=2 ** 34).run(
         W.serving_jobs(2, 1e6, seed=99, n_range=(16, 20), layer_range=(2, 2), chis=(64,)))
@@ -368,7 +426,7 @@ def serve_form(np, torch, args):
     # oracle on the two cheapest jobs (n * layers), full circuits
     from oracle import rounding as ornd
     cheap = sorted(jobs, key=lambda j: j["n"] * j["layers"] * j["chi_max"])[:2]
-    srv2 = serving.Server(serving.DeviceExecutor(), hbm_budget=2 ** 36, keep_states=True)
+    srv2 = serving.Server(serving.DeviceExecutor(precision=prec), hbm_budget=2 ** 36, keep_states=True)
     got = {r.id: r for r in srv2.run([dict(j, arrival=0.0) for j in cheap]).results}
     errs = []
     for j in cheap:
@@ -382,7 +440,8 @@ def serve_form(np, torch, args):
         errs.append((abs(ornd.norm2(g) - n2r) / n2r, abs(oref.overlap(g, cores) - n2r) / n2r))
     return {"workload": f"Poisson stream of {len(jobs)} jobs at {args.serve_rate}/s: N~U[16,128], layers~U[4,32] "
                         f"brickwork Haar O(4) gates (MPO bond 4), chi_max~U{{64,128,256,512}}, cutoff 1e-10, "
-                        f"from |0...0>; plain FP64",
+                        f"from |0...0>; {prec} (served, grouped launches)",
+            "precision": prec,
             "site_layers": rep.site_layers, "makespan_s": round(rep.makespan, 2),
             "sites_per_s": round(rep.sites_per_s, 1), "p50_latency_s": round(lat["p50"], 2),
             "p99_latency_s": round(lat["p99"], 2), "layer_steps": rep.steps, "peak_resident": rep.peak_resident,

@@ -79,6 +79,9 @@ typedef struct {
 
 /* f64 x f64 -> f64 on v_mfma_f64_16x16x4 */
 int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile, void *stream);
+/* The mixed form of the same call: FP64 operands and result, products on   */
+/* the FP32 matrix cores (operands rounded to FP32, FP32 accumulation).     */
+int syn_gemm_mixed_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* MPO x MPS site contraction, materialised                              */
@@ -304,9 +307,11 @@ int64_t syn_cholqr_workspace(int64_t n);
 /* Cholesky with the pivot-drop rule (tolrel).  Q = Q1 L^{-T} (m x n, ldq) */
 /* and L (n x n, lower) are written; *full (host, may be null) = 1 when   */
 /* the Cholesky ran.  work: syn_cholqr_workspace(n) doubles.               */
+/* mixed (both calls): the product Q = A L^{-T} on the FP32 matrix cores   */
+/* (the Gram matrix and the Cholesky stay FP64).                          */
 int syn_cholqr_refine_f64(const double *q1, int64_t m, int64_t n, int64_t ldq1, double *q, int64_t ldq, double *l,
                           int64_t ldl, double *work, double thresh, double tolrel, int *dropped, int *full,
-                          void *stream);
+                          int mixed, void *stream);
 
 /* Batched pieces of the same factorisation, for many problems of uneven  */
 /* size (the caller drives the recursion with syn_gemm_f64):              */
@@ -340,7 +345,7 @@ int syn_chol_leaf_f64(const syn_chol_item *items, int n_items, void *stream);
 int syn_potrf_f64(double *g, int64_t n, int64_t ldg, double *l, int64_t ldl, double *work, double tolrel,
                   int *dropped, void *stream);
 int syn_cholqr_f64(double *a, int64_t m, int64_t n, int64_t lda, double *q, int64_t ldq, double *l, int64_t ldl,
-                   double *work, double tolrel, int *dropped, void *stream);
+                   double *work, double tolrel, int *dropped, int mixed, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device-side truncation: for sorted singular values s[0..n-1] keep the */
@@ -380,7 +385,8 @@ int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_
 /* returns the number of tags.  Tags: 0/1 GEMM by value 64/128 tiles,     */
 /* 2/3 grouped GEMM 64/128 tiles, 4 block-Jacobi step, 5 Cholesky        */
 /* diagonal block, 6 Householder QR, 7 LDS Jacobi, 8 MPO x MPS apply,    */
-/* 9/10 Gram block-Jacobi pair solve / tile update.                      */
+/* 9/10 Gram block-Jacobi pair solve / tile update, 11/12 mixed GEMM       */
+/* 64/128 tiles.                                                          */
 /* syn_prof_note_flops sets the count attached to the next grouped GEMM */
 /* or apply launch (their tables live on the device).                    */
 /* ------------------------------------------------------------------ */

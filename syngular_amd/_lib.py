@@ -76,6 +76,7 @@ SIGNATURES = {
     "syn_device_count": (_I32, [ctypes.POINTER(ctypes.c_int)]),
     "syn_struct_sizes": (_I32, [ctypes.POINTER(ctypes.c_int64), _I32]),
     "syn_gemm_f64": (_I32, [_VP, _I32, _I64, _I32, _VP]),
+    "syn_gemm_mixed_f64": (_I32, [_VP, _I32, _I64, _I32, _VP]),
     "syn_mpo_apply_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_mpo_mpo_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_kron_bonds_f64": (_I32, [_VP, _I32, _I64, _VP]),
@@ -90,14 +91,14 @@ SIGNATURES = {
                                       ctypes.POINTER(ctypes.c_int), _VP]),
     "syn_cholqr_workspace": (_I64, [_I64]),
     "syn_potrf_f64": (_I32, [_VP, _I64, _I64, _VP, _I64, _VP, _D, _VP, _VP]),
-    "syn_cholqr_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _D, _VP, _VP]),
+    "syn_cholqr_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _D, _VP, _I32, _VP]),
     "syn_jacobi_blocked_many_f64": (_I32, [_VP, _VP, _I32, _VP, _VP, _D, _I32, ctypes.POINTER(ctypes.c_int), _VP]),
     "syn_jacobi_blocked_many_workspace": (_I64, [_VP, _VP, _I32]),
     "syn_eye_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_copy_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_diagmax_f64": (_I32, [_VP, _I32, _VP]),
     "syn_cholqr_refine_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _D, _D, _VP,
-                                     ctypes.POINTER(ctypes.c_int), _VP]),
+                                     ctypes.POINTER(ctypes.c_int), _I32, _VP]),
     "syn_jacobi_gram_blocked_f64": (_I32, [_VP, _VP, _I32, _VP, _VP, _D, _I32, ctypes.POINTER(ctypes.c_int), _VP]),
     "syn_jacobi_gram_blocked_workspace": (_I64, [_VP, _I32]),
     "syn_tune_gram_inner_sweeps": (None, [_I32]),
@@ -124,6 +125,8 @@ PROF_TAGS = {
     8: "apply_kernel",
     9: "gram_solve_kernel",
     10: "gram_update_kernel",
+    11: "gemm_mixed<64,64>",
+    12: "gemm_mixed<128,128>",
 }
 
 

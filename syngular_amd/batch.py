@@ -92,9 +92,10 @@ def _starts(nblocks):
 # --------------------------------------------------------------------------
 # grouped launches from numpy tables
 # --------------------------------------------------------------------------
-def gemm_table(A, B, C, m, n, k, lda, ldb, ldc, ta=0, tb=0, alpha=1.0, beta=0.0, tri=0):
+def gemm_table(A, B, C, m, n, k, lda, ldb, ldc, ta=0, tb=0, alpha=1.0, beta=0.0, tri=0, mixed=False):
     """C_i = alpha op(A_i) op(B_i) + beta C_i for every i, one launch.  All
-    arguments are arrays (or scalars broadcast) over the problems."""
+    arguments are arrays (or scalars broadcast) over the problems.  mixed:
+    the products on the FP32 matrix cores (FP64 operands and result)."""
     m = np.asarray(m, dtype=np.int64)
     cnt = len(m)
     if cnt == 0:
@@ -106,7 +107,7 @@ def gemm_table(A, B, C, m, n, k, lda, ldb, ldc, ta=0, tb=0, alpha=1.0, beta=0.0,
         sel = np.nonzero(live)[0]
         return gemm_table(*(bc(v, np.uint64)[sel] for v in (A, B, C)), m[sel], n[sel], k[sel],
                           *(bc(v, np.int64)[sel] for v in (lda, ldb, ldc, ta, tb)),
-                          bc(alpha, np.float64)[sel], bc(beta, np.float64)[sel], bc(tri, np.int64)[sel])
+                          bc(alpha, np.float64)[sel], bc(beta, np.float64)[sel], bc(tri, np.int64)[sel], mixed)
     area = m * n
     tile = 0
     if int(area.sum()) >= 256 * 128 * 128:
@@ -128,7 +129,7 @@ def gemm_table(A, B, C, m, n, k, lda, ldb, ldc, ta=0, tb=0, alpha=1.0, beta=0.0,
     dev = ops._upload(it)
     if ops._profiling[0]:
         call("syn_prof_note_flops", float((2.0 * m * n * k).sum()))
-    call("syn_gemm_f64", dev.data_ptr(), cnt, total, tile, ops._stream())
+    call("syn_gemm_mixed_f64" if mixed else "syn_gemm_f64", dev.data_ptr(), cnt, total, tile, ops._stream())
 
 
 def _launch_blocks(fn, it, nel):
@@ -273,10 +274,11 @@ def _chol_inv_many(G, L, X, ld, n, T, maxdiag, tolrel, dropped):
     gemm_table(X + diag, T, X + down, n2, n1, n2, ld, n1, ld, alpha=-1.0, tri=2)
 
 
-def cholqr_table(A, m, n, lda, hold):
+def cholqr_table(A, m, n, lda, hold, mixed=False):
     """One CholeskyQR pass on many tall row-major m x n matrices (A kept):
     G = A^T A, G = L L^T with X = inv(L~), Q = A X^T.  Returns the addresses
-    of Q (m x n, ld n) and L (n x n lower, ld n); buffers go to `hold`."""
+    of Q (m x n, ld n) and L (n x n lower, ld n); buffers go to `hold`.
+    mixed: Q on the FP32 matrix cores (G and the Cholesky in FP64)."""
     cnt = len(A)
     gt, G = arena(n * n)
     lt, Lp = arena(n * n)
@@ -293,16 +295,16 @@ def cholqr_table(A, m, n, lda, hold):
     diagmax_table(G, n, n, sp, dp)
     tol = np.array([ops.cholqr_tol(int(a), int(b)) for a, b in zip(m, n)])
     _chol_inv_many(G, Lp, Xp, n.copy(), n.copy(), Tp, sp, tol, dp)
-    gemm_table(A, Xp, Q, m, n, n, lda, n, n, tb=1, tri=8)
+    gemm_table(A, Xp, Q, m, n, n, lda, n, n, tb=1, tri=8, mixed=mixed)
     hold += [gt, lt, xt, tt, st, dt_, qt]
     return Q, Lp
 
 
-def cholqr2_table(A, m, n, lda, hold):
+def cholqr2_table(A, m, n, lda, hold, mixed=False):
     """Reduced QR A = Q R of many tall matrices by two CholeskyQR passes;
     returns (Q (m x n, ld n), R (n x n upper, ld n)) addresses."""
-    Q1, L1 = cholqr_table(A, m, n, lda, hold)
-    Q, L2 = cholqr_table(Q1, m, n, n, hold)
+    Q1, L1 = cholqr_table(A, m, n, lda, hold, mixed)
+    Q, L2 = cholqr_table(Q1, m, n, n, hold, mixed)
     rt_, R = arena(n * n)
     # R = R2 R1 = L2^T L1^T, both upper
     gemm_table(L2, L1, R, n, n, n, n, n, n, ta=1, tb=1, tri=4 | 8)
@@ -494,9 +496,10 @@ class ChainBatch:
         return ChainBatch(self.n, bond, phys, ptr, [t, ops_batch.hold, self.hold])
 
     # ---- left sweep --------------------------------------------------------
-    def left_orthonormalize(self):
+    def left_orthonormalize(self, mixed=False):
         """Left QR sweep over all chains in lock-step (in place on the tables;
-        the input arenas are not written)."""
+        the input arenas are not written).  mixed: CholeskyQR's Q and the
+        absorb products on the FP32 matrix cores."""
         J, nmax = self.ptr.shape
         ptr, bond = self.ptr.copy(), self.bond.copy()
         hold = list(self.hold)
@@ -533,7 +536,7 @@ class ChainBatch:
             if cq.any():
                 # CholeskyQR2 (grouped GEMMs + batched Cholesky leaves)
                 ti = cq.nonzero()[0]
-                Qc, Rc = cholqr2_table(ptr[a[ti], k], m[ti], r[ti], r[ti], hold)
+                Qc, Rc = cholqr2_table(ptr[a[ti], k], m[ti], r[ti], r[ti], hold, mixed)
                 q[ti] = Qc
                 S[ti] = Rc
             # absorb: next = S @ core_{k+1}  (new_r x d' b'')
@@ -541,7 +544,8 @@ class ChainBatch:
             ncols = dn * bn
             nt, npk = arena(new_r * ncols)
             # S from a QR is upper triangular (tri 4); S = L of a wide unfolding is not
-            gemm_table(S, ptr[a, k + 1], npk, new_r, ncols, r, lds, ncols, ncols, tri=np.where(wide, 0, 4))
+            gemm_table(S, ptr[a, k + 1], npk, new_r, ncols, r, lds, ncols, ncols, tri=np.where(wide, 0, 4),
+                       mixed=mixed)
             hold += [qt, nt]
             ptr[a, k] = q
             ptr[a, k + 1] = npk
@@ -550,9 +554,11 @@ class ChainBatch:
         return out
 
     # ---- right sweep -------------------------------------------------------
-    def right_svd_sweep(self, chi_max, cutoff=0.0, records=None):
+    def right_svd_sweep(self, chi_max, cutoff=0.0, records=None, mixed=False):
         """Right-to-left SVD truncation of left-orthonormal chains.
-        chi_max, cutoff: scalars or per-chain arrays (chi_max 0 = no cap)."""
+        chi_max, cutoff: scalars or per-chain arrays (chi_max 0 = no cap).
+        mixed: the products with the singular vectors on the FP32 matrix
+        cores (Gram matrices, Jacobi and truncation stay FP64)."""
         J, nmax = self.ptr.shape
         chi_max = np.broadcast_to(np.asarray(0 if chi_max is None else chi_max, dtype=np.int64), (J,))
         cutoff = np.broadcast_to(np.asarray(cutoff, dtype=np.float64), (J,))
@@ -594,14 +600,14 @@ class ChainBatch:
                 ti = tall.nonzero()[0]
                 mt, mpp = arena(l[ti] * cc[ti])
                 # M = C V = C (V^T)^T  (l x cc)
-                gemm_table(C[ti], rtp[ti], mpp, l[ti], cc[ti], ncol[ti], ncol[ti], g[ti], cc[ti], tb=1)
+                gemm_table(C[ti], rtp[ti], mpp, l[ti], cc[ti], ncol[ti], ncol[ti], g[ti], cc[ti], tb=1, mixed=mixed)
                 Mp[ti] = mpp
                 hold_step.append(mt)
             if wide.any():
                 wi = wide.nonzero()[0]
                 nt, npp = arena(cc[wi] * ncol[wi])
                 # newc = U^T C  (cc x ncol), then / s ; U^T *= s
-                gemm_table(rtp[wi], C[wi], npp, cc[wi], ncol[wi], l[wi], g[wi], ncol[wi], ncol[wi])
+                gemm_table(rtp[wi], C[wi], npp, cc[wi], ncol[wi], l[wi], g[wi], ncol[wi], ncol[wi], mixed=mixed)
                 rowscale_table(np.concatenate([npp, rtp[wi]]), np.concatenate([cc[wi], cc[wi]]),
                                np.concatenate([ncol[wi], l[wi]]), np.concatenate([ncol[wi], g[wi]]),
                                np.concatenate([sp[wi], sp[wi]]),
@@ -613,7 +619,7 @@ class ChainBatch:
             pt, pp = arena(rows * cc)
             # tall: M row-major (l x cc, ld cc); wide: M = (U S)^T, op(B) = rt rows^T with ld g
             gemm_table(ptr[a, k - 1], np.where(tall, Mp, rtp), pp, rows, cc, l, l,
-                       np.where(tall, cc, g), cc, tb=wide.astype(np.int64))
+                       np.where(tall, cc, g), cc, tb=wide.astype(np.int64), mixed=mixed)
             hold_step.append(pt)
             hold += hold_step
             ptr[a, k] = newc
@@ -623,8 +629,14 @@ class ChainBatch:
                 records.append(BondRecord(a, k, sp, g, cc, disc, (stt,)))
         return ChainBatch(self.n, bond, self.phys, ptr, hold)
 
-    def tt_round(self, chi_max, cutoff=0.0, records=None, compact=True):
-        out = self.left_orthonormalize().right_svd_sweep(chi_max, cutoff, records)
+    def tt_round(self, chi_max, cutoff=0.0, records=None, compact=True, precision="fp64"):
+        """TT-rounding of every chain; precision "fp64" or "mixed" (products
+        on the FP32 matrix cores, Gram matrices, Cholesky, Jacobi and
+        truncation in FP64)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        mixed = precision == "mixed"
+        out = self.left_orthonormalize(mixed).right_svd_sweep(chi_max, cutoff, records, mixed)
         if compact:
             out.compact()
         return out
@@ -797,6 +809,6 @@ def overlap(A: ChainBatch, B: ChainBatch):
     return out
 
 
-def apply_and_round(mps: ChainBatch, mpo: OperatorBatch, chi_max, cutoff=0.0, records=None):
+def apply_and_round(mps: ChainBatch, mpo: OperatorBatch, chi_max, cutoff=0.0, records=None, precision="fp64"):
     """One layer for every chain: materialised apply, then TT-rounding."""
-    return mps.apply(mpo).tt_round(chi_max, cutoff, records)
+    return mps.apply(mpo).tt_round(chi_max, cutoff, records, precision=precision)

@@ -337,7 +337,7 @@ extern "C" int syn_potrf_f64(double *G, int64_t n, int64_t ldg, double *L, int64
 }
 
 extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, double *Q, int64_t ldq, double *L,
-                              int64_t ldl, double *work, double tolrel, int *dropped, void *stream) {
+                              int64_t ldl, double *work, double tolrel, int *dropped, int mixed, void *stream) {
     using namespace syn;
     if (m <= 0 || n <= 0 || lda < n || ldq < n || ldl < n || !A || !Q || !L || !work || !dropped ||
         !(tolrel >= 0.0)) {
@@ -360,7 +360,7 @@ extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, doub
     SYN_LAUNCH_CHECK("maxdiag");
     if ((rc = chol_inv(G, n, L, ldl, X, n, n, T, scal, dropped, tolrel, s))) return rc;
     // Q = A X^T  (X = L^{-1} with dropped columns zeroed; op(B) = X^T upper)
-    return gemm_f64_launch(make_gemm(A, lda, 0, X, n, 1, Q, ldq, m, n, n, 1.0, 0.0, 8), s);
+    return gemm_f64_launch(make_gemm(A, lda, 0, X, n, 1, Q, ldq, m, n, n, 1.0, 0.0, 8), s, mixed != 0);
 }
 
 extern "C" int syn_chol_leaf_f64(const syn_chol_item *items, int n_items, void *stream) {
@@ -391,7 +391,7 @@ extern "C" int syn_diagmax_f64(const syn_diagmax_item *items, int n_items, void 
 
 extern "C" int syn_cholqr_refine_f64(const double *Q1, int64_t m, int64_t n, int64_t ldq1, double *Q, int64_t ldq,
                                      double *L, int64_t ldl, double *work, double thresh, double tolrel,
-                                     int *dropped, int *full, void *stream) {
+                                     int *dropped, int *full, int mixed, void *stream) {
     using namespace syn;
     if (m <= 0 || n <= 0 || ldq1 < n || ldq < n || ldl < n || !Q1 || !Q || !L || !work || !dropped ||
         !(thresh >= 0.0) || !(tolrel >= 0.0)) {
@@ -439,5 +439,5 @@ extern "C" int syn_cholqr_refine_f64(const double *Q1, int64_t m, int64_t n, int
         if (full) *full = 1;
     }
     // Q = Q1 X^T  (op(B) = X^T upper)
-    return gemm_f64_launch(make_gemm(Q1, ldq1, 0, X, n, 1, Q, ldq, m, n, n, 1.0, 0.0, 8), s);
+    return gemm_f64_launch(make_gemm(Q1, ldq1, 0, X, n, 1, Q, ldq, m, n, n, 1.0, 0.0, 8), s, mixed != 0);
 }

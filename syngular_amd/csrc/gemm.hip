@@ -22,15 +22,42 @@
 namespace syn {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN, int BK, int WM, int WN>
-struct GemmSmem {
-    double As[2][BK][BM + 1];
-    double Bs[2][BK][BN + 1];
+// Compute types: double (v_mfma_f64_16x16x4f64) or float, the mixed form
+// (operands rounded to FP32 when staged into LDS, v_mfma_f32_16x16x4f32 with
+// FP32 accumulation; the result is stored as FP64).  The two MFMAs take the
+// same operand layout; the output rows of lane l, register r are
+// l/16 + 4r (f64) and 4(l/16) + r (f32).
+template <typename CT>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t zero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t zero() { return v4f{0.0f, 0.0f, 0.0f, 0.0f}; }
+    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
-template <int BM, int BN, int BK, int WM, int WN, bool TA, bool TB>
-__device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t t, GemmSmem<BM, BN, BK, WM, WN> &sm) {
+template <int BM, int BN, int BK, typename CT>
+struct GemmSmem {
+    CT As[2][BK][BM + 1];
+    CT Bs[2][BK][BN + 1];
+};
+
+template <int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, typename CT>
+__device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t t, GemmSmem<BM, BN, BK, CT> &sm) {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NT = NW * 64;
     constexpr int TI = WM / 16, TJ = WN / 16;
@@ -88,22 +115,23 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
             int e = tid + NT * s;
             int r, c;
             if constexpr (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
-            As[buf][c][r] = ra[s];
+            As[buf][c][r] = (CT)ra[s];
         }
 #pragma unroll
         for (int s = 0; s < EB; ++s) {
             int e = tid + NT * s;
             int j, c;
             if constexpr (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
-            Bs[buf][c][j] = rb[s];
+            Bs[buf][c][j] = (CT)rb[s];
         }
     };
 
-    v4d acc[TI][TJ];
+    typedef Mfma<CT> M;
+    typename M::acc_t acc[TI][TJ];
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < TJ; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < TJ; ++j) acc[i][j] = M::zero();
 
     const int64_t nk = k > kmin ? (k - kmin + BK - 1) / BK : 0;
     load(kmin);
@@ -115,7 +143,7 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
         if (kt + 1 < nk) load(kmin + (kt + 1) * BK);
 #pragma unroll 1
         for (int kk = 0; kk < BK / 4; ++kk) {
-            double a[TI], b[TJ];
+            CT a[TI], b[TJ];
 #pragma unroll
             for (int i = 0; i < TI; ++i) a[i] = As[cur][kk * 4 + lk][wm0 + i * 16 + li];
 #pragma unroll
@@ -123,8 +151,7 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
 #pragma unroll
             for (int i = 0; i < TI; ++i)
 #pragma unroll
-                for (int j = 0; j < TJ; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < TJ; ++j) acc[i][j] = M::mac(a[i], b[j], acc[i][j]);
         }
         if (kt + 1 < nk) store(cur ^ 1);
         __syncthreads();
@@ -139,10 +166,10 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
         for (int j = 0; j < TJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int64_t gi = row0 + wm0 + i * 16 + lk + 4 * r;
+                int64_t gi = row0 + wm0 + i * 16 + M::row(lk, r);
                 int64_t gj = col0 + wn0 + j * 16 + li;
                 if (gi < m && gj < n) {
-                    double v = alpha * acc[i][j][r];
+                    double v = alpha * (double)acc[i][j][r];
                     if (beta != 0.0) v += beta * C[gi * ldc + gj];
                     C[gi * ldc + gj] = v;
                 }
@@ -152,19 +179,19 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
 // The operand orientation is fixed at compile time in each instance, so the
 // loads carry no per-element branches and the register budget fits two
 // work-groups per compute unit.
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, typename CT>
 __device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t) {
-    __shared__ GemmSmem<BM, BN, BK, WM, WN> sm;
+    __shared__ GemmSmem<BM, BN, BK, CT> sm;
     if (!P.transA) {
         if (!P.transB)
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, false>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, false, CT>(P, t, sm);
         else
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, true>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, true, CT>(P, t, sm);
     } else {
         if (!P.transB)
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, false>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, false, CT>(P, t, sm);
         else
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, true>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, true, CT>(P, t, sm);
     }
 }
 
@@ -186,28 +213,28 @@ __device__ __forceinline__ int64_t tile_of_block(int64_t bid, int64_t total, int
     return by_col ? pos * tiles_n + line : line * tiles_n + pos;
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, typename CT = double>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_grouped(
     const syn_gemm_item *__restrict__ items, int n_items, int64_t total) {
     if (n_items == 1) {
-        gemm_f64_tile<BM, BN, BK, WM, WN>(items[0],
-                                          tile_of_block(blockIdx.x, total, items[0].tiles_n, items[0].tri));
+        gemm_f64_tile<BM, BN, BK, WM, WN, CT>(items[0],
+                                              tile_of_block(blockIdx.x, total, items[0].tiles_n, items[0].tri));
         return;
     }
     const int64_t bid = xcd_remap(blockIdx.x, total);
     const syn_gemm_item &P = items[find_item(items, n_items, bid)];
-    gemm_f64_tile<BM, BN, BK, WM, WN>(P, bid - P.block_start);
+    gemm_f64_tile<BM, BN, BK, WM, WN, CT>(P, bid - P.block_start);
 }
 
-template <int BM, int BN, int BK, int WM, int WN>
+template <int BM, int BN, int BK, int WM, int WN, typename CT = double>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_single(syn_gemm_item P) {
     const int64_t total = (int64_t)gridDim.x;
-    gemm_f64_tile<BM, BN, BK, WM, WN>(P, tile_of_block(blockIdx.x, total, P.tiles_n, P.tri));
+    gemm_f64_tile<BM, BN, BK, WM, WN, CT>(P, tile_of_block(blockIdx.x, total, P.tiles_n, P.tri));
 }
 
 // One GEMM launched with its descriptor passed by value (no device table):
 // used by the library's own multi-step routines (Cholesky, CholeskyQR, ...).
-int gemm_f64_launch(syn_gemm_item P, hipStream_t s) {
+int gemm_f64_launch(syn_gemm_item P, hipStream_t s, bool mixed) {
     if (P.m <= 0 || P.n <= 0) return SYN_OK;
     if (P.k <= 0 && P.beta == 1.0) return SYN_OK;
     const int tile = (P.m * P.n >= (int64_t)256 * 128 * 128) ? 1 : 0;
@@ -221,6 +248,17 @@ int gemm_f64_launch(syn_gemm_item P, hipStream_t s) {
     const int nbits = __builtin_popcount((unsigned)P.tri);
     if (nbits == 1) fl *= 0.5;
     if (nbits >= 2) fl /= 3.0;
+    if (mixed) {
+        ProfScope ps(tile ? kTagGemmMixed128 : kTagGemmMixed64, fl, s);
+        if (tile)
+            hipLaunchKernelGGL((gemm_f64_single<128, 128, 16, 64, 64, float>), dim3((unsigned)tiles), dim3(256), 0,
+                               s, P);
+        else
+            hipLaunchKernelGGL((gemm_f64_single<64, 64, 16, 32, 32, float>), dim3((unsigned)tiles), dim3(256), 0, s,
+                               P);
+        SYN_LAUNCH_CHECK("gemm_f64_launch (mixed)");
+        return SYN_OK;
+    }
     ProfScope ps(tile ? kTagGemmSingle128 : kTagGemmSingle64, fl, s);
     if (tile)
         hipLaunchKernelGGL((gemm_f64_single<128, 128, 16, 64, 64>), dim3((unsigned)tiles), dim3(256), 0, s, P);
@@ -278,5 +316,33 @@ extern "C" int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t tot
         return SYN_EINVAL;
     }
     SYN_LAUNCH_CHECK("syn_gemm_f64 launch");
+    return SYN_OK;
+}
+
+extern "C" int syn_gemm_mixed_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile,
+                                  void *stream) {
+    using namespace syn;
+    if (n_items < 0 || total_blocks < 0) {
+        set_error("syn_gemm_mixed_f64: negative item or block count");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0 || total_blocks == 0) return SYN_OK;
+    if (total_blocks > 0x7fffffffLL) {
+        set_error("syn_gemm_mixed_f64: %lld tiles exceed the grid limit", (long long)total_blocks);
+        return SYN_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    ProfScope ps(tile == 1 ? kTagGemmMixed128 : kTagGemmMixed64, -1.0, s);
+    if (tile == 0)
+        hipLaunchKernelGGL((gemm_f64_grouped<64, 64, 16, 32, 32, float>), dim3((unsigned)total_blocks), dim3(256), 0,
+                           s, items, n_items, total_blocks);
+    else if (tile == 1)
+        hipLaunchKernelGGL((gemm_f64_grouped<128, 128, 16, 64, 64, float>), dim3((unsigned)total_blocks), dim3(256),
+                           0, s, items, n_items, total_blocks);
+    else {
+        set_error("syn_gemm_mixed_f64: unknown tile configuration %d", tile);
+        return SYN_EINVAL;
+    }
+    SYN_LAUNCH_CHECK("syn_gemm_mixed_f64 launch");
     return SYN_OK;
 }

@@ -17,6 +17,8 @@ enum ProfTag {
     kTagApply = 8,
     kTagGramSolve = 9,
     kTagGramUpdate = 10,
+    kTagGemmMixed64 = 11,
+    kTagGemmMixed128 = 12,
 };
 
 bool prof_on();

@@ -69,10 +69,12 @@ def as_matrix(t: torch.Tensor):
 _TILES = {0: 64, 1: 128}
 
 
-def gemm_many(problems, tile=None):
+def gemm_many(problems, tile=None, mixed=False):
     """problems: iterable of (A, B, C, alpha, beta[, tri]) with A (m x k),
     B (k x n), C (m x n) 2-D float64 views; C is overwritten with
-    alpha*A@B + beta*C.  tri is the structural-zero mask of syn_gemm_item."""
+    alpha*A@B + beta*C.  tri is the structural-zero mask of syn_gemm_item.
+    mixed: products on the FP32 matrix cores (operands rounded to FP32, FP32
+    accumulation, FP64 result)."""
     problems = [p if len(p) == 6 else (*p, 0) for p in problems if p[2].numel() > 0]
     if not problems:
         return
@@ -109,7 +111,7 @@ def gemm_many(problems, tile=None):
         fl = sum(2.0 * p[0].shape[0] * p[0].shape[1] * p[1].shape[1] * frac.get(bin(p[5]).count("1"), 1.0 / 3.0)
                  for p in problems)
         call("syn_prof_note_flops", fl)
-    call("syn_gemm_f64", _ptr(dev), len(problems), start, tile, _stream())
+    call("syn_gemm_mixed_f64" if mixed else "syn_gemm_f64", _ptr(dev), len(problems), start, tile, _stream())
     return dev
 
 
@@ -128,11 +130,11 @@ def profile_end():
     return _lib.prof_end()
 
 
-def gemm(A, B, C=None, alpha=1.0, beta=0.0, tile=None, tri=0):
+def gemm(A, B, C=None, alpha=1.0, beta=0.0, tile=None, tri=0, mixed=False):
     if C is None:
         C = torch.empty((A.shape[0], B.shape[1]), dtype=F64, device=A.device)
         beta = 0.0
-    gemm_many([(A, B, C, alpha, beta, tri)], tile=tile)
+    gemm_many([(A, B, C, alpha, beta, tri)], tile=tile, mixed=mixed)
     return C
 
 
@@ -405,9 +407,10 @@ def cholqr_tol(m, n):
     return 8.0 * (m + n) * EPS
 
 
-def cholqr(A, tolrel=None):
+def cholqr(A, tolrel=None, mixed=False):
     """One CholeskyQR pass on a row-major m x n matrix (A is not modified).
-    Returns (Q (m x n), L (n x n lower), dropped (1,) int32) with A = Q L^T."""
+    Returns (Q (m x n), L (n x n lower), dropped (1,) int32) with A = Q L^T.
+    mixed: Q = A L^{-T} on the FP32 matrix cores (Gram and Cholesky in FP64)."""
     m, n = A.shape
     if tolrel is None:
         tolrel = cholqr_tol(m, n)
@@ -419,7 +422,7 @@ def cholqr(A, tolrel=None):
     work = torch.empty(int(_lib.load().syn_cholqr_workspace(n)), dtype=F64, device=A.device)
     dropped = torch.zeros(1, dtype=torch.int32, device=A.device)
     call("syn_cholqr_f64", pa, m, n, lda, _ptr(Q), n, _ptr(L), n, _ptr(work), float(tolrel), _ptr(dropped),
-         _stream())
+         int(bool(mixed)), _stream())
     return Q, L, dropped
 
 
@@ -428,7 +431,7 @@ def cholqr(A, tolrel=None):
 CHOLQR_REFINE_THRESH = 1e-9
 
 
-def cholqr_refine(Q1, tolrel=None, thresh=CHOLQR_REFINE_THRESH):
+def cholqr_refine(Q1, tolrel=None, thresh=CHOLQR_REFINE_THRESH, mixed=False):
     """Second CholeskyQR pass on Q1 (not modified).  Returns (Q, L, full)
     with Q1 = Q L^T; full tells whether the Cholesky factorisation ran."""
     m, n = Q1.shape
@@ -443,16 +446,16 @@ def cholqr_refine(Q1, tolrel=None, thresh=CHOLQR_REFINE_THRESH):
     dropped = torch.zeros(1, dtype=torch.int32, device=Q1.device)
     full = ctypes.c_int(0)
     call("syn_cholqr_refine_f64", pa, m, n, lda, _ptr(Q), n, _ptr(L), n, _ptr(work), float(thresh), float(tolrel),
-         _ptr(dropped), ctypes.byref(full), _stream())
+         _ptr(dropped), ctypes.byref(full), int(bool(mixed)), _stream())
     return Q, L, bool(full.value)
 
 
-def cholqr2(A, tolrel=None):
+def cholqr2(A, tolrel=None, mixed=False):
     """Reduced QR of a tall row-major matrix by two CholeskyQR passes (the
     second through cholqr_refine).  Returns (Q (m x n), R (n x n upper))
-    with A = Q R."""
-    Q1, L1, _ = cholqr(A, tolrel)
-    Q, L2, _ = cholqr_refine(Q1, tolrel)
+    with A = Q R.  mixed: the Q products on the FP32 matrix cores."""
+    Q1, L1, _ = cholqr(A, tolrel, mixed)
+    Q, L2, _ = cholqr_refine(Q1, tolrel, mixed=mixed)
     R = gemm(L2.t(), L1.t(), tri=4 | 8)  # R = R2 R1 = L2^T L1^T, both upper
     return Q, R
 

@@ -48,9 +48,20 @@ CHOLQR_MIN_COLS = 96
 LDS_JACOBI_MAX = 64
 
 
-def left_orthonormalize(cores, own=False):
+PRECISIONS = ("fp64", "mixed")
+
+
+def _mixed(precision):
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}, got {precision!r}")
+    return precision == "mixed"
+
+
+def left_orthonormalize(cores, own=False, mixed=False):
     """Left QR sweep; returns new cores (left-orthonormal except the last).
-    With own=False the input cores are not modified."""
+    With own=False the input cores are not modified.  mixed: the products
+    (CholeskyQR's Q and the absorb) on the FP32 matrix cores, the Gram
+    matrices and the Cholesky in FP64."""
     cores = list(cores)
     for k in range(len(cores) - 1):
         c = cores[k].contiguous()
@@ -61,14 +72,14 @@ def left_orthonormalize(cores, own=False):
             # a wide unfolding: Q = I_m and R = L, no factorisation needed
             Q, S, tri = torch.eye(m, dtype=F64, device=L.device), L, 0
         elif r >= CHOLQR_MIN_COLS:
-            Q, S = ops.cholqr2(L)
+            Q, S = ops.cholqr2(L, mixed=mixed)
             tri = 4                      # S = R is upper triangular
         else:
             Q, S = _qr(L)
             tri = 4
         nxt = cores[k + 1]
         cores[k] = Q.reshape(l, *mid, Q.shape[1])
-        W = ops.gemm(S, nxt.reshape(nxt.shape[0], -1), tri=tri)
+        W = ops.gemm(S, nxt.reshape(nxt.shape[0], -1), tri=tri, mixed=mixed)
         cores[k + 1] = W.reshape(S.shape[0], *nxt.shape[1:])
     return cores
 
@@ -80,8 +91,10 @@ def _qr(L):
     return Q, S
 
 
-def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None):
-    """Right-to-left SVD truncation of left-orthonormal cores."""
+def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None, mixed=False):
+    """Right-to-left SVD truncation of left-orthonormal cores.  mixed: the
+    products with the singular vectors on the FP32 matrix cores; the Gram
+    matrix, the Jacobi rotations and the truncation stay FP64."""
     n = len(cores)
     for k in range(n - 1, 0, -1):
         c = cores[k]
@@ -103,16 +116,16 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None):
         Pl = prev.reshape(-1, l)                       # (l' * mid' x m)
         if tall:
             Vt = rt[:cc]                               # (cc x ncol) = V^T
-            M = ops.gemm(C, Vt.t())                    # C V = U S   (m x cc)
+            M = ops.gemm(C, Vt.t(), mixed=mixed)       # C V = U S   (m x cc)
             newc = Vt
         else:
             Ut = rt[:cc].contiguous()                  # (cc x m) = U^T
-            newc = ops.gemm(Ut, C)                     # U^T C = S V^T (cc x ncol)
+            newc = ops.gemm(Ut, C, mixed=mixed)        # U^T C = S V^T (cc x ncol)
             ops.rowscale_many([(newc, s, 1)])          # -> V^T
             ops.rowscale_many([(Ut, s, 0)])            # -> (U S)^T
             M = Ut.t()
         cores[k] = reshape(newc, (cc, *mid, r))
-        cores[k - 1] = reshape(ops.gemm(Pl, M), (*prev.shape[:-1], cc))
+        cores[k - 1] = reshape(ops.gemm(Pl, M, mixed=mixed), (*prev.shape[:-1], cc))
         if report is not None:
             report.svals.insert(0, s)
             report.kept.insert(0, cc)
@@ -120,13 +133,16 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None):
     return cores
 
 
-def tt_round(cores, chi_max=None, cutoff=0.0, report=None, own=False):
+def tt_round(cores, chi_max=None, cutoff=0.0, report=None, own=False, precision="fp64"):
     """TT-rounding of an MPS given as device cores (l, d, r).  The input
-    cores are left untouched unless own=True."""
-    return right_svd_sweep(left_orthonormalize(cores, own=own), chi_max, cutoff, report)
+    cores are left untouched unless own=True.  precision "fp64" (plain) or
+    "mixed" (products on the FP32 matrix cores; Gram matrices, Cholesky,
+    Jacobi and truncation in FP64)."""
+    mixed = _mixed(precision)
+    return right_svd_sweep(left_orthonormalize(cores, own=own, mixed=mixed), chi_max, cutoff, report, mixed=mixed)
 
 
-def apply_and_round(mpo_sites, mps_sites, chi_max=None, cutoff=0.0, report=None):
+def apply_and_round(mpo_sites, mps_sites, chi_max=None, cutoff=0.0, report=None, precision="fp64"):
     """One apply+round step: materialised MPO x MPS cores, then tt_round."""
     prod = ops.apply_many(list(zip(mpo_sites, mps_sites)))
-    return tt_round(prod, chi_max, cutoff, report, own=True)
+    return tt_round(prod, chi_max, cutoff, report, own=True, precision=precision)

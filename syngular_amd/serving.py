@@ -233,12 +233,13 @@ class DeviceExecutor:
     With a stream, every launch of this executor goes to that HIP stream (one
     per lane, so lanes overlap on the device)."""
 
-    def __init__(self, d=2, stream=None):
+    def __init__(self, d=2, stream=None, precision="fp64"):
         from . import batch, ops
         self.B = batch
         self.ops = ops
         self.d = d
         self.stream = stream
+        self.precision = precision
 
     def _ctx(self):
         import contextlib
@@ -246,10 +247,10 @@ class DeviceExecutor:
         return torch.cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
 
     @classmethod
-    def lanes(cls, d=2):
+    def lanes(cls, d=2, precision="fp64"):
         """Factory for Server(lanes=...): one executor and one stream per lane."""
         import torch
-        return lambda key: cls(d, torch.cuda.Stream())
+        return lambda key: cls(d, torch.cuda.Stream(), precision)
 
     def empty(self):
         return self.B.concat([])
@@ -277,7 +278,7 @@ class DeviceExecutor:
         recs = []
         chi = np.array([j["chi_max"] for j in jobs], dtype=np.int64)
         cut = np.array([j["cutoff"] for j in jobs], dtype=np.float64)
-        new = self.B.apply_and_round(state, mpo, chi, cut, records=recs)
+        new = self.B.apply_and_round(state, mpo, chi, cut, records=recs, precision=self.precision)
         disc = np.zeros(len(jobs))
         if recs:
             import torch

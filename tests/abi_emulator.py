@@ -109,6 +109,24 @@ class Emulator:
             C[...] = res
         return 0
 
+    def syn_gemm_mixed_f64(self, items, n, total, tile, stream):
+        """Products of FP32-rounded operands accumulated in FP32."""
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.GEMM_ITEM)
+        bm = 128 if tile else 64
+        tiles = ((it["m"] + bm - 1) // bm) * ((it["n"] + bm - 1) // bm)
+        self._blocks_ok(it, tiles, total, "gemm mixed")
+        for r in it:
+            m, nn, k = int(r["m"]), int(r["n"]), int(r["k"])
+            A = self.mat(r["A"], m, k, int(r["lda"]), int(r["transA"]), "gemm A").astype(np.float32)
+            Bm = self.mat(r["B"], k, nn, int(r["ldb"]), int(r["transB"]), "gemm B").astype(np.float32)
+            C = self.mat(r["C"], m, nn, int(r["ldc"]), 0, "gemm C")
+            res = float(r["alpha"]) * (A @ Bm).astype(np.float64)
+            if float(r["beta"]) != 0.0:
+                res = res + float(r["beta"]) * C
+            C[...] = res
+        return 0
+
     def syn_mpo_apply_f64(self, items, n, total, stream):
         from syngular_amd import _lib
         it = self.table(items, n, _lib.APPLY_ITEM)

@@ -175,3 +175,21 @@ def test_emulated_cholqr2_table(emu):
 def torch_from(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a))
+
+
+def test_emulated_mixed_precision_round(emu):
+    """The mixed form (FP32 products) stays within FP32 rounding of the FP64
+    oracle; the FP64 Gram/Jacobi/truncation keep the kept ranks."""
+    rng = np.random.default_rng(0)
+    jobs = [(_chain(rng, n, c), _mpo(rng, n, D), c) for n, c, D in SPEC]
+    X = B.ChainBatch.from_numpy([j[0] for j in jobs])
+    O = B.OperatorBatch.from_numpy([j[1] for j in jobs])
+    out = B.apply_and_round(X, O, np.array([j[2] for j in jobs]), 0.0, precision="mixed").to_numpy()
+    assert "syn_gemm_mixed_f64" in emu.calls
+    for q, (xs, os_, c) in enumerate(jobs):
+        ref = ornd.apply_and_round(os_, xs, chi_max=c)[0]
+        assert [o.shape for o in out[q]] == [r.shape for r in ref]
+        a, b = oref.dense_mps(out[q]), oref.dense_mps(ref)
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max()
+    with pytest.raises(ValueError):
+        B.apply_and_round(X, O, 4, precision="bf16")

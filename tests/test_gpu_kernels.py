@@ -344,3 +344,42 @@ def test_cholqr_refine_paths(cond, expect_full):
     np.testing.assert_allclose(Qh @ R, A, atol=1e-13)
     Q2, R2 = ops.cholqr2(T(A))
     np.testing.assert_allclose(Q2.cpu().numpy() @ R2.cpu().numpy(), A, atol=1e-13)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(37, 50, 19), (300, 260, 513)])
+def test_gemm_mixed(ta, tb, m, n, k):
+    """Mixed GEMM: FP64 in/out, FP32 operands and accumulation; checks the
+    FP32 MFMA output layout (an error would be O(1)) and the FP32 accuracy."""
+    from syngular_amd import ops
+
+    case(f"gemm mixed ta={ta} tb={tb} {m}x{n}x{k}")
+    rng = np.random.default_rng(m + n + k + 3 * ta + tb)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    Bm = rng.standard_normal((n, k) if tb else (k, n))
+    Ad, Bd = T(A), T(Bm)
+    C = ops.gemm(Ad.t() if ta else Ad, Bd.t() if tb else Bd, mixed=True).cpu().numpy()
+    opA, opB = (A.T if ta else A), (Bm.T if tb else Bm)
+    ref = opA.astype(np.float32).astype(np.float64) @ opB.astype(np.float32).astype(np.float64)
+    scale = np.abs(opA) @ np.abs(opB)
+    assert np.abs(C - ref).max() / scale.max() < 4e-6
+    assert np.abs(C - opA @ opB).max() / scale.max() < 1e-5
+
+
+def test_mixed_round_vs_oracle():
+    """The mixed form of apply+round on a chain against the FP64 oracle."""
+    from oracle import reference as oref
+    from oracle import rounding as ornd
+    from syngular_amd import linalg, rounding
+    from syngular_amd import workloads as W
+
+    case("mixed apply+round chain")
+    xs, os_ = W.chain_inputs(12, 2, 64, 8, seed=11)
+    X = [T(x) for x in xs]
+    O = [T(o) for o in os_]
+    cores = rounding.apply_and_round(O, X, chi_max=64, precision="mixed")
+    ref = ornd.apply_and_round(os_, xs, chi_max=64)[0]
+    n2, n2r = float(linalg.overlap(cores, cores).item()), ornd.norm2(ref)
+    ov, ovr = float(linalg.overlap(cores, X).item()), oref.overlap(ref, xs)
+    assert abs(n2 - n2r) / n2r < 1e-5
+    assert abs(ov - ovr) / np.sqrt(n2r * ornd.norm2(xs)) < 1e-5
