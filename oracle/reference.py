@@ -298,6 +298,43 @@ class MPS:
         # evaluated here by sequential contraction, same values.
         return dense_mps(self.sites)
 
+    def apply(self, operator, index, strict=True, mode="compress"):
+        """Dense m-site gate on sites index..index+m-1
+        (tensor/matrix_product_state.py:487-534).
+
+        operator has 2m legs, the first m the outputs and the last m the
+        inputs contracted with the sites' physical legs (:497-501), so a
+        one-site gate acts as out = G @ in.  The contracted block
+        T (l, out_0..out_{m-1}, r) is split back left to right with a
+        complete QR of its left unfolding, keeping the first rank columns
+        of Q, rank = the site's existing right bond (:508-524): the new
+        block is the projection of the old onto the span of the first rank
+        columns of each unfolding.  The receiver is not modified (copy,
+        :488); strict=False fails as the reference does (``that`` unbound).
+        ``mode`` is unused there and here."""
+        if strict:
+            that = self.copy()
+        m = len(operator.shape) // 2
+        op = np.asarray(operator)
+        if op.ndim != 2 * m:
+            # opt_einsum rejects an operand with more legs than labels
+            raise ValueError("operator must have as many output as input legs")
+        T = that.sites[index]
+        for idx in range(index + 1, index + m):
+            T = np.tensordot(T, that.sites[idx], axes=([T.ndim - 1], [0]))
+        # T (l, in_0..in_{m-1}, r); contract the gate's input legs
+        T = np.tensordot(op, T, axes=(list(range(m, 2 * m)), list(range(1, m + 1))))
+        T = np.moveaxis(T, m, 0)           # (l, out_0..out_{m-1}, r)
+        for k in range(index, m + index - 1):
+            lrank, input_shape = T.shape[0], T.shape[1]
+            L = T.reshape(input_shape * lrank, -1)
+            Q, R = np.linalg.qr(L, mode="complete")
+            rank = that.shape[k][2]
+            that.sites[k] = that._tens(Q[:, :rank], k)
+            T = R[:rank, :].reshape((rank, -1, that.shape[k + 1][2]))
+        that.sites[m + index - 1] = that._tens(T, m + index - 1)
+        return that
+
 
 class MPO:
     """Oracle MatrixProductOperator (tensor/matrix_product_operator.py:18)."""

@@ -36,6 +36,28 @@ def to_device(t):
     return torch.as_tensor(np.asarray(t, dtype=np.float64), device=ops.device()).contiguous()
 
 
+def gate_matrix(operator, m):
+    """The (D_out, D_in) float64 device matrix of a gate tensor with m output
+    legs followed by m input legs.  A complex gate is accepted only when its
+    imaginary part is zero."""
+    if len(operator.shape) != 2 * m:
+        raise ValueError(f"a gate has as many output as input legs, got shape {tuple(operator.shape)}")
+    if isinstance(operator, torch.Tensor):
+        if operator.is_complex():
+            if bool((operator.imag != 0).any()):
+                raise TypeError("complex gates are not supported: the device path computes in float64")
+            operator = operator.real
+    else:
+        operator = np.asarray(operator)
+        if np.iscomplexobj(operator):
+            if np.any(operator.imag != 0):
+                raise TypeError("complex gates are not supported: the device path computes in float64")
+            operator = operator.real
+    dout = math.prod(int(s) for s in operator.shape[:m])
+    din = math.prod(int(s) for s in operator.shape[m:])
+    return to_device(operator).reshape(dout, din)
+
+
 def site_shapes(input_shape, bond_shape):
     # tensor/matrix_product_state.py:53-58, :225-230
     n = len(input_shape)
@@ -214,6 +236,64 @@ class MatrixProductState:
         per-bond singular values, kept ranks and discarded weights."""
         cores = rounding.tt_round(self.sites, chi_max, cutoff, report)
         return MatrixProductState.from_sites(cores, orthogonality="right")
+
+    # ------------------------------------------------------------------
+    # dense gates
+    # ------------------------------------------------------------------
+    def apply(self, operator, index, strict=True, mode="compress"):
+        """Dense m-site gate on sites index..index+m-1
+        (tensor/matrix_product_state.py:487-534).
+
+        ``operator`` (NumPy or torch, real) has 2m legs: the first m are the
+        outputs, the last m the inputs contracted with the sites' physical
+        legs, so a one-site gate acts as out = G @ in.  On the device:
+
+        1. the m sites are multiplied into one block T (l, D_in, r) by the
+           GEMM, D_in = the product of their physical dimensions;
+        2. the gate goes through the MPO x MPS apply kernel as a bond-1 MPO
+           core O[0, s, o, 0] = G[o, s], giving T (l, D_out, r);
+        3. the block is split back left to right as the reference does: the
+           left unfolding L of T is factored by Householder QR (the LAPACK
+           reflector convention, so zero columns give H = I as in numpy),
+           site k takes the first ``rank`` columns of the complete Q, rank =
+           its existing right bond, and T = Q_rank^T L (the leading rows of
+           R), reshaped (rank, -1, right bond of site k+1) exactly as the
+           reference reshapes it.
+
+        The receiver is not modified.  strict=False fails as the reference
+        does (``that`` is unbound, UnboundLocalError); ``mode`` is unused
+        there and here.  Shapes that the reference cannot reshape raise
+        ValueError.  Complex gates with a nonzero imaginary part raise
+        TypeError: the device path computes in float64."""
+        if strict:
+            that = self.copy()
+        m = len(operator.shape) // 2
+        G = gate_matrix(operator, m)
+        T = that.sites[index]
+        for idx in range(index + 1, index + m):
+            nxt = that.sites[idx]
+            T = ops.gemm(T.reshape(-1, T.shape[-1]), nxt.reshape(nxt.shape[0], -1))
+            T = T.reshape(that.sites[index].shape[0], -1, nxt.shape[-1])
+        if T.shape[1] != G.shape[1]:
+            raise ValueError(f"operands could not be broadcast: gate input dimension {G.shape[1]} "
+                             f"vs sites {T.shape[1]}")
+        O = G.t().contiguous().reshape(1, G.shape[1], G.shape[0], 1)
+        T = ops.apply_many([(O, T)])[0]
+        outs = tuple(int(s) for s in operator.shape[:m])
+        T = T.reshape(T.shape[0], *outs, T.shape[-1])
+        for k in range(index, m + index - 1):
+            lrank, input_shape = T.shape[0], T.shape[1]
+            L = linalg.reshape(T, (input_shape * lrank, T.numel() // (input_shape * lrank)))
+            rank = that.shape[k][2]
+            Q = ops.qr_many([(L, rank)])[0]                  # complete Q[:, :rank]
+            that.sites[k] = that.tensoricization(Q, k)
+            R = ops.gemm(Q.t(), L)                          # R[:rank] = Q_rank^T L
+            nr = that.shape[k + 1][2]
+            if R.numel() % (rank * nr):
+                raise ValueError(f"cannot reshape array of size {R.numel()} into shape ({rank},newaxis,{nr})")
+            T = R.reshape(rank, -1, nr)
+        that.sites[m + index - 1] = that.tensoricization(T, m + index - 1)
+        return that
 
     # ------------------------------------------------------------------
     # orthonormalisation

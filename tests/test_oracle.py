@@ -404,3 +404,164 @@ def test_mpo_compress_non_strict_and_retrieve():
     M.compress(2, strict=False)
     got = R.dense_mps([c.reshape(c.shape[0], -1, c.shape[-1]) for c in M.sites]).reshape(-1)
     np.testing.assert_allclose(got, expect, atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# MPS.apply (tensor/matrix_product_state.py:487-534) and the qubit register
+# (quantum/qbit.py, quantum/circuit.py)
+# --------------------------------------------------------------------------
+from oracle import quantum as QO  # noqa: E402
+
+
+def dense_gate(T, G, qubits):
+    """Apply the 2m-leg gate G (outputs first) to axes `qubits` of T."""
+    m = len(qubits)
+    out = np.tensordot(G, T, axes=(list(range(m, 2 * m)), list(qubits)))
+    return np.moveaxis(out, list(range(m)), list(qubits))
+
+
+def statevector(size, structure):
+    """Dense simulation of a gate list as the reference's register applies
+    it (quantum/qbit.py:30-61): (G, i) acts on qubits i..i+m-1; (G, a, b)
+    on neighbours acts on (min, max) whatever the order; on qubits further
+    apart it acts on (a, b), a the first leg, and when a < b the SWAPs are
+    undone from the top, which leaves qubits a and b exchanged."""
+    psi = np.zeros((2,) * size)
+    psi[(0,) * size] = 1.0
+    for op in structure:
+        G = op[0]
+        m = G.ndim // 2
+        if len(op) == 2:
+            psi = dense_gate(psi, G, list(range(op[1], op[1] + m)))
+            continue
+        a, b = op[1], op[2]
+        if abs(a - b) == 1:
+            psi = dense_gate(psi, G, [min(a, b), max(a, b)])
+        else:
+            psi = dense_gate(psi, G, [a, b])
+            if a < b:
+                psi = np.swapaxes(psi, a, b)
+    return psi
+
+
+def test_mps_apply_cx_known_answer():
+    # test/test_mps.py:9-19 (printed there): CX on sites 1, 2 of arange(8)
+    x = np.arange(8.0).reshape(2, 2, 2)
+    X = R.MPS(x, bond_shape=(2, 2)).decompose()
+    before = [s.copy() for s in X.sites]
+    Y = X.apply(QO.CX, 1)
+    want = x.copy()
+    want[:, 1, :] = x[:, 1, ::-1]          # flip qubit 2 where qubit 1 is set
+    np.testing.assert_allclose(Y.to_tensor(), want, atol=1e-12)
+    assert Y.shape == X.shape
+    for a, b in zip(X.sites, before):      # the receiver is untouched
+        np.testing.assert_array_equal(a, b)
+
+
+def test_mps_apply_one_site_exact():
+    rng = np.random.default_rng(21)
+    sites = random_sites(rng, (2, 3, 2, 4), (2, 5, 3))
+    X = R.MPS.from_sites(sites)
+    G = rng.standard_normal((3, 3))
+    Y = X.apply(G, 1)
+    np.testing.assert_allclose(Y.to_tensor(), dense_gate(X.to_tensor(), G, [1]), atol=1e-12)
+    assert Y.shape == X.shape
+
+
+def test_mps_apply_two_site_projector_closed_form():
+    # the split keeps the span of the first `rank` columns of the block's
+    # left unfolding: block' = P B with P = B_r B_r^+ (no QR involved)
+    rng = np.random.default_rng(22)
+    dims, bonds = (2, 3, 2, 3, 2), (2, 4, 3, 2)
+    sites = random_sites(rng, dims, bonds)
+    X = R.MPS.from_sites(sites)
+    G = rng.standard_normal((3, 2, 3, 2))          # sites 1, 2 have d = 3, 2
+    Y = X.apply(G, 1)
+    blk = np.einsum("asb,btc->astc", sites[1], sites[2])       # (2, 3, 2, 3)
+    blk = np.einsum("stuv,auvc->astc", G, blk)
+    B = blk.reshape(2 * 3, 2 * 3)
+    B = proj(B[:, :bonds[1]]) @ B                  # rank 4 of 6: a real truncation
+    new = [sites[0], B.reshape(2, 3, 2, 3)] + sites[3:]
+    T = new[0]
+    for s in new[1:]:
+        T = np.tensordot(T, s, axes=([T.ndim - 1], [0]))
+    np.testing.assert_allclose(Y.to_tensor(), T.reshape(dims), atol=1e-12)
+    assert Y.shape == X.shape
+    # bond large enough for the block: the gate is applied exactly
+    sites = random_sites(rng, (2, 2, 2), (2, 2))
+    Z = R.MPS.from_sites(sites)
+    U = rng.standard_normal((2, 2, 2, 2))
+    np.testing.assert_allclose(Z.apply(U, 0).to_tensor(), dense_gate(Z.to_tensor(), U, [0, 1]), atol=1e-12)
+
+
+def test_mps_apply_three_site_reshape_rule():
+    # three sites at bond 2 with r_1 = d_2 r_2: the reference's reshape
+    # (rank, -1, right bond of the next site) is consistent, the gate exact
+    rng = np.random.default_rng(23)
+    sites = random_sites(rng, (2, 2, 2), (2, 2))
+    X = R.MPS.from_sites(sites)
+    G = rng.standard_normal((2,) * 6)
+    np.testing.assert_allclose(X.apply(G, 0).to_tensor(), dense_gate(X.to_tensor(), G, [0, 1, 2]), atol=1e-12)
+    # a middle bond that is not d_2 r_2 cannot be reshaped back: ValueError
+    sites = random_sites(rng, (2, 2, 2, 2), (2, 3, 2))
+    with pytest.raises(ValueError):
+        R.MPS.from_sites(sites).apply(G, 0)
+
+
+def test_mps_apply_errors():
+    rng = np.random.default_rng(24)
+    X = R.MPS.from_sites(random_sites(rng, (2, 2, 2), (2, 2)))
+    with pytest.raises(UnboundLocalError):
+        X.apply(QO.CX, 0, strict=False)
+    with pytest.raises(ValueError):
+        X.apply(np.ones((2, 2, 2)), 0)
+
+
+def test_qbit_ground_and_bell():
+    q = QO.Qbit(3)
+    np.testing.assert_allclose(q.to_tensor(), np.eye(8)[0], atol=0)
+    # test/test_quantum.py:155-164: H on 0 then CX(0, 1)
+    b = QO.Qbit(2) @ (QO.H, 0) @ (QO.CX, 0, 1)
+    np.testing.assert_allclose(b.to_tensor(), np.array([1, 0, 0, 1]) / np.sqrt(2), atol=1e-12)
+
+
+def test_qbit_binary_and_toffoli():
+    q = QO.Qbit.from_binary("110")
+    QO.Qbit.LSB = False
+    try:
+        assert q.to_binary() == "110"
+        assert (q @ (QO.TOFFOLI, 0)).to_binary() == "111"
+        assert (QO.Qbit.from_binary("010") @ (QO.TOFFOLI, 0)).to_binary() == "010"
+    finally:
+        QO.Qbit.LSB = True
+    assert QO.Qbit.from_binary("110").to_binary() == "011"
+
+
+def test_qbit_ghz_and_bernstein_vazirani_dense():
+    # GHZ: Schmidt rank 2 on every cut, so bond 2 holds it exactly
+    ghz = [(QO.H, 0)] + [(QO.CX, i, i + 1) for i in range(4)]
+    np.testing.assert_allclose(QO.run_circuit(5, ghz).to_tensor(), statevector(5, ghz).reshape(-1), atol=1e-12)
+    # test/test_quantum.py:82-113 with a short token: CX(ancilla, i)
+    token = "1011"
+    n = len(token)
+    bv = [(QO.H, n), (QO.Z, n)] + [(QO.H, i) for i in range(n)]
+    bv += [(QO.CX, n, i) for i in range(n) if token[i] == "1"] + [(QO.H, i) for i in range(n)]
+    got = QO.run_circuit(n + 1, bv).to_tensor()
+    want = statevector(n + 1, bv).reshape(-1)
+    np.testing.assert_allclose(got, want, atol=1e-12)
+    # only the neighbour of the ancilla sees the phase kick-back (its CX has
+    # the query qubit as control): the register ends as |0001> (x) |->
+    np.testing.assert_allclose(got[[2, 3]], [2 ** -0.5, -2 ** -0.5], atol=1e-12)
+
+
+def test_qbit_swap_network_quirk():
+    # (G, a, b) with a < b not neighbours: the SWAPs are undone from the top
+    # (quantum/qbit.py:88-103), which exchanges qubits a and b afterwards
+    prep = [(QO.H, 0), (QO.X, 1)]
+    q = QO.run_circuit(3, prep) @ (QO.CX, 0, 2)
+    want = dense_gate(statevector(3, prep), QO.CX, [0, 2]).transpose(2, 1, 0)
+    np.testing.assert_allclose(q.to_tensor(), want.reshape(-1), atol=1e-12)
+    # with the control above the target the network is undone correctly
+    q = QO.run_circuit(3, [(QO.X, 2), (QO.H, 0)]) @ (QO.CX, 2, 0)
+    want = dense_gate(statevector(3, [(QO.X, 2), (QO.H, 0)]), QO.CX, [2, 0])
+    np.testing.assert_allclose(q.to_tensor(), want.reshape(-1), atol=1e-12)
