@@ -45,7 +45,8 @@ int syn_version(void);
 int syn_device_count(int *count);
 /* ABI self-check: writes sizeof() of the item structs below, in declaration
  * order (gemm, apply, mpo_mpo, dsum, qr, jacobi, trunc, rowscale, bjac,
- * eye, copy, diagmax, chol, gbj), into out[0..n-1]; returns the number of structs. */
+ * eye, copy, diagmax, chol, gbj, lusign, colscale), into out[0..n-1]; returns the
+ * number of structs. */
 int syn_struct_sizes(int64_t *out, int n);
 
 /* ------------------------------------------------------------------ */
@@ -165,6 +166,58 @@ typedef struct {
     double *tau;
 } syn_qr_item;
 int syn_qr_f64(const syn_qr_item *items, int n_items, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Householder signs of a CholeskyQR factor (the fast path of the QR   */
+/* above for tall, well-conditioned matrices).  If A = Q R with R's     */
+/* diagonal positive (CholeskyQR2 on the MFMA GEMM), the Householder QR */
+/* of A in the LAPACK convention is (Q D, D R), D = diag(+-1): the      */
+/* compact-WY form Q_h - [I; 0] = -Y T Y1^T is an LU factorisation, so */
+/* D follows from the LU without pivoting of Q[:n, :n] - D in which     */
+/* each D_jj = -sign(c_j), c_j the diagonal entry of the Schur          */
+/* complement when column j is reached (the pivot c_j - D_jj = -tau_j  */
+/* D_jj has magnitude in [1, 2]).  Ballard et al., "Reconstructing     */
+/* Householder vectors from tall-skinny QR", JPDC 85 (2015).            */
+/* LAPACK does not reflect a column whose part below the diagonal is   */
+/* exactly zero (tau = 0, D_jj = sign(c_j)); that is always the case   */
+/* for row m-1 of an m-row matrix (free_col), and otherwise shows as    */
+/* |c_j| = 1, so |c_j| > amb sets *flag and the caller falls back to    */
+/* syn_qr_f64.                                                          */
+/* syn_lu_sign_leaf_f64 factors diagonal blocks of at most 64 (one      */
+/* work-group per item): a (bs x bs, lda) is read, d[0..bs) receives D, */
+/* linv / uinv (bs x bs, ldl / ldu; skipped when linv is NULL) the      */
+/* inverses of the unit lower L and of U.  The caller updates the rest  */
+/* of the matrix with syn_gemm_f64 (right-looking blocked LU).          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *a;
+    int64_t lda, bs;
+    int64_t free_col; /* column (within the block) that LAPACK cannot reflect, or -1 */
+    double *d;
+    double *linv;
+    int64_t ldl;
+    double *uinv;
+    int64_t ldu;
+    int32_t *flag;
+} syn_lusign_item;
+int syn_lu_sign_leaf_f64(const syn_lusign_item *items, int n_items, double amb, void *stream);
+
+/* Column scaling with optional transpose: per item dst = src diag(s)    */
+/* (rows x cols, trans = 0) or its transpose (cols x rows, trans = 1);  */
+/* s = NULL copies without scaling.  Row-major with leading dimensions  */
+/* lds / ldd.  64 x 64 tiles through LDS; block_start is the prefix sum */
+/* of the tiles of the items before.                                    */
+typedef struct {
+    const double *src;
+    int64_t lds;
+    double *dst;
+    int64_t ldd;
+    int64_t rows, cols;
+    const double *s;
+    int32_t trans;
+    int64_t block_start;
+} syn_colscale_item;
+int syn_colscale_f64(const syn_colscale_item *items, int n_items, int64_t total_blocks, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* One-sided Jacobi SVD (Hestenes), one work-group per item.             */

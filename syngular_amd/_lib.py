@@ -65,8 +65,15 @@ GBJ_ITEM = np.dtype(
     [("g", _P), ("ldg", _I), ("rt", _P), ("ldr", _I), ("s", _P), ("w", _P), ("nv", _I), ("solve_start", _I),
      ("tile_start", _I), ("counter", _P), ("active", np.int32)], align=True)
 
+LUSIGN_ITEM = np.dtype(
+    [("a", _P), ("lda", _I), ("bs", _I), ("free_col", _I), ("d", _P), ("linv", _P), ("ldl", _I), ("uinv", _P),
+     ("ldu", _I), ("flag", _P)], align=True)
+COLSCALE_ITEM = np.dtype(
+    [("src", _P), ("lds", _I), ("dst", _P), ("ldd", _I), ("rows", _I), ("cols", _I), ("s", _P),
+     ("trans", np.int32), ("block_start", _I)], align=True)
+
 ITEM_DTYPES = [GEMM_ITEM, APPLY_ITEM, MPO_MPO_ITEM, DSUM_ITEM, QR_ITEM, JACOBI_ITEM, TRUNC_ITEM, ROWSCALE_ITEM,
-               BJAC_ITEM, EYE_ITEM, COPY_ITEM, DIAGMAX_ITEM, CHOL_ITEM, GBJ_ITEM]
+               BJAC_ITEM, EYE_ITEM, COPY_ITEM, DIAGMAX_ITEM, CHOL_ITEM, GBJ_ITEM, LUSIGN_ITEM, COLSCALE_ITEM]
 
 # every symbol the header declares, with its ctypes signature
 _VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -82,6 +89,8 @@ SIGNATURES = {
     "syn_kron_bonds_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_direct_sum_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_qr_f64": (_I32, [_VP, _I32, _VP]),
+    "syn_lu_sign_leaf_f64": (_I32, [_VP, _I32, _D, _VP]),
+    "syn_colscale_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_jacobi_f64": (_I32, [_VP, _I32, _D, _I32, _VP]),
     "syn_jacobi_gram_f64": (_I32, [_VP, _I32, _I64, _D, _I32, _VP]),
     "syn_truncate_f64": (_I32, [_VP, _I32, _VP]),

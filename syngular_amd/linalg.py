@@ -46,7 +46,7 @@ def qr_keep(L, keep):
     """First `keep` columns of the complete Householder Q of L and the matching
     rows of R (numpy.linalg.qr(L, 'complete') then Q[:, :keep], R[:keep]).
     Returns (Q (m x kq), S (kq x n)) with kq = min(keep, m)."""
-    Q = ops.qr_many([(L, keep)])[0]
+    Q = ops.householder_q(L, keep)
     S = ops.gemm(Q.t(), L)
     return Q, S
 
@@ -90,7 +90,7 @@ def right_orthonormalize(sites, shape):
         Rm = right_unfold(sites[k], shape[k])
         Lm = left_unfold(sites[k - 1], shape[k - 1])
         l = Rm.shape[0]
-        Vt = ops.qr_many([(Rm.t(), l)], q_trans=True)[0]   # (kq x d*r) = V^T
+        Vt = ops.householder_q(Rm.t(), l, q_trans=True)   # (kq x d*r) = V^T
         RQ = ops.gemm(Rm, Vt.t())                          # R V = U^T (l x kq)
         sites[k] = reshape(Vt, shape[k])
         sites[k - 1] = reshape(ops.gemm(Lm, RQ), shape[k - 1])
@@ -127,7 +127,7 @@ def decompose_right(T, shape):
             raise ValueError(f"cannot reshape array of size {T.numel()} into shape (newaxis,{cols})")
         Rm = T.contiguous().reshape(-1, cols)            # (rest x d*rrank)
         rank = shape[k][0]
-        Qt = ops.qr_many([(Rm.t(), rank)], q_trans=True)[0]   # (kq x d*rrank)
+        Qt = ops.householder_q(Rm.t(), rank, q_trans=True)   # (kq x d*rrank)
         sites[k] = reshape(Qt, shape[k])
         T = ops.gemm(Rm, Qt.t())                          # R[:rank].T = Rm @ Q  (rest x kq)
         params += sites[k].numel()

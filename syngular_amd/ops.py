@@ -309,6 +309,127 @@ def qr_many(problems, q_trans=False):
     return [o[0] for o in outs]
 
 
+def colscale_many(problems):
+    """problems: list of (src (rows x cols row-major view), dst, s (cols,) or
+    None, trans): dst = src diag(s), or its transpose (cols x rows) when
+    trans; s None copies."""
+    items = np.zeros(len(problems), dtype=_lib.COLSCALE_ITEM)
+    start = 0
+    for i, (src, dst, sc, trans) in enumerate(problems):
+        ps, lds, ts = as_matrix(src)
+        pd, ldd, td = as_matrix(dst)
+        rows, cols = src.shape
+        if ts or td:
+            raise ValueError("colscale needs row-major views")
+        if tuple(dst.shape) != ((cols, rows) if trans else (rows, cols)):
+            raise ValueError(f"colscale: destination {tuple(dst.shape)} does not fit source {(rows, cols)}")
+        it = items[i]
+        it["src"], it["lds"], it["dst"], it["ldd"] = ps, lds, pd, ldd
+        it["rows"], it["cols"], it["s"], it["trans"] = rows, cols, (0 if sc is None else _ptr(sc)), int(trans)
+        it["block_start"] = start
+        start += ((rows + 63) // 64) * ((cols + 63) // 64)
+    if start:
+        dev = _upload(items)
+        call("syn_colscale_f64", _ptr(dev), len(problems), start, _stream())
+
+
+def transpose(X):
+    """Row-major copy of X^T (X a row- or column-major matrix view)."""
+    px, ld, tr = as_matrix(X)
+    if tr:
+        # X is the transpose of a row-major (cols x rows) view: copy that
+        X = X.t()
+        out = torch.empty((X.shape[0], X.shape[1]), dtype=F64, device=X.device)
+        colscale_many([(X, out, None, False)])
+        return out
+    out = torch.empty((X.shape[1], X.shape[0]), dtype=F64, device=X.device)
+    colscale_many([(X, out, None, True)])
+    return out
+
+
+# Householder QR through CholeskyQR2: from this many kept columns on, the Q
+# factor comes from two CholeskyQR passes on the MFMA GEMM and the LAPACK signs
+# from a blocked LU of its top block; fewer take the one-work-group kernel.
+HH_CHOLQR_MIN = 64
+# Switched off: the first device run of this path ended in a GPU memory fault
+# whose cause is not identified (the kernels' and the host flow's indexing
+# check in bounds on the CPU).  Until it is, every QR takes the one-work-group
+# Householder kernel.
+HH_FAST_PATH = False
+# |c_j| above this marks a column that LAPACK may leave unreflected (its part
+# below the diagonal exactly zero): the one-work-group kernel decides those.
+HH_SIGN_AMBIGUOUS = 1.0 - 1e-10
+LU_LEAF = 64
+
+
+def lu_signs(Qtop, m, flag):
+    """Householder signs D (nb,) for a positive-diagonal orthonormal factor
+    whose top nb x nb block is Qtop (row-major view), of an m-row matrix: the
+    blocked right-looking LU without pivoting of Qtop - D with D chosen per
+    column (syn_lu_sign_leaf_f64 on 64-column diagonal blocks, the GEMM for
+    the panels and the trailing update).  flag (1,) int32 is set when a sign
+    is ambiguous."""
+    nb = Qtop.shape[0]
+    dev = Qtop.device
+    M = torch.empty((nb, nb), dtype=F64, device=dev)
+    colscale_many([(Qtop, M, None, False)])
+    D = torch.empty(nb, dtype=F64, device=dev)
+    inv = torch.empty((2, LU_LEAF, LU_LEAF), dtype=F64, device=dev)
+    for j0 in range(0, nb, LU_LEAF):
+        bs = min(LU_LEAF, nb - j0)
+        last = j0 + bs >= nb
+        it = np.zeros(1, dtype=_lib.LUSIGN_ITEM)
+        it["a"], it["lda"], it["bs"] = _ptr(M) + 8 * (j0 * nb + j0), nb, bs
+        it["free_col"] = (m - 1 - j0) if j0 <= m - 1 < j0 + bs else -1
+        it["d"] = _ptr(D) + 8 * j0
+        it["linv"], it["ldl"] = (0 if last else _ptr(inv[0])), LU_LEAF
+        it["uinv"], it["ldu"] = _ptr(inv[1]), LU_LEAF
+        it["flag"] = _ptr(flag)
+        call("syn_lu_sign_leaf_f64", _ptr(_upload(it)), 1, HH_SIGN_AMBIGUOUS, _stream())
+        if last:
+            break
+        e = j0 + bs
+        L21 = gemm(M[e:, j0:e], inv[1][:bs, :bs])         # M21 U^-1
+        U12 = gemm(inv[0][:bs, :bs], M[j0:e, e:])         # L^-1 M12
+        gemm(L21, U12, C=M[e:, e:], alpha=-1.0, beta=1.0)
+    return D
+
+
+def householder_q(A, keep, q_trans=False):
+    """The first kq = min(keep, m) columns of the complete Householder Q of A
+    (m x n view) in the LAPACK convention, i.e. numpy.linalg.qr(A,
+    'complete')[0][:, :keep]; its transpose (kq x m) when q_trans.  Same
+    result as qr_many([(A, keep)], q_trans)[0].
+
+    With nb = min(n, keep) >= HH_CHOLQR_MIN and keep <= min(m, n) the first
+    keep columns of Q are those of the Householder QR of A[:, :keep], taken as
+    CholeskyQR2 (MFMA GEMM) times the sign matrix from lu_signs.  A pivot
+    dropped by the first pass (a rank-deficient block) or an ambiguous sign
+    sends the problem to the one-work-group Householder kernel."""
+    m, n = A.shape
+    if keep <= 0:
+        raise ValueError("keep must be positive")
+    if A.dtype != F64:
+        raise TypeError("qr expects float64")
+    nb = min(n, keep)
+    if not HH_FAST_PATH or nb < HH_CHOLQR_MIN or keep > n or keep > m:
+        return qr_many([(A, keep)], q_trans=q_trans)[0]
+    Ak = A[:, :nb]
+    if as_matrix(Ak)[2]:
+        Ak = transpose(Ak.t())
+    Q1, L1, dropped = cholqr(Ak)
+    if int(dropped.item()):
+        return qr_many([(A, keep)], q_trans=q_trans)[0]
+    Q, _, _ = cholqr_refine(Q1)
+    flag = torch.zeros(1, dtype=torch.int32, device=A.device)
+    D = lu_signs(Q[:nb], m, flag)
+    if int(flag.item()):
+        return qr_many([(A, keep)], q_trans=q_trans)[0]
+    out = torch.empty((nb, m) if q_trans else (m, nb), dtype=F64, device=A.device)
+    colscale_many([(Q, out, D, q_trans)])
+    return out
+
+
 JACOBI_TOL = 1e-15
 JACOBI_SWEEPS = 60
 

@@ -229,7 +229,7 @@ class MatrixProductOperator:
             for k in range(self.sites_number - 1, 0, -1):
                 Rm = linalg.right_unfold(self.sites[k], self.shape[k])
                 Lm = linalg.left_unfold(self.sites[k - 1], self.shape[k - 1])
-                Qt = ops.qr_many([(Rm.t(), min(min(Rm.shape), dim))], q_trans=True)[0]
+                Qt = ops.householder_q(Rm.t(), min(min(Rm.shape), dim), q_trans=True)
                 St = ops.gemm(Rm, Qt.t())
                 self.shape[k] = (dim,) + tuple(self.shape[k][1:])
                 self.shape[k - 1] = tuple(self.shape[k - 1][:-1]) + (dim,)

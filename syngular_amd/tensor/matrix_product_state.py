@@ -221,7 +221,7 @@ class MatrixProductState:
             Rm = linalg.right_unfold(self.sites[k], self.shape[k])
             Lm = linalg.left_unfold(self.sites[k - 1], self.shape[k - 1])
             red = min(Rm.shape)
-            Qt = ops.qr_many([(Rm.t(), min(red, dim))], q_trans=True)[0]    # Q[:, :dim].T
+            Qt = ops.householder_q(Rm.t(), min(red, dim), q_trans=True)    # Q[:, :dim].T
             St = ops.gemm(Rm, Qt.t())                                       # S[:dim].T = R Q
             self.shape[k] = (dim,) + tuple(self.shape[k][1:])
             self.shape[k - 1] = tuple(self.shape[k - 1][:-1]) + (dim,)
@@ -285,7 +285,7 @@ class MatrixProductState:
             lrank, input_shape = T.shape[0], T.shape[1]
             L = linalg.reshape(T, (input_shape * lrank, T.numel() // (input_shape * lrank)))
             rank = that.shape[k][2]
-            Q = ops.qr_many([(L, rank)])[0]                  # complete Q[:, :rank]
+            Q = ops.householder_q(L, rank)                  # complete Q[:, :rank]
             that.sites[k] = that.tensoricization(Q, k)
             R = ops.gemm(Q.t(), L)                          # R[:rank] = Q_rank^T L
             nr = that.shape[k + 1][2]

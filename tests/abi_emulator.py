@@ -34,6 +34,7 @@ class Emulator:
     def __init__(self):
         self.regions = []          # (start, end, tensor)
         self.calls = []
+        self.checked = True        # bounds-check every access against `regions`
 
     # ---- memory ------------------------------------------------------------
     def register(self, t):
@@ -43,7 +44,7 @@ class Emulator:
 
     def _check(self, addr, nbytes, what):
         addr = int(addr)
-        if nbytes <= 0:
+        if nbytes <= 0 or not self.checked:
             return
         for s, e, _ in self.regions:
             if s <= addr and addr + nbytes <= e:
@@ -297,6 +298,88 @@ class Emulator:
             self.buf(r["dropped"], 1, np.int32, "chol dropped")[0] += int((~keep).sum())
         return 0
 
+    def _chol_drop(self, G, tolrel):
+        """Cholesky with the pivot-drop rule; returns (L, X = inv(L~), drops)."""
+        nn = G.shape[0]
+        thr = tolrel * float(np.max(np.diag(G))) if nn else 0.0
+        a = G.copy()
+        L = np.zeros((nn, nn))
+        keep = np.ones(nn, bool)
+        for j in range(nn):
+            d = a[j, j]
+            if d > thr:
+                sd = np.sqrt(d)
+                L[j, j] = sd
+                L[j + 1:, j] = a[j + 1:, j] / sd
+                a[j + 1:, j + 1:] -= np.outer(a[j + 1:, j], a[j + 1:, j]) / d
+            else:
+                keep[j] = False
+        Lt = L.copy()
+        for i in np.nonzero(~keep)[0]:
+            Lt[i, :] = 0.0
+            Lt[i, i] = 1.0
+        X = np.linalg.inv(Lt)
+        X[:, ~keep] = 0.0
+        return L, X, int((~keep).sum())
+
+    def syn_cholqr_f64(self, a, m, n, lda, q, ldq, l, ldl, work, tolrel, dropped, mixed, stream):
+        A = self.mat(a, m, n, lda, 0, "cholqr a").copy()
+        L, X, drops = self._chol_drop(A.T @ A, tolrel)
+        self.mat(q, m, n, ldq, 0, "cholqr q")[...] = A @ X.T
+        self.mat(l, n, n, ldl, 0, "cholqr l")[...] = L
+        self.buf(dropped, 1, np.int32, "cholqr dropped")[0] += drops
+        return 0
+
+    def syn_cholqr_refine_f64(self, q1, m, n, ldq1, q, ldq, l, ldl, work, thresh, tolrel, dropped, full, mixed,
+                              stream):
+        # the first-order branch of the device is an optimisation of the same
+        # factorisation; the emulation always takes the Cholesky
+        return self.syn_cholqr_f64(q1, m, n, ldq1, q, ldq, l, ldl, work, tolrel, dropped, mixed, stream)
+
+    def syn_lu_sign_leaf_f64(self, items, n, amb, stream):
+        from syngular_amd import _lib
+        for r in self.table(items, n, _lib.LUSIGN_ITEM):
+            bs = int(r["bs"])
+            if not 1 <= bs <= 64:
+                raise BoundsError(f"lu sign leaf of size {bs}")
+            M = self.mat(r["a"], bs, bs, int(r["lda"]), 0, "lusign a").copy()
+            D = self.buf(r["d"], bs, what="lusign d")
+            amb_hit = False
+            for j in range(bs):
+                c = M[j, j]
+                if j == int(r["free_col"]):
+                    d = 1.0 if c >= 0 else -1.0
+                else:
+                    d = -1.0 if c >= 0 else 1.0
+                    amb_hit |= abs(c) > amb
+                D[j] = d
+                M[j, j] = c - d
+                M[j + 1:, j] /= M[j, j]
+                M[j + 1:, j + 1:] -= np.outer(M[j + 1:, j], M[j, j + 1:])
+            if int(r["linv"]):
+                Lm = np.tril(M, -1) + np.eye(bs)
+                self.mat(r["linv"], bs, bs, int(r["ldl"]), 0, "lusign linv")[...] = np.linalg.inv(Lm)
+                self.mat(r["uinv"], bs, bs, int(r["ldu"]), 0, "lusign uinv")[...] = np.linalg.inv(np.triu(M))
+            if amb_hit:
+                self.buf(r["flag"], 1, np.int32, "lusign flag")[0] = 1
+        return 0
+
+    def syn_colscale_f64(self, items, n, total, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.COLSCALE_ITEM)
+        self._blocks_ok(it, [((int(r["rows"]) + 63) // 64) * ((int(r["cols"]) + 63) // 64) for r in it], total,
+                        "colscale")
+        for r in it:
+            rows, cols = int(r["rows"]), int(r["cols"])
+            S = self.mat(r["src"], rows, cols, int(r["lds"]), 0, "colscale src").copy()
+            if int(r["s"]):
+                S = S * self.buf(r["s"], cols, what="colscale s")
+            if int(r["trans"]):
+                self.mat(r["dst"], cols, rows, int(r["ldd"]), 0, "colscale dst")[...] = S.T
+            else:
+                self.mat(r["dst"], rows, cols, int(r["ldd"]), 0, "colscale dst")[...] = S
+        return 0
+
     def syn_jacobi_gram_blocked_f64(self, items_dev, items_host, n, work, counters, tol, max_sweeps, sweeps, stream):
         from syngular_amd import _lib
         self.buf(items_dev, n * _lib.GBJ_ITEM.itemsize, np.uint8, "gram device table")
@@ -347,6 +430,9 @@ def _gram_workspace(nv):
 
 
 class _FakeLib:
+    def syn_cholqr_workspace(self, n):
+        return 1
+
     def syn_jacobi_gram_blocked_workspace(self, nv_ptr, n):
         nv = np.ctypeslib.as_array(ctypes.cast(nv_ptr, ctypes.POINTER(ctypes.c_int64)), (n,))
         return _gram_workspace(nv)
@@ -357,11 +443,18 @@ class _FakeLib:
         return _workspace(nv, ln)
 
 
-def install(monkeypatch):
-    """Route syngular_amd.batch / serving through the emulator on the CPU."""
+def install(monkeypatch, tensor_api=False):
+    """Route syngular_amd.batch / serving through the emulator on the CPU.
+    tensor_api: also route the single-problem calls of syngular_amd.ops (the
+    MatrixProductState / Operator path).  Those allocate with torch directly,
+    so accesses are not bounds-checked in that mode (the GPU tests cover the
+    same calls)."""
     from syngular_amd import _lib, batch, ops
 
     emu = Emulator()
+    if tensor_api:
+        emu.checked = False
+        monkeypatch.setattr(ops, "call", emu.call)
     cpu = torch.device("cpu")
 
     def empty(n, dtype=torch.float64, zero=False):

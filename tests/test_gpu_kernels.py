@@ -135,6 +135,45 @@ def test_householder_qr_strided_and_rank_deficient():
     np.testing.assert_allclose(q @ (q.T @ x), x, atol=1e-10)
 
 
+FAST_QR_SKIP = ("the CholeskyQR2 + sign-LU QR path is switched off (ops.HH_FAST_PATH): its first device run "
+                "faulted the GPU and the cause is not identified")
+
+
+@pytest.mark.skip(reason=FAST_QR_SKIP)
+@pytest.mark.parametrize("m,n,keep", [(512, 512, 256), (300, 200, 150), (200, 200, 200), (129, 64, 64),
+                                      (2048, 1024, 1024), (1000, 700, 640)])
+def test_householder_q_fast_path_matches_lapack(m, n, keep):
+    """CholeskyQR2 + blocked sign LU + column scaling: the same columns as
+    numpy.linalg.qr (LAPACK signs), in both output orientations."""
+    from syngular_amd import ops
+
+    case(f"householder_q m={m} n={n} keep={keep}")
+    rng = np.random.default_rng(m * 7 + n * 3 + keep)
+    A = rng.standard_normal((m, n))
+    Qn = np.linalg.qr(A, mode="complete")[0][:, :keep]
+    Q = ops.householder_q(T(A), keep).cpu().numpy()
+    assert Q.shape == Qn.shape
+    np.testing.assert_allclose(Q, Qn, atol=1e-11)
+    At = T(A.T.copy()).t()                           # transposed view, as in the right sweeps
+    Qt = ops.householder_q(At, keep, q_trans=True).cpu().numpy()
+    np.testing.assert_allclose(Qt, Qn.T, atol=1e-11)
+
+
+@pytest.mark.skip(reason=FAST_QR_SKIP)
+def test_householder_q_fallbacks_on_device():
+    from syngular_amd import ops
+
+    case("householder_q fallbacks")
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((200, 100))
+    A[:, 5] = A[:, 3]                                # rank-deficient block: first pass drops a pivot
+    Q = ops.householder_q(T(A), 80).cpu().numpy()
+    np.testing.assert_allclose(Q, np.linalg.qr(A, mode="complete")[0][:, :80], atol=1e-11)
+    E = np.eye(128)[:, :96].copy()                   # unreflected columns: ambiguous signs
+    Q = ops.householder_q(T(E), 96).cpu().numpy()
+    np.testing.assert_allclose(Q, np.linalg.qr(E, mode="complete")[0][:, :96], atol=1e-12)
+
+
 @pytest.mark.parametrize("nv,ln", [(2, 3), (5, 8), (16, 40), (33, 33), (64, 200), (100, 120)])
 def test_jacobi_svd(nv, ln):
     from syngular_amd import ops
