@@ -565,3 +565,79 @@ def test_qbit_swap_network_quirk():
     q = QO.run_circuit(3, [(QO.X, 2), (QO.H, 0)]) @ (QO.CX, 2, 0)
     want = dense_gate(statevector(3, [(QO.X, 2), (QO.H, 0)]), QO.CX, [2, 0])
     np.testing.assert_allclose(q.to_tensor(), want.reshape(-1), atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# DMRG environment blocks (variational/dmrg.py:65-112) and restore (:219-239)
+# --------------------------------------------------------------------------
+from oracle import variational as VO  # noqa: E402
+
+
+def random_mpo_sites(rng, dims, bonds):
+    b = [1] + list(bonds) + [1]
+    return [rng.standard_normal((b[k], d, d, b[k + 1])) for k, d in enumerate(dims)]
+
+
+def dense_sandwich(x, w_sites):
+    """sum_{s,t} x[s] W[in=t, out=s] x[t] from the dense operator."""
+    W = R.dense_mpo(w_sites)                      # (in..., out...)
+    n = x.ndim
+    Wm = W.reshape(int(np.prod(x.shape)), -1)     # rows: inputs t, cols: outputs s
+    v = x.reshape(-1)
+    return float(v @ Wm.T @ v) if n else 0.0
+
+
+@pytest.mark.parametrize("dims,bonds,wb", [((2, 2, 2, 2, 2), (2, 4, 4, 2), (3, 2, 3, 2)),
+                                           ((3, 2, 4, 2, 3, 2), (3, 5, 6, 4, 2), (2, 4, 3, 3, 2))])
+def test_environment_blocks_pair_to_dense_form(dims, bonds, wb):
+    rng = np.random.default_rng(sum(dims) + sum(bonds))
+    xs = random_sites(rng, dims, bonds)
+    ws = random_mpo_sites(rng, dims, wb)
+    E = dense_sandwich(R.dense_mps(xs), ws)
+    n = len(dims)
+    Lb = VO.left_blocks(ws, xs)
+    Rb = VO.right_blocks(ws, xs)
+    assert [b is None for b in Lb] == [False] * (n - 2) + [True] * 2
+    assert [b is None for b in Rb] == [True] * 2 + [False] * (n - 2)
+    for k in range(2, n - 1):
+        # left block of sites <= k-1 against right block of sites >= k
+        assert abs(np.sum(Lb[k - 1] * Rb[k]) - E) <= 1e-10 * max(1.0, abs(E))
+    # the outermost blocks closed with the remaining two sites by hand
+    Z1 = VO._sandwich(xs[n - 2], ws[n - 2])
+    Z2 = VO._sandwich(xs[n - 1], ws[n - 1])
+    tail = np.einsum("ace,acebdf,bdfghi->", Lb[n - 3], Z1, Z2)
+    assert abs(tail - E) <= 1e-10 * max(1.0, abs(E))
+    assert Lb[0].shape == (bonds[0], wb[0], bonds[0]) and Rb[n - 1].shape == (bonds[-1], wb[-1], bonds[-1])
+
+
+def test_restore_projector_closed_form():
+    rng = np.random.default_rng(31)
+    shapes = [(1, 2, 2), (2, 3, 4), (4, 2, 3), (3, 2, 1)]
+    site = rng.standard_normal((2, 3, 2, 3))
+    s1, s2 = VO.restore(shapes, site, 1)
+    assert s1.shape == shapes[1] and s2.shape == shapes[2]
+    B = site.reshape(6, 6)
+    got = s1.reshape(6, 4) @ s2.reshape(4, 6)
+    np.testing.assert_allclose(got, proj(B[:, :4]) @ B, atol=1e-12)
+    with pytest.raises(Exception):
+        VO.restore(shapes, site, 4)
+
+
+def test_environment_block_legs_against_dense_halves():
+    """A left block is the dense left half of ket, operator and bra with
+    their physical legs contracted: the ket's with the operator's output leg
+    and the bra's with its input leg (labels [1,3,2], [4,6,3,5], [7,6,8] at
+    variational/dmrg.py:96-100), indices [ket, operator, bra]."""
+    rng = np.random.default_rng(33)
+    dims, bonds, wb = (2, 3, 2, 2), (2, 3, 2), (2, 3, 2)
+    xs = random_sites(rng, dims, bonds)
+    ws = random_mpo_sites(rng, dims, wb)
+    # dense halves of sites 0..1
+    A = np.tensordot(xs[0], xs[1], axes=([2], [0]))[0]            # (s0, s1, b)
+    Wl = np.tensordot(ws[0], ws[1], axes=([3], [0]))[0]           # (t0, s0, t1, s1, d)
+    want = np.einsum("xyb,uxvyd,uvf->bdf", A, Wl, A)              # ket s meets out, bra t meets in
+    Lb = VO.left_blocks(ws, xs)
+    np.testing.assert_allclose(Lb[1], want, atol=1e-12)
+    # swapping the operator's in/out legs moves the result off (W is not symmetric)
+    swapped = np.einsum("xyb,xuyvd,uvf->bdf", A, Wl, A)
+    assert np.abs(swapped - Lb[1]).max() > 1e-3
