@@ -559,6 +559,56 @@ class MPO:
         R = self._right(self.sites[index], index)
         return R @ R.T
 
+    def apply(self, operator, indices):
+        """Sub-chain operator product, in place (tensor/matrix_product_operator.py:582-646).
+
+        For an MPO operator of m = len(indices) sites on consecutive sites of
+        the receiver: each receiver site's output leg is contracted with the
+        operator site's input leg (:591-592), the chain is contracted into
+        T (l, in_0..in_{m-1}, out_0..out_{m-1}, r) after squeezing the
+        operator's outer bonds (:594-603, which must be 1), and T is split
+        back left to right: reshape to (l * d_in * d_out, -1) in T's own
+        memory order (so for m > 1 the rows mix in_0 with in_1, as there),
+        complete QR, the first r_k columns of Q become site k with shape
+        (l, d_in, d_out, r_k) and T = R[:r_k] (:605-627).  Sites are
+        replaced in the receiver's list; shapes are not updated; returns
+        None.  Non-consecutive indices leave labels unmatched and raise
+        ValueError, as opt_einsum does.  A non-MPO operator is contracted and
+        the result discarded there (:628-646); here nothing is computed."""
+        if not self.decomposed:
+            raise Exception("MatrixProductState not decomposed")
+        if not isinstance(operator, MPO):
+            return None
+        m = len(indices)
+        if any(indices[k + 1] != indices[k] + 1 for k in range(m - 1)):
+            raise ValueError("indices must be consecutive")
+        letters = iter("abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ")
+        b = [next(letters) for _ in range(m + 1)]     # receiver bonds
+        c = [next(letters) for _ in range(m + 1)]     # operator bonds
+        i = [next(letters) for _ in range(m)]         # receiver inputs
+        j = [next(letters) for _ in range(m)]         # receiver outputs = operator inputs
+        p = [next(letters) for _ in range(m)]         # operator outputs
+        terms, ops_ = [], []
+        for k in range(m):
+            terms += [b[k] + i[k] + j[k] + b[k + 1], c[k] + j[k] + p[k] + c[k + 1]]
+            ops_ += [self.sites[indices[k]], operator.sites[k]]
+        out = b[0] + c[0] + "".join(i) + "".join(p) + b[m] + c[m]
+        T = np.einsum(",".join(terms) + "->" + out, *ops_, optimize=True)
+        T = np.squeeze(T, axis=(1, T.ndim - 1))       # (l, in..., out..., r)
+        for idx in range(m - 1):
+            j = indices[idx]
+            l, din = self.sites[j].shape[0], self.sites[j].shape[1]
+            dout = operator.sites[idx].shape[2]
+            L = np.reshape(T, (l * din * dout, -1))
+            Q, Rr = np.linalg.qr(L, mode="complete")
+            rr = self.sites[j].shape[3]
+            self.sites[j] = np.reshape(Q[:, :rr], (l, din, dout, rr))
+            T = Rr[:rr, :]
+        j = indices[-1]
+        l, din, rr = self.sites[j].shape[0], self.sites[j].shape[1], self.sites[j].shape[3]
+        self.sites[j] = np.reshape(T, (l, din, operator.sites[m - 1].shape[2], rr))
+        return None
+
     def retrieve(self, input_indices, output_indices):
         # tensor/matrix_product_operator.py:663-670
         v = np.ones((1, 1))

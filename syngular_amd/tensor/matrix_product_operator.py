@@ -286,6 +286,72 @@ class MatrixProductOperator:
     def tensoricization(self, matrix, index):
         return linalg.reshape(matrix, self.shape[index])
 
+    # ------------------------------------------------------------------
+    def apply(self, operator, indices):
+        """Sub-chain operator product, in place (tensor/matrix_product_operator.py:582-646).
+
+        operator (an MPO of m = len(indices) sites) acts on the consecutive
+        receiver sites `indices`: each receiver site's output leg meets the
+        operator site's input leg.  On the device:
+
+        1. P_k = receiver site x operator site (the MPO x MPO site kernel),
+           bonds (receiver, operator) pairs;
+        2. the chain of P_k is contracted into T (l, in_0..in_{m-1},
+           out_0..out_{m-1}, r), the reference's leg order (:594-603), one
+           grouped GEMM launch per site: block (l, ins) x in_k of the new T
+           is  T_k[(l, ins), outs, :] @ P_k[:, in_k, :, :];  the operator's
+           outer bonds must be 1 (the reference squeezes them);
+        3. the split of :605-627: T reshaped to (l d_in d_out, -1) in its own
+           memory order (for m > 1 the rows mix in_0 with in_1, as there),
+           Householder QR, the first r_k columns of Q as site k
+           (l, d_in, d_out, r_k) and T = Q^T L.
+
+        The receiver's site list is rewritten; shapes are not updated;
+        returns None.  Non-consecutive indices raise ValueError (unmatched
+        labels in the reference's contraction).  A non-MPO operator is
+        contracted and discarded by the reference (:628-646); here nothing is
+        computed."""
+        if not self.decomposed:
+            raise Exception("MatrixProductState not decomposed")
+        if not isinstance(operator, MatrixProductOperator):
+            return None
+        m = len(indices)
+        if any(indices[k + 1] != indices[k] + 1 for k in range(m - 1)):
+            raise ValueError("indices must be consecutive")
+        P = ops.mpo_mpo_many([(self.sites[j], operator.sites[k]) for k, j in enumerate(indices)])
+        squeeze = "cannot select an axis to squeeze out which has size not equal to one"
+        if operator.sites[0].shape[0] != 1 or operator.sites[m - 1].shape[3] != 1:
+            raise ValueError(squeeze)
+        l = int(P[0].shape[0])
+        T = P[0]                                           # (l, i0, p0, B1), grouped for m = 1
+        n_in, n_out = int(T.shape[1]), int(T.shape[2])     # products of the ins / outs so far
+        for k in range(1, m):
+            Pk = P[k]
+            bk, ik, pk, bn = (int(x) for x in Pk.shape)
+            Tk = T.reshape(l * n_in, n_out, bk)
+            Tn = torch.empty((l * n_in, ik, n_out, pk * bn), dtype=torch.float64, device=T.device)
+            Pv = Pk.reshape(bk, ik, pk * bn)
+            ops.gemm_many([(Tk[q], Pv[:, a, :], Tn[q, a], 1.0, 0.0) for q in range(l * n_in) for a in range(ik)])
+            T = Tn
+            n_in, n_out = n_in * ik, n_out * pk
+        T = T.reshape(l, n_in, n_out, -1)                  # (l, ins, outs, r) with the unit operator bond
+        for idx in range(m - 1):
+            j = indices[idx]
+            sl, din = int(self.sites[j].shape[0]), int(self.sites[j].shape[1])
+            dout = int(operator.sites[idx].shape[2])
+            rows = sl * din * dout
+            if rows == 0 or T.numel() % rows:
+                raise ValueError(f"cannot reshape array of size {T.numel()} into shape ({rows},newaxis)")
+            L = T.contiguous().reshape(rows, -1)
+            rr = int(self.sites[j].shape[3])
+            Q, S = linalg.qr_keep(L, rr)
+            self.sites[j] = linalg.reshape(Q, (sl, din, dout, rr))
+            T = S
+        j = indices[-1]
+        sl, din, rr = (int(self.sites[j].shape[x]) for x in (0, 1, 3))
+        self.sites[j] = linalg.reshape(T, (sl, din, int(operator.sites[m - 1].shape[2]), rr))
+        return None
+
 
 def _kron_bonds(A, B):
     """site[:, i, o, :] = kron(A[:, i, o, :], B[:, i, o, :]) for every (i, o)

@@ -141,6 +141,19 @@ class Emulator:
             P[:] = np.einsum("asb,csoe->acobe", X, O).reshape(-1)
         return 0
 
+    def syn_mpo_mpo_f64(self, items, n, total, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.MPO_MPO_ITEM)
+        nel = it["al"] * it["bl"] * it["din"] * it["dout"] * it["ar"] * it["br"]
+        self._blocks_ok(it, nel, total, "mpo_mpo", per=_lib.APPLY_BLOCK)
+        for r in it:
+            al, di, dm, ar, bl, do, br = (int(r[f]) for f in ("al", "din", "dmid", "ar", "bl", "dout", "br"))
+            A = self.buf(r["a"], al * di * dm * ar, what="mpo_mpo a").reshape(al, di, dm, ar)
+            B = self.buf(r["b"], bl * dm * do * br, what="mpo_mpo b").reshape(bl, dm, do, br)
+            P = self.buf(r["p"], al * bl * di * do * ar * br, what="mpo_mpo p")
+            P[:] = np.einsum("aijb,cjpe->acipbe", A, B).reshape(-1)
+        return 0
+
     def syn_eye_f64(self, items, n, total, stream):
         from syngular_amd import _lib
         it = self.table(items, n, _lib.EYE_ITEM)

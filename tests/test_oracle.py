@@ -641,3 +641,68 @@ def test_environment_block_legs_against_dense_halves():
     # swapping the operator's in/out legs moves the result off (W is not symmetric)
     swapped = np.einsum("xyb,xuyvd,uvf->bdf", A, Wl, A)
     assert np.abs(swapped - Lb[1]).max() > 1e-3
+
+
+# --------------------------------------------------------------------------
+# MPO.apply on a sub-chain (tensor/matrix_product_operator.py:582-646)
+# --------------------------------------------------------------------------
+def random_mpo_rect(rng, din, dout, bonds):
+    b = [1] + list(bonds) + [1]
+    return [rng.standard_normal((b[k], din[k], dout[k], b[k + 1])) for k in range(len(din))]
+
+
+def test_mpo_apply_single_site_dense():
+    rng = np.random.default_rng(51)
+    sites = random_mpo_rect(rng, (2, 3, 2), (3, 2, 2), (3, 4))
+    W = R.MPO.from_sites(list(sites))
+    B = rng.standard_normal((1, 2, 4, 1))              # site 1: output 2 -> 4
+    before = R.dense_mpo(sites)                        # (i0, i1, i2, o0, o1, o2)
+    assert W.apply(R.MPO.from_sites([B]), [1]) is None
+    want = np.einsum("abcdef,ep->abcdpf", before, B[0, :, :, 0])
+    np.testing.assert_allclose(R.dense_mpo(W.sites), want, atol=1e-12)
+    assert W.sites[1].shape == (3, 3, 4, 4) and W.shape[1] == (3, 3, 2, 4)   # shapes not updated
+
+
+def test_mpo_apply_two_sites_reshape_rule_exact():
+    # r_j0 = l d_in d_out: the split is exact, so site j0 x site j1 is T in
+    # its own memory order (l, in0, in1, out0, out1, r)
+    rng = np.random.default_rng(52)
+    sites = random_mpo_rect(rng, (2, 2, 2), (2, 2, 2), (4, 3))
+    sites[1] = rng.standard_normal((4, 2, 2, 16))
+    sites[2] = rng.standard_normal((16, 2, 2, 1))
+    ops_ = random_mpo_rect(rng, (2, 2), (2, 2), (3,))
+    T = np.einsum("aijb,cjpe,bklf,elqg->aikpqfcg", sites[1], ops_[0], sites[2], ops_[1])[..., 0, 0]
+    W = R.MPO.from_sites(list(sites))
+    W.apply(R.MPO.from_sites(list(ops_)), [1, 2])
+    got = np.tensordot(W.sites[1], W.sites[2], axes=([3], [0]))
+    np.testing.assert_allclose(got.reshape(-1), T.reshape(-1), atol=1e-10)
+
+
+def test_mpo_apply_two_sites_projector_closed_form():
+    rng = np.random.default_rng(53)
+    sites = random_mpo_rect(rng, (2, 2, 2, 2), (2, 2, 2, 2), (3, 5, 3))
+    ops_ = random_mpo_rect(rng, (2, 2), (2, 2), (2,))
+    T = np.einsum("aijb,cjpe,bklf,elqg->aikpqfcg", sites[1], ops_[0], sites[2], ops_[1])[..., 0, 0]
+    L = T.reshape(3 * 2 * 2, -1)                       # rows (l, in0, in1) in T's order
+    want = proj(L[:, :5]) @ L                          # rank 5 of 12
+    W = R.MPO.from_sites(list(sites))
+    W.apply(R.MPO.from_sites(list(ops_)), [1, 2])
+    got = W.sites[1].reshape(12, 5) @ W.sites[2].reshape(5, -1)
+    np.testing.assert_allclose(got, want, atol=1e-10)
+
+
+def test_mpo_apply_errors():
+    rng = np.random.default_rng(54)
+    sites = random_mpo_rect(rng, (2, 2, 2), (2, 2, 2), (3, 3))
+    W = R.MPO.from_sites(list(sites))
+    with pytest.raises(ValueError):
+        W.apply(R.MPO.from_sites(random_mpo_rect(rng, (2, 2), (2, 2), (2,))), [0, 2])
+    wide = [rng.standard_normal((2, 2, 2, 1))]         # operator left bond 2 cannot be squeezed
+    with pytest.raises(ValueError):
+        W.apply(R.MPO.from_sites(wide), [1])
+    assert W.apply(np.eye(2), [0]) is None
+    for a, b in zip(W.sites, sites):
+        np.testing.assert_array_equal(a, b)
+    D = R.MPO(rng.standard_normal((2, 2, 2, 2)), bond_shape=(2,))
+    with pytest.raises(Exception):
+        D.apply(R.MPO.from_sites(wide), [0])
