@@ -73,6 +73,7 @@ def parse():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-api", action="store_true", help="skip the timings of the drop-in API paths")
     ap.add_argument("--chain-only", action="store_true", help="only the headline chain (for profiling)")
     ap.add_argument("--cholqr-probe", action="store_true",
                     help="only the copy probe and one CholeskyQR pass on an 8192 x 4096 unfolding (the chain's "
@@ -91,7 +92,7 @@ def main():
               f"(asked for --gpus {args.gpus}, WORLD_SIZE={world}).", file=sys.stderr)
         sys.exit(2)
     if args.chain_only:
-        args.no_batch = args.no_serve = args.no_cpu_baseline = args.no_parity = True
+        args.no_batch = args.no_serve = args.no_cpu_baseline = args.no_parity = args.no_api = True
 
     import numpy as np
     import torch
@@ -155,6 +156,9 @@ def main():
     if not args.no_serve:
         forms["serve"] = serve_form(np, torch, args)
         log(f"serve form: {forms['serve']}")
+    if not args.no_api:
+        forms["api_paths"] = api_form(np, torch)
+        log(f"api paths: {forms['api_paths']}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -456,6 +460,128 @@ def serve_form(np, torch, args):
             "oracle_jobs": [f"N={j['n']} layers={j['layers']} chi_max={j['chi_max']}" for j in cheap],
             "norm2_rel_err_max": max(e[0] for e in errs),
             "overlap_rel_err_max": max(e[1] for e in errs)}
+
+
+def api_form(np, torch, small=False):
+    """The drop-in paths beside the headline, each timed on the device
+    (mean of a few calls after one warm-up, inputs resident) and as the NumPy
+    oracle on this host (one call, the reference's own algorithm), with the
+    relative error between the two on the same inputs:
+    dense two-site gates (MPS.apply), DMRG environment blocks, the
+    tensor-train Linear forward on a batch, MPO.apply on two sites and a
+    24-qubit GHZ circuit on the qubit register.  small: reduced sizes, for
+    a quick check of this function itself."""
+    from oracle import quantum as oq
+    from oracle import reference as oref
+    from oracle import variational as ovar
+
+    from syngular_amd.core import Linear
+    from syngular_amd.quantum import Qbit, gate
+    from syngular_amd.tensor import MatrixProductOperator as MPO, MatrixProductState as MPS
+    from syngular_amd.variational import left_blocks, right_blocks
+
+    rng = np.random.default_rng(77)
+
+    def sites(dims, bonds, four=False):
+        b = [1] + list(bonds) + [1]
+        if four:
+            return [rng.standard_normal((b[k], d, d, b[k + 1])) / np.sqrt(b[k] * d) for k, d in enumerate(dims)]
+        return [rng.standard_normal((b[k], d, b[k + 1])) / np.sqrt(b[k] * d) for k, d in enumerate(dims)]
+
+    def dev_time(f, reps=3):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = f()
+        torch.cuda.synchronize()
+        return out, 1e3 * (time.perf_counter() - t0) / reps
+
+    def cpu_time(f):
+        t0 = time.perf_counter()
+        out = f()
+        return out, 1e3 * (time.perf_counter() - t0)
+
+    def rel(a, b):
+        return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+    res = {"reduced_sizes": True} if small else {}
+    # dense two-site gate in the middle of a chi = 128 chain of 16 sites
+    n, chi = (8, 8) if small else (16, 128)
+    bonds = [min(chi, 2 ** (k + 1), 2 ** (n - k - 1)) for k in range(n - 1)]
+    xs = sites((2,) * n, bonds)
+    G = np.linalg.qr(rng.standard_normal((4, 4)))[0].reshape(2, 2, 2, 2)
+    X = MPS.from_sites(xs)
+    Y, td = dev_time(lambda: X.apply(G, n // 2 - 1))
+    Yo, tc = cpu_time(lambda: oref.MPS.from_sites(xs).apply(G, n // 2 - 1))
+    res["mps_apply_2site"] = {"workload": "chain N=16 d=2 chi=128, gate on sites 7-8", "device_ms": round(td, 3),
+                              "oracle_cpu_ms": round(tc, 3), "rel_err_dense": rel(Y.to_tensor(), Yo.to_tensor())}
+    log(f"api mps_apply_2site: {res['mps_apply_2site']}")
+    # DMRG environment blocks of a chi = 128 chain with a bond-8 operator
+    n, chi = (8, 8) if small else (24, 128)
+    bonds = [min(chi, 2 ** (k + 1), 2 ** (n - k - 1)) for k in range(n - 1)]
+    xs, ws = sites((2,) * n, bonds), sites((2,) * n, [8] * (n - 1), four=True)
+    X, Wm = MPS.from_sites(xs), MPO.from_sites(list(ws))
+    (Lg, Rg), td = dev_time(lambda: (left_blocks(Wm, X), right_blocks(Wm, X)))
+    (Lo, Ro), tc = cpu_time(lambda: (ovar.left_blocks(ws, xs), ovar.right_blocks(ws, xs)))
+    err = max(rel(g.cpu().numpy(), w) for g, w in zip(Lg + Rg, Lo + Ro) if w is not None)
+    res["dmrg_blocks"] = {"workload": "N=24 d=2 chi=128 D_w=8, all left and right blocks", "device_ms": round(td, 3),
+                          "oracle_cpu_ms": round(tc, 3), "max_rel_err": err}
+    log(f"api dmrg_blocks: {res['dmrg_blocks']}")
+    # tensor-train Linear forward on a batch of 64 states
+    np.random.seed(3)
+    layer = Linear(256, 256, core=4, bond=4) if small else Linear(4096, 4096, core=6, bond=8)
+    layer.build(None)
+    layer.is_built = True
+    Wl = layer.trainable_tensor_weights[0]["weight"]
+    xs_b = [sites((4,) * 4, (4, 8, 4)) if small else sites((4,) * 6, (4, 16, 16, 16, 4))
+            for _ in range(4 if small else 64)]
+    Xb = [MPS.from_sites(x) for x in xs_b]
+    outs, td = dev_time(lambda: layer(Xb))
+    _, td1 = dev_time(lambda: [Wl @ x for x in Xb], reps=1)
+    Wo = oref.MPO.from_sites([s.cpu().numpy() for s in Wl.sites])
+    wants, tc = cpu_time(lambda: [Wo @ oref.MPS.from_sites(x) for x in xs_b])
+    err = max(rel(g.to_tensor(), w.to_tensor()) for g, w in zip(outs[:4], wants[:4]))
+    res["linear_forward_batch64"] = {"workload": "Linear 4096->4096, core 6, bond 8, 64 states of bonds (4,16,16,16,4)",
+                                     "device_batched_ms": round(td, 3), "device_one_by_one_ms": round(td1, 3),
+                                     "oracle_cpu_ms": round(tc, 3), "max_rel_err_first4": err}
+    log(f"api linear_forward_batch64: {res['linear_forward_batch64']}")
+    # MPO.apply of a two-site operator on an operator chain of bond 16
+    ms_ = sites((2,) * 8, [4, 4, 4, 4, 4, 4, 4] if small else [4, 16, 16, 16, 16, 16, 4], four=True)
+    op2 = sites((2, 2), [4], four=True)
+
+    def dev_mpo():
+        W = MPO.from_sites(list(ms_))
+        W.apply(MPO.from_sites(list(op2)), [3, 4])
+        return W
+    Wd, td = dev_time(dev_mpo)
+
+    def cpu_mpo():
+        W = oref.MPO.from_sites(list(ms_))
+        W.apply(oref.MPO.from_sites(list(op2)), [3, 4])
+        return W
+    Wc, tc = cpu_time(cpu_mpo)
+    res["mpo_apply_2site"] = {"workload": "operator chain 8 sites d=2 bond 16, 2-site operator on sites 3-4",
+                              "device_ms": round(td, 3), "oracle_cpu_ms": round(tc, 3),
+                              "max_rel_err": max(rel(g.cpu().numpy(), w) for g, w in zip(Wd.sites, Wc.sites))}
+    log(f"api mpo_apply_2site: {res['mpo_apply_2site']}")
+    # 24-qubit GHZ on the register
+    nq = 6 if small else 24
+    ghz = [(gate.H, 0)] + [(gate.CX, i, i + 1) for i in range(nq - 1)]
+
+    def dev_ghz():
+        q = Qbit(nq)
+        for op in ghz:
+            q = q @ op
+        return q
+    q, td = dev_time(dev_ghz, reps=1)
+    qo, tc = cpu_time(lambda: oq.run_circuit(nq, [(oq.H, 0)] + [(oq.CX, i, i + 1) for i in range(nq - 1)]))
+    idx = [(0,) * nq, (1,) * nq]
+    res["qbit_ghz24"] = {"workload": "24-qubit GHZ (H + 23 CX) on the bond-2 register", "device_ms": round(td, 3),
+                         "oracle_cpu_ms": round(tc, 3),
+                         "max_abs_err_amplitudes": max(abs(q.state[i] - qo.state[i]) for i in idx)}
+    log(f"api qbit_ghz24: {res['qbit_ghz24']}")
+    return res
 
 
 def copy_probe(np, torch):

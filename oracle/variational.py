@@ -23,30 +23,32 @@ def _sandwich(X, W):
 def right_blocks(operator_sites, state_sites):
     """variational/dmrg.py:65-86: blocks[k] (k = n-1 .. 2), indices
     [ket, operator, bra] of bond k's left side; the last one contracts no
-    previous block (its right bonds are summed over)."""
+    previous block (its right bonds are summed over: a block of ones)."""
     n = len(operator_sites)
     blocks = [None] * n
     for k in range(n - 1, 1, -1):
-        Z = _sandwich(state_sites[k], operator_sites[k])
-        if k == n - 1:
-            blocks[k] = Z.sum(axis=(3, 4, 5))
-        else:
-            blocks[k] = np.einsum("acebdf,bdf->ace", Z, blocks[k + 1], optimize=True)
+        X, W = state_sites[k], operator_sites[k]
+        R = blocks[k + 1] if k != n - 1 else np.ones((X.shape[2], W.shape[3], X.shape[2]))
+        # pairwise, cheapest first: (e,t,f)(b,d,f) -> (e,t,b,d); with W -> (e,b,c,s); with X -> (a,e,c)
+        T = np.tensordot(X, R, axes=([2], [2]))
+        T = np.tensordot(T, W, axes=([1, 3], [1, 3]))
+        blocks[k] = np.tensordot(X, T, axes=([1, 2], [3, 1])).transpose(0, 2, 1)
     return blocks
 
 
 def left_blocks(operator_sites, state_sites):
     """variational/dmrg.py:89-110: blocks[k] (k = 0 .. n-3), indices [ket,
     operator, bra] of bond k+1's right side; the first one contracts no
-    previous block (its left bonds are summed over)."""
+    previous block (its left bonds are summed over: a block of ones)."""
     n = len(operator_sites)
     blocks = [None] * n
     for k in range(n - 2):
-        Z = _sandwich(state_sites[k], operator_sites[k])
-        if k == 0:
-            blocks[k] = Z.sum(axis=(0, 1, 2))
-        else:
-            blocks[k] = np.einsum("acebdf,ace->bdf", Z, blocks[k - 1], optimize=True)
+        X, W = state_sites[k], operator_sites[k]
+        L = blocks[k - 1] if k != 0 else np.ones((X.shape[0], W.shape[0], X.shape[0]))
+        # pairwise: (a,c,e)(a,s,b) -> (c,e,s,b); with W -> (e,b,t,d); with X -> (b,d,f)
+        T = np.tensordot(L, X, axes=([0], [0]))
+        T = np.tensordot(T, W, axes=([0, 2], [0, 2]))
+        blocks[k] = np.tensordot(T, X, axes=([0, 2], [0, 1]))
     return blocks
 
 
