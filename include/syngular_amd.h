@@ -167,6 +167,21 @@ typedef struct {
 } syn_qr_item;
 int syn_qr_f64(const syn_qr_item *items, int n_items, void *stream);
 
+/* Panel of the blocked Householder QR (one work-group of 1024 threads). */
+/* Factors columns [j0, j0+b) of the m-row matrix w (row-major, ldw;     */
+/* rows j0..m-1, the columns left of j0 already factored and the panel   */
+/* already updated by them) in place with the reflectors of syn_qr_f64   */
+/* (LAPACK convention): R on and above the diagonal, the scaled vectors  */
+/* below it; tau[j0..j0+b) the factors; v (row-major, ldv) receives in    */
+/* rows j0..m-1 of columns [j0, j0+b) the explicit V (unit diagonal,      */
+/* zeros above); t (b x b, ldt) the upper-triangular T of the compact WY  */
+/* form H_j0 ... H_{j0+b-1} = I - V T V^T.  b <= 32.  The caller updates  */
+/* the columns right of the panel and accumulates Q with syn_gemm_f64.   */
+/* Replaces numpy.linalg.qr on large matrices at the same call sites as  */
+/* syn_qr_f64.                                                            */
+int syn_hh_panel_f64(double *w, int64_t ldw, double *v, int64_t ldv, double *tau, double *t, int64_t ldt,
+                     int64_t m, int64_t j0, int64_t b, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Householder signs of a CholeskyQR factor (the fast path of the QR   */
 /* above for tall, well-conditioned matrices).  If A = Q R with R's     */

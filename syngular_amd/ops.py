@@ -276,6 +276,37 @@ def rowscale_many(problems):
         call("syn_rowscale_f64", _ptr(dev), len(problems), start, _stream())
 
 
+def copy_many(pairs):
+    """pairs: list of (src, dst) contiguous device tensors of equal size."""
+    items = np.zeros(len(pairs), dtype=_lib.COPY_ITEM)
+    start = 0
+    for i, (src, dst) in enumerate(pairs):
+        if src.numel() != dst.numel() or not (src.is_contiguous() and dst.is_contiguous()):
+            raise ValueError("copy needs contiguous tensors of equal size")
+        it = items[i]
+        it["src"], it["dst"], it["n"], it["block_start"] = _ptr(src), _ptr(dst), src.numel(), start
+        start += _blocks(src.numel())
+    if start:
+        dev = _upload(items)
+        call("syn_copy_f64", _ptr(dev), len(pairs), start, _stream())
+
+
+def eye_many(views):
+    """views: list of square row-major matrix views, each set to the identity."""
+    items = np.zeros(len(views), dtype=_lib.EYE_ITEM)
+    start = 0
+    for i, X in enumerate(views):
+        px, ld, tr = as_matrix(X)
+        if tr or X.shape[0] != X.shape[1]:
+            raise ValueError("eye needs a square row-major view")
+        it = items[i]
+        it["x"], it["n"], it["ld"], it["block_start"] = px, X.shape[0], ld, start
+        start += _blocks(X.shape[0] * X.shape[0])
+    if start:
+        dev = _upload(items)
+        call("syn_eye_f64", _ptr(dev), len(views), start, _stream())
+
+
 # --------------------------------------------------------------------------
 # factorizations
 # --------------------------------------------------------------------------
@@ -347,6 +378,80 @@ def transpose(X):
     return out
 
 
+# Blocked Householder QR: panels of this many columns factored by
+# syn_hh_panel_f64, everything else on the GEMM (compact WY form).
+HH_PANEL = 32
+# Problems with at least this many reflectors whose leading block does not
+# fit the one-work-group kernel's LDS (96 KB) take the blocked path.
+HH_BLOCKED_MIN_REFLECTORS = 64
+HH_LDS_DOUBLES = 96 * 1024 // 8
+
+
+def _row_major_copy(X):
+    """Contiguous row-major copy of a matrix view (GEMM with the identity for
+    strided or transposed views, a flat copy otherwise)."""
+    if X.is_contiguous():
+        out = torch.empty_like(X)
+        copy_many([(X.reshape(-1), out.reshape(-1))])
+        return out
+    I = torch.empty((X.shape[1], X.shape[1]), dtype=F64, device=X.device)
+    eye_many([I])
+    return gemm(X, I)
+
+
+def qr_blocked(A, keep, q_trans=False):
+    """Same result as qr_many([(A, keep)], q_trans)[0] (the first
+    kq = min(keep, m) columns of the complete Householder Q of A, LAPACK
+    convention), by the blocked algorithm: p = min(m, n, keep) reflectors,
+    panels of HH_PANEL columns factored by syn_hh_panel_f64, the columns
+    right of each panel updated as C -= V (T^T (V^T C)), and Q accumulated
+    backwards from [I; 0] as Q_sub -= V (T (V^T Q_sub)) (for Q^T:
+    Q^T_sub -= ((Q^T_sub V) T^T) V^T), all products on the MFMA GEMM."""
+    m, n = A.shape
+    if keep <= 0:
+        raise ValueError("keep must be positive")
+    if A.dtype != F64:
+        raise TypeError("qr expects float64")
+    nb = min(n, keep)
+    kq = min(keep, m)
+    p = min(m, nb)
+    dev = A.device
+    W = _row_major_copy(A[:, :nb])
+    V = torch.empty((m, nb), dtype=F64, device=dev)
+    tau = torch.empty(max(nb, 1), dtype=F64, device=dev)
+    panels = [(j0, min(HH_PANEL, p - j0)) for j0 in range(0, p, HH_PANEL)]
+    T = torch.empty((len(panels), HH_PANEL, HH_PANEL), dtype=F64, device=dev)
+    for q, (j0, b) in enumerate(panels):
+        call("syn_hh_panel_f64", _ptr(W), nb, _ptr(V), nb, _ptr(tau), _ptr(T[q]), HH_PANEL, m, j0, b, _stream())
+        if j0 + b < nb:
+            Vp, C = V[j0:, j0:j0 + b], W[j0:, j0 + b:]
+            Y = gemm(Vp.t(), C)
+            Z = gemm(T[q][:b, :b].t(), Y)
+            gemm(Vp, Z, C=C, alpha=-1.0, beta=1.0)
+    if q_trans:
+        Q = torch.zeros((kq, m), dtype=F64, device=dev)
+        eye_many([Q[:, :kq]])
+    else:
+        Q = torch.zeros((m, kq), dtype=F64, device=dev)
+        eye_many([Q[:kq, :]])
+    for q in range(len(panels) - 1, -1, -1):
+        j0, b = panels[q]
+        if j0 >= kq:
+            continue
+        Vp, Tq = V[j0:, j0:j0 + b], T[q][:b, :b]
+        if q_trans:
+            Qs = Q[j0:, j0:]                               # (kq - j0) x (m - j0)
+            Y = gemm(Qs, Vp)
+            Z = gemm(Y, Tq.t())
+            gemm(Z, Vp.t(), C=Qs, alpha=-1.0, beta=1.0)
+        else:
+            Qs = Q[j0:, j0:]                               # (m - j0) x (kq - j0)
+            Y = gemm(Vp.t(), Qs)
+            Z = gemm(Tq, Y)
+            gemm(Vp, Z, C=Qs, alpha=-1.0, beta=1.0)
+    return Q
+
+
 # Householder QR through CholeskyQR2: from this many kept columns on, the Q
 # factor comes from two CholeskyQR passes on the MFMA GEMM and the LAPACK signs
 # from a blocked LU of its top block; fewer take the one-work-group kernel.
@@ -413,6 +518,8 @@ def householder_q(A, keep, q_trans=False):
         raise TypeError("qr expects float64")
     nb = min(n, keep)
     if not HH_FAST_PATH or nb < HH_CHOLQR_MIN or keep > n or keep > m:
+        if min(m, nb) >= HH_BLOCKED_MIN_REFLECTORS and m * nb > HH_LDS_DOUBLES:
+            return qr_blocked(A, keep, q_trans)
         return qr_many([(A, keep)], q_trans=q_trans)[0]
     Ak = A[:, :nb]
     if as_matrix(Ak)[2]:

@@ -349,6 +349,46 @@ class Emulator:
         # factorisation; the emulation always takes the Cholesky
         return self.syn_cholqr_f64(q1, m, n, ldq1, q, ldq, l, ldl, work, tolrel, dropped, mixed, stream)
 
+    def syn_hh_panel_f64(self, w, ldw, v, ldv, tau, t, ldt, m, j0, b, stream):
+        """Unblocked Householder on panel columns [j0, j0+b) (LAPACK
+        convention), explicit V and the compact-WY T (dlarft, forward)."""
+        if not (0 <= j0 < m and 1 <= b <= 32 and ldw >= j0 + b and ldv >= j0 + b and ldt >= b):
+            raise BoundsError("hh panel arguments")
+        W = self.mat(w, m, ldw, ldw, 0, "hh panel w")
+        Vm = self.mat(v, m, ldv, ldv, 0, "hh panel v")
+        taus = self.buf(tau, j0 + b, what="hh panel tau")
+        T = self.mat(t, b, b, ldt, 0, "hh panel t")
+        for jj in range(b):
+            j = j0 + jj
+            x = W[j + 1:, j]
+            xn2 = float(x @ x)
+            alpha = W[j, j]
+            tj = 0.0
+            if xn2 > 0.0:
+                beta = -np.copysign(np.sqrt(alpha * alpha + xn2), alpha)
+                tj = (beta - alpha) / beta
+                W[j + 1:, j] = x / (alpha - beta)
+                W[j, j] = beta
+            taus[j] = tj
+            col = np.zeros(m - j0)
+            col[jj] = 1.0
+            if tj != 0.0:
+                col[jj + 1:] = W[j + 1:, j]
+            Vm[j0:, j] = col
+            if tj != 0.0 and jj + 1 < b:
+                vv = col[jj:]
+                C = W[j:, j + 1:j0 + b]
+                C -= tj * np.outer(vv, vv @ C)
+        Vp = Vm[j0:, j0:j0 + b]
+        Tl = np.zeros((b, b))
+        for jj in range(b):
+            tj = taus[j0 + jj]
+            Tl[jj, jj] = tj
+            if jj:
+                Tl[:jj, jj] = -tj * Tl[:jj, :jj] @ (Vp[:, :jj].T @ Vp[:, jj])
+        T[...] = Tl
+        return 0
+
     def syn_lu_sign_leaf_f64(self, items, n, amb, stream):
         from syngular_amd import _lib
         for r in self.table(items, n, _lib.LUSIGN_ITEM):

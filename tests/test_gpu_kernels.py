@@ -135,6 +135,40 @@ def test_householder_qr_strided_and_rank_deficient():
     np.testing.assert_allclose(q @ (q.T @ x), x, atol=1e-10)
 
 
+@pytest.mark.parametrize("m,n,keep", [(512, 512, 256), (300, 200, 150), (200, 200, 200), (129, 100, 100),
+                                      (520, 700, 300), (100, 300, 120), (256, 70, 256), (2048, 1024, 1024),
+                                      (4096, 512, 512)])
+def test_blocked_householder_qr_matches_lapack(m, n, keep):
+    """Panels of 32 on syn_hh_panel_f64, compact-WY updates and Q on the
+    GEMM: the same columns as numpy.linalg.qr (LAPACK signs), both
+    orientations, including kept columns beyond n and wide matrices."""
+    from syngular_amd import ops
+
+    case(f"blocked qr m={m} n={n} keep={keep}")
+    rng = np.random.default_rng(m * 5 + n * 11 + keep)
+    A = rng.standard_normal((m, n))
+    Qn = np.linalg.qr(A, mode="complete")[0][:, :min(keep, m)]
+    Q = ops.qr_blocked(T(A), keep).cpu().numpy()
+    assert Q.shape == Qn.shape
+    np.testing.assert_allclose(Q, Qn, atol=1e-11)
+    At = T(A.T.copy()).t()
+    Qt = ops.qr_blocked(At, keep, q_trans=True).cpu().numpy()
+    np.testing.assert_allclose(Qt, Qn.T, atol=1e-11)
+
+
+def test_blocked_householder_qr_rank_deficient():
+    from syngular_amd import ops
+
+    case("blocked qr rank deficient / structured")
+    x = np.arange(300 * 90, dtype=float).reshape(300, 90)        # rank 2
+    Q = ops.qr_blocked(T(x), 80).cpu().numpy()
+    np.testing.assert_allclose(Q.T @ Q, np.eye(80), atol=1e-12)
+    np.testing.assert_allclose(Q @ (Q.T @ x), x, atol=1e-8 * np.abs(x).max())
+    E = np.eye(200)[:, :120].copy()                              # columns LAPACK leaves unreflected
+    Q = ops.qr_blocked(T(E), 120).cpu().numpy()
+    np.testing.assert_allclose(Q, np.linalg.qr(E, mode="complete")[0][:, :120], atol=1e-12)
+
+
 FAST_QR_SKIP = ("the CholeskyQR2 + sign-LU QR path is switched off (ops.HH_FAST_PATH): its first device run "
                 "faulted the GPU and the cause is not identified")
 
