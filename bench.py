@@ -191,11 +191,22 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+# Kernels bound by latency (serial rotation rounds in LDS, 64-column
+# Cholesky leaves, one-work-group factorisations): no flop or byte count
+# describes them, so they get no roofline; bench.py names the largest of
+# them beside the roofline when it takes more GPU time than the dominant
+# work kernel.
+LATENCY_KERNELS = ("gram_solve_kernel", "chol_diag_kernel", "jacobi_kernel", "qr_kernel", "bjacobi_step_kernel")
+
+
 def roofline(prof, steps, ms_per_step, W, n, chi, dw, d, rep):
-    """Dominant kernel of the timed chain steps: algorithmic work (flops for
-    the GEMMs, bytes for the memory-bound kernels) over its event time."""
+    """Dominant work kernel of the timed chain steps (the largest share of GPU
+    time among kernels with an algorithmic work count): algorithmic work
+    (flops for the GEMMs, bytes for the memory-bound kernels) over its event
+    time."""
     total_ms = sum(v[1] for v in prof.values())
-    dom, (dl, dms, dwork) = max(prof.items(), key=lambda kv: kv[1][1])
+    work = {k: v for k, v in prof.items() if k not in LATENCY_KERNELS}
+    dom, (dl, dms, dwork) = max(work.items(), key=lambda kv: kv[1][1])
     if dom in BYTE_KERNELS:
         achieved = dwork / (dms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
@@ -209,6 +220,14 @@ def roofline(prof, steps, ms_per_step, W, n, chi, dw, d, rep):
     roof["launches_per_step"] = dl / steps
     roof["avg_launch_us"] = round(1e3 * dms / dl, 2)
     roof["share_of_gpu_time"] = round(dms / total_ms, 3) if total_ms else None
+    lat = {k: v for k, v in prof.items() if k in LATENCY_KERNELS}
+    if lat:
+        lk, (ll, lms, _) = max(lat.items(), key=lambda kv: kv[1][1])
+        if lms > dms:
+            roof["largest_kernel_by_time"] = {"kernel": lk, "share_of_gpu_time": round(lms / total_ms, 3),
+                                              "launches_per_step": ll / steps,
+                                              "avg_launch_us": round(1e3 * lms / ll, 2),
+                                              "bound": "latency (serial rounds in LDS; no flop/byte roofline)"}
     kept = list(rep.kept)
     textbook = W.apply_round_flops(*W.chain_shapes(n, chi, dw), d, kept)
     pipeline = {"textbook_flops_per_step": textbook,
@@ -220,18 +239,20 @@ def roofline(prof, steps, ms_per_step, W, n, chi, dw, d, rep):
     return roof, pipeline
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cholqr_8192x4096_gemm_traffic.json")
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE scaled by a known-size copy), or None
     when no counter run covers it.  The counters cannot be read from inside
-    this process; profiles/gemm_f64_single_128_traffic.md has the commands."""
-    path = os.path.join(ROOT, "profiles", "gemm_f64_single_128_traffic.json")
+    this process; profiles/cholqr_8192x4096_gemm_traffic.md has the commands."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_FILE) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    return rec["bytes_per_launch"] if rec.get("kernel") == kernel else None
+    return rec.get("bytes_per_launch", {}).get(kernel)
 
 
 # kernels whose profiler work figure is bytes moved (the rest: flops)

@@ -137,6 +137,7 @@ PROF_TAGS = {
     10: "gram_update_kernel",
     11: "gemm_mixed<64,64>",
     12: "gemm_mixed<128,128>",
+    13: "gemm_f64_splitk<128,128>",
 }
 
 

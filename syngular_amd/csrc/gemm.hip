@@ -57,7 +57,8 @@ struct GemmSmem {
 };
 
 template <int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, typename CT>
-__device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t t, GemmSmem<BM, BN, BK, CT> &sm) {
+__device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t t, GemmSmem<BM, BN, BK, CT> &sm,
+                                                int64_t klo, int64_t khi) {
     constexpr int NW = (BM / WM) * (BN / WN);
     constexpr int NT = NW * 64;
     constexpr int TI = WM / 16, TJ = WN / 16;
@@ -76,6 +77,8 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
     if (tri & 8) k = k < col0 + BN ? k : col0 + BN;
     if (tri & 4) kmin = row0;
     if (tri & 16) kmin = kmin > col0 ? kmin : col0;
+    if (kmin < klo) kmin = klo;   // this work-group's K slice (split-K); klo is a multiple of BK
+    if (k > khi) k = khi;
     kmin = (kmin / BK) * BK;
     constexpr bool ta = TA, tb = TB;
     const double *__restrict__ A = (const double *)P.A;
@@ -180,18 +183,19 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
 // loads carry no per-element branches and the register budget fits two
 // work-groups per compute unit.
 template <int BM, int BN, int BK, int WM, int WN, typename CT>
-__device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t) {
+__device__ __forceinline__ void gemm_f64_tile(const syn_gemm_item &P, int64_t t, int64_t klo = 0,
+                                              int64_t khi = INT64_MAX) {
     __shared__ GemmSmem<BM, BN, BK, CT> sm;
     if (!P.transA) {
         if (!P.transB)
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, false, CT>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, false, CT>(P, t, sm, klo, khi);
         else
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, true, CT>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, false, true, CT>(P, t, sm, klo, khi);
     } else {
         if (!P.transB)
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, false, CT>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, false, CT>(P, t, sm, klo, khi);
         else
-            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, true, CT>(P, t, sm);
+            gemm_f64_tile_t<BM, BN, BK, WM, WN, true, true, CT>(P, t, sm, klo, khi);
     }
 }
 
@@ -232,6 +236,41 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_single
     gemm_f64_tile<BM, BN, BK, WM, WN, CT>(P, tile_of_block(blockIdx.x, total, P.tiles_n, P.tri));
 }
 
+// Split-K form of a single problem: work-group b computes K slice
+// b / tiles of tile b % tiles; slice 0 writes C (with beta), slice s > 0
+// writes the partial product into ws + (s - 1) m ws_ld, and
+// gemm_splitk_reduce adds the partials into C.  Used where the tiles alone
+// would leave most of the chip idle for a last partial round (a Gram
+// matrix's lower triangle: 528 tiles of 128 on 512 work-group slots).
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_splitk(syn_gemm_item P, int64_t tiles,
+                                                                               int64_t kc, double *ws,
+                                                                               int64_t ws_ld) {
+    const int64_t bid = blockIdx.x;
+    const int64_t slice = bid / tiles, tloc = bid - slice * tiles;
+    const int64_t t = tile_of_block(tloc, tiles, P.tiles_n, P.tri);
+    const int64_t klo = slice * kc;
+    const int64_t khi = klo + kc < P.k ? klo + kc : P.k;
+    if (slice > 0) {
+        P.C = ws + (slice - 1) * P.m * ws_ld;
+        P.ldc = ws_ld;
+        P.beta = 0.0;
+    }
+    gemm_f64_tile<BM, BN, BK, WM, WN, double>(P, t, klo, khi);
+}
+
+__global__ __launch_bounds__(256) void gemm_splitk_reduce(double *C, int64_t ldc, const double *ws, int64_t ws_ld,
+                                                          int64_t m, int64_t n, int extra, int lower) {
+    const int64_t total = m * n;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e / n, j = e - i * n;
+        if (lower && j > i) continue;
+        double v = C[i * ldc + j];
+        for (int q = 0; q < extra; ++q) v += ws[q * m * ws_ld + i * ws_ld + j];
+        C[i * ldc + j] = v;
+    }
+}
+
 // One GEMM launched with its descriptor passed by value (no device table):
 // used by the library's own multi-step routines (Cholesky, CholeskyQR, ...).
 int gemm_f64_launch(syn_gemm_item P, hipStream_t s, bool mixed) {
@@ -265,6 +304,55 @@ int gemm_f64_launch(syn_gemm_item P, hipStream_t s, bool mixed) {
     else
         hipLaunchKernelGGL((gemm_f64_single<64, 64, 16, 32, 32>), dim3((unsigned)tiles), dim3(256), 0, s, P);
     SYN_LAUNCH_CHECK("gemm_f64_launch");
+    return SYN_OK;
+}
+
+// A lower-triangle product (tri bit 1) such as a Gram matrix, split along K
+// when that shortens the last round of tiles.  ws holds ws_cap doubles for
+// the partial products (the K slices after the first).  With 2 work-groups
+// per CU the chip holds 2 * CUs tiles of 128 at a time; for `live` computed
+// tiles, s slices take ceil(live s / slots) rounds of 1/s of the work.
+int gemm_f64_lower_launch(syn_gemm_item P, hipStream_t s, double *ws, int64_t ws_cap) {
+    if (P.m <= 0 || P.n <= 0) return SYN_OK;
+    const int tile = (P.m * P.n >= (int64_t)256 * 128 * 128) ? 1 : 0;
+    if (!(P.tri & 1) || tile == 0 || ws == nullptr || P.k < 1024) return gemm_f64_launch(P, s);
+    static int slots = 0;
+    if (slots == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                     hipSuccess || cus <= 0)
+            cus = 256;
+        slots = 2 * cus;
+    }
+    const int64_t bm = 128;
+    const int64_t tm = (P.m + bm - 1) / bm, tn = (P.n + bm - 1) / bm;
+    int64_t live = 0;
+    for (int64_t r = 0; r < tm; ++r) live += (r + 1 < tn ? r + 1 : tn);
+    int best = 1;
+    double best_cost = (double)((live + slots - 1) / slots);
+    for (int sl = 2; sl <= 3; ++sl) {
+        const int64_t kc = ((P.k + sl - 1) / sl + 15) / 16 * 16;
+        if (kc < 512 || (int64_t)(sl - 1) * P.m * P.n > ws_cap) break;
+        const double cost = (double)((live * sl + slots - 1) / slots) / sl;
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = sl;
+        }
+    }
+    if (best == 1) return gemm_f64_launch(P, s);
+    const int64_t kc = ((P.k + best - 1) / best + 15) / 16 * 16;
+    P.tiles_n = tn;
+    P.block_start = 0;
+    const int64_t tiles = tm * tn;
+    ProfScope ps(kTagGemmSplitK128, 2.0 * (double)P.m * (double)P.n * (double)P.k * 0.5, s);
+    hipLaunchKernelGGL((gemm_f64_splitk<128, 128, 16, 64, 64>), dim3((unsigned)(tiles * best)), dim3(256), 0, s, P,
+                       tiles, kc, ws, P.n);
+    SYN_LAUNCH_CHECK("gemm_f64_lower_launch split");
+    int64_t blocks = (P.m * P.n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(gemm_splitk_reduce, dim3((unsigned)blocks), dim3(256), 0, s, (double *)P.C, P.ldc,
+                       (const double *)ws, P.n, P.m, P.n, best - 1, 1);
+    SYN_LAUNCH_CHECK("gemm_f64_lower_launch reduce");
     return SYN_OK;
 }
 

@@ -19,6 +19,7 @@ enum ProfTag {
     kTagGramUpdate = 10,
     kTagGemmMixed64 = 11,
     kTagGemmMixed128 = 12,
+    kTagGemmSplitK128 = 13,
 };
 
 bool prof_on();

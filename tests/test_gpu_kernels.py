@@ -169,6 +169,26 @@ def test_blocked_householder_qr_rank_deficient():
     np.testing.assert_allclose(Q, np.linalg.qr(E, mode="complete")[0][:, :120], atol=1e-12)
 
 
+def test_cholqr_split_k_gram():
+    """An 8192 x 4096 CholeskyQR pass: its Gram matrix (528 live tiles of 128)
+    takes the split-K launch; Q is orthonormal and Q L^T = A."""
+    from syngular_amd import ops
+
+    case("cholqr 8192 x 4096 (split-K Gram)")
+    g = torch.Generator(device="cpu").manual_seed(5)
+    A = (torch.randn(8192, 4096, generator=g, dtype=torch.float64) / 64.0).to(dev())
+    Q, L, dropped = ops.cholqr(A)
+    assert int(dropped.item()) == 0
+    G = ops.gemm(Q.t(), Q)
+    I = torch.eye(4096, dtype=torch.float64, device=dev())
+    assert float((G - I).abs().max()) < 1e-10           # one pass: cond(A)^2 eps
+    R = ops.gemm(Q, L.t())
+    assert float((R - A).abs().max()) <= 1e-12 * float(A.abs().max()) * 64
+    Q2, R2 = ops.cholqr2(A)
+    G2 = ops.gemm(Q2.t(), Q2)
+    assert float((G2 - I).abs().max()) < 1e-13
+
+
 FAST_QR_SKIP = ("the CholeskyQR2 + sign-LU QR path is switched off (ops.HH_FAST_PATH): its first device run "
                 "faulted the GPU and the cause is not identified")
 
