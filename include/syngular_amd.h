@@ -454,8 +454,9 @@ int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_
 /* 2/3 grouped GEMM 64/128 tiles, 4 block-Jacobi step, 5 Cholesky        */
 /* diagonal block, 6 Householder QR, 7 LDS Jacobi, 8 MPO x MPS apply,    */
 /* 9/10 Gram block-Jacobi pair solve / tile update, 11/12 mixed GEMM       */
-/* 64/128 tiles, 13 split-K lower-triangle GEMM (CholeskyQR Gram matrices, */
-/* the partial-sum pass included).                                        */
+/* 64/128 tiles, 13/14 split-K GEMM of 128/64 tiles (CholeskyQR Gram       */
+/* matrices and the recursive Cholesky's products, partial-sum pass        */
+/* included).                                                             */
 /* syn_prof_note_flops sets the count attached to the next grouped GEMM */
 /* or apply launch (their tables live on the device).                    */
 /* ------------------------------------------------------------------ */

@@ -138,6 +138,7 @@ PROF_TAGS = {
     11: "gemm_mixed<64,64>",
     12: "gemm_mixed<128,128>",
     13: "gemm_f64_splitk<128,128>",
+    14: "gemm_f64_splitk<64,64>",
 }
 
 

@@ -43,6 +43,7 @@ __global__ __launch_bounds__(256) void maxdiag_kernel(const double *g, int64_t n
 // zeroed into zb (nb x nb, ld nb).  L~11 equals L11 with every dropped row
 // replaced by the unit row, so that a panel solve X L11^T = B over the kept
 // columns is X = B Z^T with zero columns at the dropped positions.
+template <int TG>
 __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, double *L, int64_t ldl, int64_t n,
                                                int64_t j0, double *zb, int64_t ldz, const double *maxdiag,
                                                double tolrel, int *dropped) {
@@ -50,33 +51,34 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
     __shared__ double lo[kNb][kNb + 1];  // L, then L~ (dropped rows -> unit rows)
     __shared__ int keep[kNb];
     const int nb = (int)((n - j0) < kNb ? (n - j0) : kNb);
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    constexpr int RB = kNb / TG, NT = TG * TG;
+    const int tid = threadIdx.x, tx = tid % TG, ty = tid / TG;
     const double thr = tolrel * (*maxdiag);
     {
-        double v[kNb * kNb / 256];
+        double v[kNb * kNb / NT];
 #pragma unroll
-        for (int q = 0; q < kNb * kNb / 256; ++q) {
-            const int e = tid + 256 * q, i = e >> 6, k = e & 63;
+        for (int q = 0; q < kNb * kNb / NT; ++q) {
+            const int e = tid + NT * q, i = e >> 6, k = e & 63;
             v[q] = (i < nb && k <= i) ? G[(j0 + i) * ldg + (j0 + k)] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < kNb * kNb / 256; ++q) {
-            const int e = tid + 256 * q, i = e >> 6, k = e & 63;
+        for (int q = 0; q < kNb * kNb / NT; ++q) {
+            const int e = tid + NT * q, i = e >> 6, k = e & 63;
             a[i][k] = v[q];
             lo[i][k] = 0.0;
         }
     }
     __syncthreads();
     // right-looking with the Schur complement in registers: thread (ty, tx)
-    // owns rows ty+16u and columns tx+16v (u, v < 4).  Column j is published
-    // through a double-buffered LDS column by its owners, so one step costs
-    // eight broadcast reads, sixteen register FMAs and one barrier.
+    // owns rows ty+TG*u and columns tx+TG*v (u, v < RB).  Column j is
+    // published through a double-buffered LDS column by its owners, so one
+    // step costs 2*RB broadcast reads, RB*RB register FMAs and one barrier.
     __shared__ double col[2][kNb];
-    double r[4][4];
+    double r[RB][RB];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < RB; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) r[u][v] = a[ty + 16 * u][tx + 16 * v];
+        for (int v = 0; v < RB; ++v) r[u][v] = a[ty + TG * u][tx + TG * v];
     if (tid < kNb) col[0][tid] = a[tid][0];
     __syncthreads();
     int ndrop = 0;
@@ -88,62 +90,62 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
         const double sd = kp ? sqrt(d) : 0.0, rd = kp ? 1.0 / d : 0.0;
         if (tid >= j && tid < nb) lo[tid][j] = (tid == j) ? sd : (kp ? cj[tid] / sd : 0.0);
         if (tid == 0) keep[j] = kp;
-        double ci[4], ck[4];
+        double ci[RB], ck[RB];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) ci[u] = cj[ty + 16 * u] * rd;
+        for (int u = 0; u < RB; ++u) ci[u] = cj[ty + TG * u] * rd;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) ck[v] = cj[tx + 16 * v];
+        for (int v = 0; v < RB; ++v) ck[v] = cj[tx + TG * v];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = ty + 16 * u;
+        for (int u = 0; u < RB; ++u) {
+            const int i = ty + TG * u;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int k = tx + 16 * v;
+            for (int v = 0; v < RB; ++v) {
+                const int k = tx + TG * v;
                 if (k > j && k <= i) r[u][v] = fma(-ci[u], ck[v], r[u][v]);
             }
         }
-        // publish column j+1 (owners: tx == (j+1) % 16)
+        // publish column j+1 (owners: tx == (j+1) % TG)
         const int jn = j + 1;
-        if (jn < nb && tx == (jn & 15)) {
+        if (jn < nb && tx == (jn % TG)) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < RB; ++u)
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    if (tx + 16 * v == jn) col[jn & 1][ty + 16 * u] = r[u][v];
+                for (int v = 0; v < RB; ++v)
+                    if (tx + TG * v == jn) col[jn & 1][ty + TG * u] = r[u][v];
         }
         __syncthreads();
     }
     if (tid == 0 && ndrop) atomicAdd(dropped, ndrop);
     // write L, then turn dropped rows of lo into unit rows (L~)
-    for (int e = tid; e < nb * nb; e += 256) {
+    for (int e = tid; e < nb * nb; e += NT) {
         const int i = e / nb, k = e % nb;
         L[(j0 + i) * ldl + (j0 + k)] = (k <= i) ? lo[i][k] : 0.0;
     }
     __syncthreads();
-    for (int e = tid; e < nb * nb; e += 256) {
+    for (int e = tid; e < nb * nb; e += NT) {
         const int i = e / nb, k = e % nb;
         if (!keep[i] && k <= i) lo[i][k] = (i == k) ? 1.0 : 0.0;
     }
     __syncthreads();
     // Z = inv(L~) by right-looking forward substitution in registers: thread
-    // (ty, tx) holds Z_acc at rows ty+16u, columns tx+16v; row k of Z is final
+    // (ty, tx) holds Z_acc at rows ty+TG*u, columns tx+TG*v; row k of Z is final
     // once updated by rows < k, its owners divide by L~_kk, publish it through
     // a double-buffered LDS row and write it out.
     __shared__ double zrow[2][kNb];
-    double zr[4][4];
+    double zr[RB][RB];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < RB; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) zr[u][v] = (ty + 16 * u == tx + 16 * v) ? 1.0 : 0.0;
+        for (int v = 0; v < RB; ++v) zr[u][v] = (ty + TG * u == tx + TG * v) ? 1.0 : 0.0;
     auto publish = [&](int k) {
-        if (k < nb && ty == (k & 15)) {
+        if (k < nb && ty == (k % TG)) {
             const double inv = 1.0 / lo[k][k];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (ty + 16 * u == k) {
+            for (int u = 0; u < RB; ++u)
+                if (ty + TG * u == k) {
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int c = tx + 16 * v;
+                    for (int v = 0; v < RB; ++v) {
+                        const int c = tx + TG * v;
                         const double zv = zr[u][v] * inv;
                         zr[u][v] = zv;
                         zrow[k & 1][c] = zv;
@@ -156,32 +158,39 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
     __syncthreads();
     for (int k = 0; k + 1 < nb; ++k) {
         const double *zk = zrow[k & 1];
-        double lk[4], zc[4];
+        double lk[RB], zc[RB];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) lk[u] = lo[ty + 16 * u][k];
+        for (int u = 0; u < RB; ++u) lk[u] = lo[ty + TG * u][k];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) zc[v] = zk[tx + 16 * v];
+        for (int v = 0; v < RB; ++v) zc[v] = zk[tx + TG * v];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (ty + 16 * u > k) {
+        for (int u = 0; u < RB; ++u)
+            if (ty + TG * u > k) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) zr[u][v] = fma(-lk[u], zc[v], zr[u][v]);
+                for (int v = 0; v < RB; ++v) zr[u][v] = fma(-lk[u], zc[v], zr[u][v]);
             }
         publish(k + 1);
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(256) void chol_diag_kernel(const double *G, int64_t ldg, double *L, int64_t ldl,
-                                                        int64_t n, int64_t j0, double *zb, int64_t ldz,
-                                                        const double *maxdiag, double tolrel, int *dropped) {
-    chol_diag_body(G, ldg, L, ldl, n, j0, zb, ldz, maxdiag, tolrel, dropped);
+// 16 x 16 threads with 4 x 4 register blocks: the four waves spread each
+// step's FMAs over the four SIMDs (one wave with 8 x 8 blocks measured twice
+// as slow: the barrier it saves costs less than the serialised FMAs).
+constexpr int kCholTG = 16;
+constexpr int kCholThreads = kCholTG * kCholTG;
+
+__global__ __launch_bounds__(kCholThreads) void chol_diag_kernel(const double *G, int64_t ldg, double *L,
+                                                                 int64_t ldl, int64_t n, int64_t j0, double *zb,
+                                                                 int64_t ldz, const double *maxdiag, double tolrel,
+                                                                 int *dropped) {
+    chol_diag_body<kCholTG>(G, ldg, L, ldl, n, j0, zb, ldz, maxdiag, tolrel, dropped);
 }
 
 // One leaf (n <= 64) per work-group, for many problems at once.
-__global__ __launch_bounds__(256) void chol_leaf_many_kernel(const syn_chol_item *__restrict__ items) {
+__global__ __launch_bounds__(kCholThreads) void chol_leaf_many_kernel(const syn_chol_item *__restrict__ items) {
     const syn_chol_item &P = items[blockIdx.x];
-    chol_diag_body(P.g, P.ldg, P.l, P.ldl, P.n, 0, P.x, P.ldx, P.maxdiag, P.tolrel, P.dropped);
+    chol_diag_body<kCholTG>(P.g, P.ldg, P.l, P.ldl, P.n, 0, P.x, P.ldx, P.maxdiag, P.tolrel, P.dropped);
 }
 
 __global__ __launch_bounds__(256) void diagmax_many_kernel(const syn_diagmax_item *__restrict__ items) {
@@ -221,7 +230,7 @@ static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, doub
         const int64_t j1 = j0 + nb;
         double *zb = Z + b * kNb * kNb;
         ProfScope ps(kTagCholDiag, (double)nb * nb * nb * (1.0 / 3.0 + 1.0 / 3.0), s);
-        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, G, ldg, L, ldl, n, j0, zb, (int64_t)kNb,
+        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(kCholThreads), 0, s, G, ldg, L, ldl, n, j0, zb, (int64_t)kNb,
                            scal, tolrel, dropped);
         SYN_LAUNCH_CHECK("chol_diag");
         if (j1 >= n) break;
@@ -245,33 +254,39 @@ static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, doub
 //   (L22, X22) = chol_inv(G22);  X21 = -X22 (L21 X11).
 // Every step above the 64 x 64 leaves is a GEMM.  L and X must be zero above
 // the diagonal on entry; T is scratch of (n/2)^2 doubles.
+// ws (ws_cap doubles), when given, lets the products split along K; the
+// CholeskyQR callers pass none: on the 8192 x 4096 chain the split saved
+// under 1% of a step there.
 static int chol_inv(double *G, int64_t ldg, double *L, int64_t ldl, double *X, int64_t ldx, int64_t n, double *T,
-                    const double *scal, int *dropped, double tolrel, hipStream_t s) {
+                    const double *scal, int *dropped, double tolrel, hipStream_t s, double *ws = nullptr,
+                    int64_t ws_cap = 0) {
     if (n <= kNb) {
         ProfScope ps(kTagCholDiag, (double)n * n * n * (2.0 / 3.0), s);
-        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, G, ldg, L, ldl, n, (int64_t)0, X, ldx, scal,
+        hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(kCholThreads), 0, s, G, ldg, L, ldl, n, (int64_t)0, X, ldx, scal,
                            tolrel, dropped);
         SYN_LAUNCH_CHECK("chol_diag");
         return SYN_OK;
     }
     const int64_t n1 = ((n / kNb + 1) / 2) * kNb, n2 = n - n1;
-    int rc = chol_inv(G, ldg, L, ldl, X, ldx, n1, T, scal, dropped, tolrel, s);
+    int rc = chol_inv(G, ldg, L, ldl, X, ldx, n1, T, scal, dropped, tolrel, s, ws, ws_cap);
     if (rc) return rc;
     double *L21 = L + n1 * ldl;
     // L21 = G21 X11^T (op(B) = X11^T upper)
-    rc = gemm_f64_launch(make_gemm(G + n1 * ldg, ldg, 0, X, ldx, 1, L21, ldl, n2, n1, n1, 1.0, 0.0, 8), s);
+    rc = gemm_f64_split_launch(make_gemm(G + n1 * ldg, ldg, 0, X, ldx, 1, L21, ldl, n2, n1, n1, 1.0, 0.0, 8), s, ws,
+                               ws_cap);
     if (rc) return rc;
     // G22 -= L21 L21^T, lower triangle only
-    rc = gemm_f64_launch(make_gemm(L21, ldl, 0, L21, ldl, 1, G + n1 * ldg + n1, ldg, n2, n2, n1, -1.0, 1.0, 1), s);
+    rc = gemm_f64_split_launch(
+        make_gemm(L21, ldl, 0, L21, ldl, 1, G + n1 * ldg + n1, ldg, n2, n2, n1, -1.0, 1.0, 1), s, ws, ws_cap);
     if (rc) return rc;
     rc = chol_inv(G + n1 * ldg + n1, ldg, L + n1 * ldl + n1, ldl, X + n1 * ldx + n1, ldx, n2, T, scal, dropped,
-                  tolrel, s);
+                  tolrel, s, ws, ws_cap);
     if (rc) return rc;
     // T = L21 X11 (op(B) = X11 lower);  X21 = -X22 T (op(A) = X22 lower)
-    rc = gemm_f64_launch(make_gemm(L21, ldl, 0, X, ldx, 0, T, n1, n2, n1, n1, 1.0, 0.0, 16), s);
+    rc = gemm_f64_split_launch(make_gemm(L21, ldl, 0, X, ldx, 0, T, n1, n2, n1, n1, 1.0, 0.0, 16), s, ws, ws_cap);
     if (rc) return rc;
-    return gemm_f64_launch(
-        make_gemm(X + n1 * ldx + n1, ldx, 0, T, n1, 0, X + n1 * ldx, ldx, n2, n1, n2, -1.0, 0.0, 2), s);
+    return gemm_f64_split_launch(
+        make_gemm(X + n1 * ldx + n1, ldx, 0, T, n1, 0, X + n1 * ldx, ldx, n2, n1, n2, -1.0, 0.0, 2), s, ws, ws_cap);
 }
 
 // max over the lower triangle of |G_ij - delta_ij|, grid-wide: every block
@@ -354,7 +369,7 @@ extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, doub
     double *scal = T + half * half + nblk * kNb * kNb;
     // G = A^T A, lower triangle only (the Cholesky reads nothing else)
     double *splitws = scal + 16;
-    int rc = gemm_f64_lower_launch(make_gemm(A, lda, 1, A, lda, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,
+    int rc = gemm_f64_split_launch(make_gemm(A, lda, 1, A, lda, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,
                                    2 * n * n);
     if (rc) return rc;
     if ((rc = zero(L, n, n, ldl, s))) return rc;
@@ -375,7 +390,7 @@ extern "C" int syn_chol_leaf_f64(const syn_chol_item *items, int n_items, void *
     if (n_items == 0) return SYN_OK;
     hipStream_t s = (hipStream_t)stream;
     ProfScope ps(kTagCholDiag, 0.0, s);
-    hipLaunchKernelGGL(chol_leaf_many_kernel, dim3(n_items), dim3(256), 0, s, items);
+    hipLaunchKernelGGL(chol_leaf_many_kernel, dim3(n_items), dim3(kCholThreads), 0, s, items);
     SYN_LAUNCH_CHECK("syn_chol_leaf_f64 launch");
     return SYN_OK;
 }
@@ -410,7 +425,7 @@ extern "C" int syn_cholqr_refine_f64(const double *Q1, int64_t m, int64_t n, int
     double *scal = T + half * half + nblk * kNb * kNb;
     // G = Q1^T Q1 (lower triangle) and max |G - I| on it
     double *splitws = scal + 16;
-    int rc = gemm_f64_lower_launch(make_gemm(Q1, ldq1, 1, Q1, ldq1, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,
+    int rc = gemm_f64_split_launch(make_gemm(Q1, ldq1, 1, Q1, ldq1, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,
                                    2 * n * n);
     if (rc) return rc;
     if ((rc = hip_check(hipMemsetAsync(scal + 1, 0, sizeof(double), s), "refine emax reset"))) return rc;

@@ -307,32 +307,38 @@ int gemm_f64_launch(syn_gemm_item P, hipStream_t s, bool mixed) {
     return SYN_OK;
 }
 
-// A lower-triangle product (tri bit 1) such as a Gram matrix, split along K
-// when that shortens the last round of tiles.  ws holds ws_cap doubles for
-// the partial products (the K slices after the first).  With 2 work-groups
-// per CU the chip holds 2 * CUs tiles of 128 at a time; for `live` computed
-// tiles, s slices take ceil(live s / slots) rounds of 1/s of the work.
-int gemm_f64_lower_launch(syn_gemm_item P, hipStream_t s, double *ws, int64_t ws_cap) {
+// One problem split along K when its tiles alone would leave work-group
+// slots idle in the last round (a Gram matrix's lower triangle: 528 tiles of
+// 128 on 512 slots).  ws holds ws_cap doubles for the partial products (the K slices
+// after the first).  With 2 work-groups per CU the chip holds 2 * CUs tiles
+// at a time; for `live` computed tiles (the lower triangle for tri bit 1),
+// s slices take ceil(live s / slots) rounds of 1/s of the work; s <= 4 and
+// every slice keeps at least 256 of K.
+int gemm_f64_split_launch(syn_gemm_item P, hipStream_t s, double *ws, int64_t ws_cap) {
     if (P.m <= 0 || P.n <= 0) return SYN_OK;
+    if (P.k <= 0 && P.beta == 1.0) return SYN_OK;
     const int tile = (P.m * P.n >= (int64_t)256 * 128 * 128) ? 1 : 0;
-    if (!(P.tri & 1) || tile == 0 || ws == nullptr || P.k < 1024) return gemm_f64_launch(P, s);
+    if (ws == nullptr || P.k < 512) return gemm_f64_launch(P, s);
     static int slots = 0;
     if (slots == 0) {
         int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-                                                     hipSuccess || cus <= 0)
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
         slots = 2 * cus;
     }
-    const int64_t bm = 128;
+    const int64_t bm = tile ? 128 : 64;
     const int64_t tm = (P.m + bm - 1) / bm, tn = (P.n + bm - 1) / bm;
     int64_t live = 0;
-    for (int64_t r = 0; r < tm; ++r) live += (r + 1 < tn ? r + 1 : tn);
+    if (P.tri & 1)
+        for (int64_t r = 0; r < tm; ++r) live += (r + 1 < tn ? r + 1 : tn);
+    else
+        live = tm * tn;
     int best = 1;
     double best_cost = (double)((live + slots - 1) / slots);
-    for (int sl = 2; sl <= 3; ++sl) {
+    for (int sl = 2; sl <= 4; ++sl) {
         const int64_t kc = ((P.k + sl - 1) / sl + 15) / 16 * 16;
-        if (kc < 512 || (int64_t)(sl - 1) * P.m * P.n > ws_cap) break;
+        if (kc < 256 || (int64_t)(sl - 1) * P.m * P.n > ws_cap) break;
         const double cost = (double)((live * sl + slots - 1) / slots) / sl;
         if (cost < best_cost - 1e-9) {
             best_cost = cost;
@@ -344,15 +350,23 @@ int gemm_f64_lower_launch(syn_gemm_item P, hipStream_t s, double *ws, int64_t ws
     P.tiles_n = tn;
     P.block_start = 0;
     const int64_t tiles = tm * tn;
-    ProfScope ps(kTagGemmSplitK128, 2.0 * (double)P.m * (double)P.n * (double)P.k * 0.5, s);
-    hipLaunchKernelGGL((gemm_f64_splitk<128, 128, 16, 64, 64>), dim3((unsigned)(tiles * best)), dim3(256), 0, s, P,
-                       tiles, kc, ws, P.n);
-    SYN_LAUNCH_CHECK("gemm_f64_lower_launch split");
+    double fl = 2.0 * (double)P.m * (double)P.n * (double)P.k;
+    const int nbits = __builtin_popcount((unsigned)P.tri);
+    if (nbits == 1) fl *= 0.5;
+    if (nbits >= 2) fl /= 3.0;
+    ProfScope ps(tile ? kTagGemmSplitK128 : kTagGemmSplitK64, fl, s);
+    if (tile)
+        hipLaunchKernelGGL((gemm_f64_splitk<128, 128, 16, 64, 64>), dim3((unsigned)(tiles * best)), dim3(256), 0, s,
+                           P, tiles, kc, ws, P.n);
+    else
+        hipLaunchKernelGGL((gemm_f64_splitk<64, 64, 16, 32, 32>), dim3((unsigned)(tiles * best)), dim3(256), 0, s,
+                           P, tiles, kc, ws, P.n);
+    SYN_LAUNCH_CHECK("gemm_f64_split_launch");
     int64_t blocks = (P.m * P.n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(gemm_splitk_reduce, dim3((unsigned)blocks), dim3(256), 0, s, (double *)P.C, P.ldc,
-                       (const double *)ws, P.n, P.m, P.n, best - 1, 1);
-    SYN_LAUNCH_CHECK("gemm_f64_lower_launch reduce");
+                       (const double *)ws, P.n, P.m, P.n, best - 1, (P.tri & 1) ? 1 : 0);
+    SYN_LAUNCH_CHECK("gemm_f64_split_launch reduce");
     return SYN_OK;
 }
 

@@ -20,6 +20,7 @@ enum ProfTag {
     kTagGemmMixed64 = 11,
     kTagGemmMixed128 = 12,
     kTagGemmSplitK128 = 13,
+    kTagGemmSplitK64 = 14,
 };
 
 bool prof_on();
