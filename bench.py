@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-api", action="store_true", help="skip the timings of the drop-in API paths")
+    ap.add_argument("--precision-form", action="store_true",
+                    help="only the precision workload (N=64, d=4, chi=1024, D_w=8, cutoff 1e-8): one plain FP64 "
+                         "step and one mixed step, their times and the mixed form's error (a few minutes)")
     ap.add_argument("--chain-only", action="store_true", help="only the headline chain (for profiling)")
     ap.add_argument("--cholqr-probe", action="store_true",
                     help="only the copy probe and one CholeskyQR pass on an 8192 x 4096 unfolding (the chain's "
@@ -103,6 +106,10 @@ def main():
     dev = ops.device()
     if args.copy_probe or args.cholqr_probe:
         copy_probe(np, torch)
+    if args.precision_form:
+        res = precision_form(np, torch)
+        print(json.dumps({"precision_form": res}), flush=True)
+        return
     if args.cholqr_probe:
         A = torch.randn(8192, 4096, dtype=torch.float64, device=dev) / 64.0
         ops.cholqr(A)
@@ -332,6 +339,74 @@ def chain_mixed_form(np, torch, args, O, X, plain_sites_per_s):
             "error_vs_fp64_oracle": {"reduced_chain": "N=12 d=2 chi=64 D_w=8", "norm2_rel_err": abs(n2 - n2r) / n2r,
                                      "overlap_rel_err": abs(ov - ovr) / float(np.sqrt(n2r * ornd.norm2(xs))),
                                      "svals_max_rel_err": err_s, "kept_match": r.kept == kept[1:]}}
+
+
+def precision_form(np, torch):
+    """The precision workload: N=64 sites, d=4, chi=1024, MPO bond 8
+    (product bond 8192), Gaussian cores as in the headline chain, rounded to
+    chi=1024 with relative cutoff 1e-8.  One step in plain FP64 and one in the
+    mixed form (products on the FP32 matrix cores; Gram matrices, Cholesky,
+    Jacobi and truncation in FP64), inputs resident.  The mixed result is
+    compared with the plain FP64 result of the same inputs (norm, overlap,
+    retained singular values per bond); the plain form itself is compared
+    with the NumPy FP64 oracle on a reduced chain of the same d and MPO bond
+    (N=8, chi=64), since the oracle cannot hold the full size."""
+    from oracle import reference as oref
+    from oracle import rounding as ornd
+    from syngular_amd import linalg, ops, rounding
+    from syngular_amd import workloads as W
+
+    n, d, chi, dw, cut = 64, 4, 1024, 8, 1e-8
+    xs, os_ = W.chain_inputs(n, d, chi, dw, seed=64)
+    dev = ops.device()
+    X = [torch.from_numpy(x).to(dev) for x in xs]
+    O = [torch.from_numpy(o).to(dev) for o in os_]
+    del xs, os_
+    out = {"workload": f"chain N={n} d={d} chi={chi} D_w={dw} (product bond {chi * dw}), round to chi={chi}, "
+                       f"relative cutoff {cut:g}", "steps_each": 1}
+    reps = {}
+    res = {}
+    for prec in ("fp64", "mixed"):
+        log(f"precision form {prec}: one step of {out['workload']}")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = rounding.RoundingReport()
+        Z = rounding.apply_and_round(O, X, chi_max=chi, cutoff=cut, report=rep, precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        reps[prec], res[prec] = rep, Z
+        out[prec] = {"seconds_per_step": round(dt, 3), "sites_per_s": round(n / dt, 3)}
+        log(f"precision form {prec}: {dt:.1f} s per step")
+    zp, zm = res["fp64"], res["mixed"]
+    n2p = float(linalg.overlap(zp, zp).item())
+    n2m = float(linalg.overlap(zm, zm).item())
+    ov = float(linalg.overlap(zm, zp).item())
+    err_s = 0.0
+    for sp, sm, kp in zip(reps["fp64"].svals, reps["mixed"].svals, reps["fp64"].kept):
+        a, b = sp.cpu().numpy()[:kp], sm.cpu().numpy()[:kp]
+        err_s = max(err_s, float(np.max(np.abs(a - b)) / a[0]))
+    out["mixed"]["speedup_vs_fp64"] = round(out["fp64"]["seconds_per_step"] / out["mixed"]["seconds_per_step"], 3)
+    out["mixed"]["error_vs_fp64_gpu"] = {"norm2_rel_err": abs(n2m - n2p) / n2p,
+                                         "overlap_rel_err": abs(ov - n2p) / n2p,
+                                         "svals_max_rel_err": err_s,
+                                         "kept_match": reps["fp64"].kept == reps["mixed"].kept}
+    out["kept_bonds_fp64"] = [min(reps["fp64"].kept), max(reps["fp64"].kept)]
+    del res, zp, zm
+    # the plain form against the oracle, reduced chain of the same d and D_w
+    xs, os_ = W.chain_inputs(8, d, 64, dw, seed=65)
+    Xs = [torch.from_numpy(x).to(dev) for x in xs]
+    Os = [torch.from_numpy(o).to(dev) for o in os_]
+    r = rounding.RoundingReport()
+    cores = rounding.apply_and_round(Os, Xs, chi_max=64, cutoff=cut, report=r)
+    ref, sv, kept, _ = ornd.apply_and_round(os_, xs, chi_max=64, cutoff=cut)
+    n2, n2r = float(linalg.overlap(cores, cores).item()), ornd.norm2(ref)
+    ovg, ovr = float(linalg.overlap(cores, Xs).item()), oref.overlap(ref, xs)
+    es = max(float(np.max(np.abs(r.svals[k - 1].cpu().numpy()[: kept[k]] - sv[k][: kept[k]])) / sv[k][0])
+             for k in range(1, 8))
+    out["fp64_vs_oracle"] = {"reduced_chain": f"N=8 d={d} chi=64 D_w={dw}", "norm2_rel_err": abs(n2 - n2r) / n2r,
+                             "overlap_rel_err": abs(ovg - ovr) / float(np.sqrt(n2r * ornd.norm2(xs))),
+                             "svals_max_rel_err": es, "kept_match": r.kept == kept[1:]}
+    return out
 
 
 def batch_form(np, torch, args):
