@@ -459,9 +459,13 @@ int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_
 /* included).                                                             */
 /* syn_prof_note_flops sets the count attached to the next grouped GEMM */
 /* or apply launch (their tables live on the device).                    */
+/* syn_prof_select(mask) limits the events to the tags whose bit is set  */
+/* (all by default; kept until changed): two event packets per launch    */
+/* slow a stream of short kernels down by microseconds each.             */
 /* ------------------------------------------------------------------ */
 #define SYN_PROF_NTAGS 16
 int syn_prof_begin(void);
+int syn_prof_select(uint32_t mask);
 int syn_prof_note_flops(double flops);
 int syn_prof_end(int64_t *launches, double *ms, double *flops, int ntags);
 /* Tuning knob of syn_jacobi_blocked_f64: rows per Jacobi block (8 or 16;  */

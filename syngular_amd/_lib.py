@@ -115,6 +115,7 @@ SIGNATURES = {
     "syn_tune_gram_block_rows": (None, [_I32]),
     "syn_chol_leaf_f64": (_I32, [_VP, _I32, _VP]),
     "syn_prof_begin": (_I32, []),
+    "syn_prof_select": (_I32, [ctypes.c_uint32]),
     "syn_prof_note_flops": (_I32, [_D]),
     "syn_prof_end": (_I32, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_double), _I32]),
@@ -142,7 +143,20 @@ PROF_TAGS = {
 }
 
 
-def prof_begin():
+def prof_begin(tags=None):
+    """Start event timing of the launches whose kernel name is in `tags`
+    (every instrumented kernel when None)."""
+    if tags is None:
+        mask = 0xFFFFFFFF
+    else:
+        names = {v: k for k, v in PROF_TAGS.items()}
+        unknown = [t for t in tags if t not in names]
+        if unknown:
+            raise ValueError(f"unknown kernel tags {unknown}; known: {sorted(names)}")
+        mask = 0
+        for t in tags:
+            mask |= 1 << names[t]
+    call("syn_prof_select", mask)
     call("syn_prof_begin")
 
 

@@ -82,28 +82,30 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
     if (tid < kNb) col[0][tid] = a[tid][0];
     __syncthreads();
     int ndrop = 0;
+    // the loop keeps the unscaled columns in lo; the square roots and
+    // divisions of L = col / sqrt(pivot) run afterwards for all columns at
+    // once, so a step's critical path holds only the pivot's reciprocal
     for (int j = 0; j < nb; ++j) {
         const double *cj = col[j & 1];
         const double d = cj[j];
         const bool kp = d > thr;
         ndrop += !kp;
-        const double sd = kp ? sqrt(d) : 0.0, rd = kp ? 1.0 / d : 0.0;
-        if (tid >= j && tid < nb) lo[tid][j] = (tid == j) ? sd : (kp ? cj[tid] / sd : 0.0);
+        const double rd = kp ? 1.0 / d : 0.0;
+        if (tid >= j && tid < nb) lo[tid][j] = cj[tid];
         if (tid == 0) keep[j] = kp;
         double ci[RB], ck[RB];
 #pragma unroll
         for (int u = 0; u < RB; ++u) ci[u] = cj[ty + TG * u] * rd;
 #pragma unroll
         for (int v = 0; v < RB; ++v) ck[v] = cj[tx + TG * v];
+        // every register entry is updated: only the trailing lower triangle
+        // (i >= k > j) is read again, and it takes valid operands only (rows
+        // and columns > j of column j); entries above the diagonal or of
+        // eliminated columns hold values that no later step reads
 #pragma unroll
-        for (int u = 0; u < RB; ++u) {
-            const int i = ty + TG * u;
+        for (int u = 0; u < RB; ++u)
 #pragma unroll
-            for (int v = 0; v < RB; ++v) {
-                const int k = tx + TG * v;
-                if (k > j && k <= i) r[u][v] = fma(-ci[u], ck[v], r[u][v]);
-            }
-        }
+            for (int v = 0; v < RB; ++v) r[u][v] = fma(-ci[u], ck[v], r[u][v]);
         // publish column j+1 (owners: tx == (j+1) % TG)
         const int jn = j + 1;
         if (jn < nb && tx == (jn % TG)) {
@@ -116,6 +118,20 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
         __syncthreads();
     }
     if (tid == 0 && ndrop) atomicAdd(dropped, ndrop);
+    // L_jj = sqrt(d_j), L_ij = col_ij / L_jj for kept pivots, zero columns for
+    // dropped ones; 1 / L~_jj for the inversion below (1 on dropped rows)
+    __shared__ double sdiag[kNb], rdiag[kNb];
+    if (tid < nb) {
+        const double sd = keep[tid] ? sqrt(lo[tid][tid]) : 0.0;
+        sdiag[tid] = sd;
+        rdiag[tid] = keep[tid] ? 1.0 / sd : 1.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < nb * nb; e += NT) {
+        const int i = e / nb, k = e % nb;
+        if (k <= i) lo[i][k] = (i == k) ? sdiag[k] : (keep[k] ? lo[i][k] / sdiag[k] : 0.0);
+    }
+    __syncthreads();
     // write L, then turn dropped rows of lo into unit rows (L~)
     for (int e = tid; e < nb * nb; e += NT) {
         const int i = e / nb, k = e % nb;
@@ -139,7 +155,7 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
         for (int v = 0; v < RB; ++v) zr[u][v] = (ty + TG * u == tx + TG * v) ? 1.0 : 0.0;
     auto publish = [&](int k) {
         if (k < nb && ty == (k % TG)) {
-            const double inv = 1.0 / lo[k][k];
+            const double inv = rdiag[k];
 #pragma unroll
             for (int u = 0; u < RB; ++u)
                 if (ty + TG * u == k) {
@@ -163,12 +179,12 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
         for (int u = 0; u < RB; ++u) lk[u] = lo[ty + TG * u][k];
 #pragma unroll
         for (int v = 0; v < RB; ++v) zc[v] = zk[tx + TG * v];
+        // rows <= k were published already and are not read again, so all
+        // rows are updated
 #pragma unroll
         for (int u = 0; u < RB; ++u)
-            if (ty + TG * u > k) {
 #pragma unroll
-                for (int v = 0; v < RB; ++v) zr[u][v] = fma(-lk[u], zc[v], zr[u][v]);
-            }
+            for (int v = 0; v < RB; ++v) zr[u][v] = fma(-lk[u], zc[v], zr[u][v]);
         publish(k + 1);
         __syncthreads();
     }

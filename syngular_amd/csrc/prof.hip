@@ -1,9 +1,13 @@
 // Live per-kernel timing with HIP events, for the benchmark's roofline figure.
 //
-// While profiling is on, every launch of an instrumented kernel is bracketed
-// by two events recorded on the launch stream, and tagged with the kernel and
-// its algorithmic flop count.  syn_prof_end synchronises the events and sums
+// While profiling is on, every launch of an instrumented kernel whose tag is
+// selected (syn_prof_select; all tags by default) is bracketed by two events
+// recorded on the launch stream, and tagged with the kernel and its
+// algorithmic flop count.  syn_prof_end synchronises the events and sums
 // elapsed time and flops per tag.  Off by default; costs nothing when off.
+// The two event packets per launch cost the stream a few microseconds each,
+// so a timing that should stay close to the uninstrumented run selects only
+// the kernels it reports.
 #include <mutex>
 #include <vector>
 
@@ -20,6 +24,7 @@ struct Rec {
 };
 std::mutex g_mu;
 bool g_on = false;
+uint32_t g_mask = 0xffffffffu;   // tags bracketed by events
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_pool;
 double g_pending_flops = 0.0;
@@ -39,6 +44,8 @@ hipEvent_t take_event() {
 
 bool prof_on() { return g_on; }
 
+bool prof_wants(int tag) { return g_on && tag >= 0 && tag < 32 && ((g_mask >> tag) & 1u); }
+
 void prof_add_work(int tag, double work) {
     if (!g_on || tag < 0 || tag >= SYN_PROF_NTAGS) return;
     std::lock_guard<std::mutex> lk(g_mu);
@@ -46,7 +53,7 @@ void prof_add_work(int tag, double work) {
 }
 
 ProfScope::ProfScope(int tag, double flops, hipStream_t s) : tag_(tag), flops_(flops), s_(s), a_(nullptr) {
-    if (!g_on) return;
+    if (!prof_wants(tag)) return;
     std::lock_guard<std::mutex> lk(g_mu);
     a_ = take_event();
     if (a_ && hipEventRecord(a_, s_) != hipSuccess) a_ = nullptr;
@@ -76,6 +83,12 @@ extern "C" int syn_prof_begin(void) {
     syn::g_pending_flops = 0.0;
     for (double &e : syn::g_extra) e = 0.0;
     syn::g_on = true;
+    return SYN_OK;
+}
+
+extern "C" int syn_prof_select(uint32_t mask) {
+    std::lock_guard<std::mutex> lk(syn::g_mu);
+    syn::g_mask = mask;
     return SYN_OK;
 }
 

@@ -24,6 +24,8 @@ enum ProfTag {
 };
 
 bool prof_on();
+// Profiling is on and the tag is selected for event timing.
+bool prof_wants(int tag);
 
 // Adds work (flops or bytes) to a tag's total without events: for kernels
 // whose work is only known on the device and summed up after the call.

@@ -58,22 +58,27 @@ __device__ __forceinline__ int tj_row(int64_t step, int64_t w, int64_t np, int64
     return (blk < nblk && r < nv) ? (int)r : -1;
 }
 
-// Plane k of round r in the round-robin order over the TP indices of a
-// pair tile, as (smaller, larger) index.
+// Plane k of the round-robin order over the TP indices of a pair tile:
+// in round r it joins a = r, b = m (k = 0) or a = (r + k) mod m,
+// b = (r - k) mod m (k > 0), m = TP - 1.  From one round to the next both
+// move up by one modulo m (b stays at m for k = 0), so a thread steps its
+// planes along instead of dividing by m every round.
 template <int TP>
-__device__ __forceinline__ void plane_rr(int r, int k, int &p, int &q) {
-    constexpr int m = TP - 1;
+struct RrPlane {
     int a, b;
-    if (k == 0) {
-        a = r;
-        b = m;
-    } else {
-        a = (r + k) % m;
-        b = (r - k + m) % m;
+    __device__ __forceinline__ void start(int k) {
+        constexpr int m = TP - 1;
+        a = k;
+        b = k == 0 ? m : m - k;
     }
-    p = a < b ? a : b;
-    q = a < b ? b : a;
-}
+    __device__ __forceinline__ void next() {
+        constexpr int m = TP - 1;
+        a = a + 1 == m ? 0 : a + 1;
+        if (b != m) b = b + 1 == m ? 0 : b + 1;
+    }
+    __device__ __forceinline__ int p() const { return a < b ? a : b; }
+    __device__ __forceinline__ int q() const { return a < b ? b : a; }
+};
 
 // Rotation test of plane (p, q): |G_pq| > max(tol sqrt(|G_pp G_qq|), floor),
 // in squares (no square root on the critical path of a round).
@@ -88,17 +93,53 @@ __device__ __forceinline__ bool rotates(double apq, double app, double aqq, doub
 // c = 1 / sqrt(1 + t^2), s = c t.  The scaled form needs one division and one
 // square root for t; operands far from 1 (|d|, |e| outside 1e-150 .. 1e150)
 // take the tau form, which cannot overflow.
+#ifndef SYN_TJ_FAST_ROT
+#define SYN_TJ_FAST_ROT 1
+#endif
+// 1 / x and 1 / sqrt(x) for normal positive x from the hardware estimates
+// (v_rcp_f64, v_rsq_f64) and two Newton-Raphson steps each: r <- r + r (1 - x r)
+// and y <- y + y (1 - x y^2) / 2, each step squaring the relative error, to
+// within a unit or two in the last place.  The rotation needs c^2 + s^2 = 1 to
+// that level, not correctly rounded c and s; the steps are shorter than the
+// IEEE division and square root sequences on a round's critical path.
+__device__ __forceinline__ double recip_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double e = fma(-x * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
 __device__ __forceinline__ void rotation(double apq, double app, double aqq, double &c, double &s) {
     const double d = aqq - app, e = 2.0 * apq;
     const double ad = fabs(d), ae = fabs(e), m = fmax(ad, ae);
     double t;
     if (m < 1e150 && m > 1e-150) {
+#if SYN_TJ_FAST_ROT
+        const double h2 = fma(d, d, e * e);
+        t = copysign(ae, d * e) * recip_nr(ad + h2 * rsqrt_nr(h2));
+#else
         t = copysign(ae, d * e) / (ad + sqrt(fma(d, d, e * e)));
+#endif
     } else {
         const double tau = d / e, at = fabs(tau);
         t = copysign(1.0, tau) / (at < 1e150 ? at + sqrt(fma(at, at, 1.0)) : 2.0 * at);
     }
+#if SYN_TJ_FAST_ROT
+    c = rsqrt_nr(fma(t, t, 1.0));
+#else
     c = 1.0 / sqrt(fma(t, t, 1.0));
+#endif
     s = c * t;
 }
 
@@ -189,10 +230,19 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
         if (tid == 0) any_rot = 0;
         __syncthreads();
+        RrPlane<TP> pa, pb, pw2[2];
+        pa.start(ka);
+        pb.start(kb);
+        pw2[0].start(wk);
+        pw2[1].start(wk + TP / 4);
         for (int r = 0; r < TP - 1; ++r) {
-            int p, q, u, v;
-            plane_rr<TP>(r, ka, p, q);
-            plane_rr<TP>(r, kb, u, v);
+            if (r) {
+                pa.next();
+                pb.next();
+                pw2[0].next();
+                pw2[1].next();
+            }
+            const int p = pa.p(), q = pa.q(), u = pb.p(), v = pb.q();
             // this round's operands were final at the last barrier and phase 1
             // only reads S: load them now, so their latency overlaps phase 1
             const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
@@ -200,7 +250,8 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
             double wx[2], wy[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                plane_rr<TP>(r, wk + h * (TP / 4), wp[h], wq[h]);
+                wp[h] = pw2[h].p();
+                wq[h] = pw2[h].q();
                 wx[h] = W[wi][wp[h]];
                 wy[h] = W[wi][wq[h]];
             }
@@ -550,7 +601,7 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items, 64), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 0);
     hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items, 64), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 1);
     SYN_LAUNCH_CHECK("gram permute");
-    if (prof_on()) {
+    if (prof_wants(kTagGramUpdate)) {
         unsigned long long hb = 0;
         if (int rc = hip_check(hipMemcpyAsync(&hb, dbytes, sizeof(hb), hipMemcpyDeviceToHost, st), "gram bytes"))
             return rc;

@@ -118,10 +118,11 @@ def gemm_many(problems, tile=None, mixed=False):
 _profiling = [False]
 
 
-def profile_begin():
-    """Start live per-kernel event timing inside the library."""
+def profile_begin(tags=None):
+    """Start live per-kernel event timing inside the library, of the kernels
+    named in `tags` (all instrumented kernels when None)."""
     _profiling[0] = True
-    _lib.prof_begin()
+    _lib.prof_begin(tags)
 
 
 def profile_end():
