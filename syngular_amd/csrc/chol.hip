@@ -24,6 +24,20 @@ namespace syn {
 
 constexpr int kNb = 64;
 
+// compile-time integer and an unrolled loop over 0 .. N-1 handing f the
+// index as a constant
+template <int V>
+struct IntC {
+    static constexpr int value = V;
+};
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(IntC<I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+
 __global__ __launch_bounds__(256) void maxdiag_kernel(const double *g, int64_t n, int64_t ld, double *out,
                                                       int *dropped) {
     __shared__ double red[4];
@@ -84,20 +98,23 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
     int ndrop = 0;
     // the loop keeps the unscaled columns in lo; the square roots and
     // divisions of L = col / sqrt(pivot) run afterwards for all columns at
-    // once, so a step's critical path holds only the pivot's reciprocal
-    for (int j = 0; j < nb; ++j) {
+    // once, so a step's critical path holds only the pivot's reciprocal.
+    // Step j with VN = the register column (tx + TG * VN) of column j + 1:
+    auto col_step = [&](int j, auto vn_c) {
+        constexpr int VN = decltype(vn_c)::value;
         const double *cj = col[j & 1];
+        // every operand of the step is read before the pivot's reciprocal
+        double ci[RB], ck[RB];
         const double d = cj[j];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) ci[u] = cj[ty + TG * u];
+#pragma unroll
+        for (int v = 0; v < RB; ++v) ck[v] = cj[tx + TG * v];
         const bool kp = d > thr;
         ndrop += !kp;
         const double rd = kp ? 1.0 / d : 0.0;
-        if (tid >= j && tid < nb) lo[tid][j] = cj[tid];
-        if (tid == 0) keep[j] = kp;
-        double ci[RB], ck[RB];
 #pragma unroll
-        for (int u = 0; u < RB; ++u) ci[u] = cj[ty + TG * u] * rd;
-#pragma unroll
-        for (int v = 0; v < RB; ++v) ck[v] = cj[tx + TG * v];
+        for (int u = 0; u < RB; ++u) ci[u] *= rd;
         // every register entry is updated: only the trailing lower triangle
         // (i >= k > j) is read again, and it takes valid operands only (rows
         // and columns > j of column j); entries above the diagonal or of
@@ -106,17 +123,34 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
         for (int u = 0; u < RB; ++u)
 #pragma unroll
             for (int v = 0; v < RB; ++v) r[u][v] = fma(-ci[u], ck[v], r[u][v]);
-        // publish column j+1 (owners: tx == (j+1) % TG)
-        const int jn = j + 1;
-        if (jn < nb && tx == (jn % TG)) {
+        // publish column j+1 (owners: tx == (j+1) % TG, register column VN)
+        if constexpr (VN < RB) {
+            const int jn = j + 1;
+            if (jn < nb && tx == (jn % TG)) {
 #pragma unroll
-            for (int u = 0; u < RB; ++u)
-#pragma unroll
-                for (int v = 0; v < RB; ++v)
-                    if (tx + TG * v == jn) col[jn & 1][ty + TG * u] = r[u][v];
+                for (int u = 0; u < RB; ++u) col[jn & 1][ty + TG * u] = r[u][VN];
+            }
         }
+        if (tid >= j && tid < nb) lo[tid][j] = cj[tid];
+        if (tid == 0) keep[j] = kp;
         __syncthreads();
-    }
+    };
+    // columns in blocks of TG: column j + 1 sits in register column j / TG
+    // except after the last column of a block, so both are compile-time
+    // constants (no selects over the register block)
+    static_for<RB>([&](auto jb_c) {
+        constexpr int JB = decltype(jb_c)::value;
+        // kept rolled: each launch runs the code once, so an unrolled
+        // column sequence would be fetched from L2 instruction by instruction
+#pragma unroll 1
+        for (int jj = 0; jj + 1 < TG; ++jj) {
+            const int j = JB * TG + jj;
+            if (j >= nb) return;   // nb is uniform over the work-group
+            col_step(j, IntC<JB>{});
+        }
+        const int j = JB * TG + TG - 1;
+        if (j < nb) col_step(j, IntC<JB + 1>{});
+    });
     if (tid == 0 && ndrop) atomicAdd(dropped, ndrop);
     // L_jj = sqrt(d_j), L_ij = col_ij / L_jj for kept pivots, zero columns for
     // dropped ones; 1 / L~_jj for the inversion below (1 on dropped rows)

@@ -37,6 +37,7 @@ constexpr int kTUpdThreads = 256;    // four waves per tile update
 constexpr int kTCol = 64;            // columns of an rt slab
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void tj_pair(int64_t step, int64_t w, int64_t np, int64_t &a, int64_t &b) {
     if (w == 0) {
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
     constexpr int TP = 2 * TB, NT = TB * TB;
     __shared__ double S[TP][TP + 1];
     __shared__ double W[TP][TP + 1];
-    __shared__ double cs[TP / 2][2];
+    __shared__ v2d cs[TP / 2];   // (c, s) of each plane of the round
     __shared__ int rows[TP];
     __shared__ int do_rot, any_rot;
     const int64_t bid = blockIdx.x;
@@ -262,12 +263,14 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
                     rotation(apq, app, aqq, c, sn);
                     any_rot = 1;
                 }
-                cs[tid][0] = c;
-                cs[tid][1] = sn;
+                cs[tid] = v2d{c, sn};
             }
             __syncthreads();
+            // every rotation this thread applies, read in one batch of 16-byte
+            // loads (one LDS round trip instead of one per use)
+            const v2d csa = cs[ka], csb = cs[kb], csw[2] = {cs[wk], cs[wk + TP / 4]};
             {
-                const double ca = cs[ka][0], sa = cs[ka][1], cb = cs[kb][0], sb = cs[kb][1];
+                const double ca = csa.x, sa = csa.y, cb = csb.x, sb = csb.y;
                 if (sa != 0.0 || sb != 0.0) {
                     const double rpu = ca * spu - sa * squ, rpv = ca * spv - sa * sqv;
                     const double rqu = sa * spu + ca * squ, rqv = sa * spv + ca * sqv;
@@ -283,10 +286,8 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
             }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {  // W <- W J (columns of plane wk + h TP/4)
-                const int k = wk + h * (TP / 4);
-                const double sn = cs[k][1];
+                const double c = csw[h].x, sn = csw[h].y;
                 if (sn != 0.0) {
-                    const double c = cs[k][0];
                     W[wi][wp[h]] = c * wx[h] - sn * wy[h];
                     W[wi][wq[h]] = sn * wx[h] + c * wy[h];
                 }
