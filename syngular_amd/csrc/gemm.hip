@@ -88,7 +88,39 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
     const int wm0 = (w / (BN / WN)) * WM, wn0 = (w % (BN / WN)) * WN;
 
     double ra[EA], rb[EB];
+    // Interior tiles (all BM rows, BN columns and BK steps of the K tile in
+    // range) load through buffer instructions: element s of a thread sits at
+    // base(k0) + voff + s * stride, with voff one VGPR computed once, s *
+    // stride a scalar offset and the K tile's base in the scalar descriptor,
+    // so the steady state spends no vector instructions on addresses or
+    // bounds (the guarded form below costs about two per MFMA).  Offsets are
+    // 32-bit: the path is taken for leading dimensions below 2^20 doubles,
+    // which keeps every offset of a tile under 2^31.
+    static_assert(NT % BK == 0 && NT % BM == 0 && NT % BN == 0, "thread/tile shapes");
+    constexpr int kStepA = ta ? NT / BM : NT / BK;   // rows of the stored A per element step
+    constexpr int kStepB = tb ? NT / BK : NT / BN;   // rows of the stored B per element step
+    const bool ld_small = lda < (1 << 20) && ldb < (1 << 20);
+    const int voff_a = ta ? ((tid / BM) * (int)lda + tid % BM) * 8 : ((tid / BK) * (int)lda + tid % BK) * 8;
+    const int voff_b = tb ? ((tid / BK) * (int)ldb + tid % BK) * 8 : ((tid / BN) * (int)ldb + tid % BN) * 8;
+    const int sstr_a = kStepA * (int)lda * 8, sstr_b = kStepB * (int)ldb * 8;
+    auto load_fast = [&](int64_t k0) {
+        const double *pa = ta ? A + k0 * lda + row0 : A + row0 * lda + k0;
+        const double *pb = tb ? B + col0 * ldb + k0 : B + k0 * ldb + col0;
+        const __amdgpu_buffer_rsrc_t sa = __builtin_amdgcn_make_buffer_rsrc((void *)pa, (short)0, 0x7fffffff, 0x00020000);
+        const __amdgpu_buffer_rsrc_t sb = __builtin_amdgcn_make_buffer_rsrc((void *)pb, (short)0, 0x7fffffff, 0x00020000);
+#pragma unroll
+        for (int s = 0; s < EA; ++s)
+            ra[s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(sa, voff_a, s * sstr_a, 0));
+#pragma unroll
+        for (int s = 0; s < EB; ++s)
+            rb[s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(sb, voff_b, s * sstr_b, 0));
+    };
+    const bool tile_in = row0 + BM <= m && col0 + BN <= n && ld_small;
     auto load = [&](int64_t k0) {
+        if (tile_in && k0 + BK <= k) {
+            load_fast(k0);
+            return;
+        }
 #pragma unroll
         for (int s = 0; s < EA; ++s) {
             int e = tid + NT * s;
@@ -112,21 +144,20 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
                 rb[s] = (gj < n && gk < k) ? B[gk * ldb + gj] : 0.0;
         }
     };
+    // Element s of a thread lands at a fixed LDS base plus a compile-time
+    // multiple of s (the same (row, k) map as the loads: e = tid + NT s), so
+    // the stores take immediate offsets instead of per-element arithmetic.
+    constexpr int kLdsStepA = ta ? (NT / BM) * (BM + 1) : NT / BK;
+    constexpr int kLdsStepB = tb ? NT / BK : (NT / BN) * (BN + 1);
+    const int lds_a = ta ? (tid / BM) * (BM + 1) + tid % BM : (tid % BK) * (BM + 1) + tid / BK;
+    const int lds_b = tb ? (tid % BK) * (BN + 1) + tid / BK : (tid / BN) * (BN + 1) + tid % BN;
     auto store = [&](int buf) {
+        CT *sa = &As[buf][0][0] + lds_a;
+        CT *sb = &Bs[buf][0][0] + lds_b;
 #pragma unroll
-        for (int s = 0; s < EA; ++s) {
-            int e = tid + NT * s;
-            int r, c;
-            if constexpr (!ta) { r = e / BK; c = e % BK; } else { c = e / BM; r = e % BM; }
-            As[buf][c][r] = (CT)ra[s];
-        }
+        for (int s = 0; s < EA; ++s) sa[s * kLdsStepA] = (CT)ra[s];
 #pragma unroll
-        for (int s = 0; s < EB; ++s) {
-            int e = tid + NT * s;
-            int j, c;
-            if constexpr (!tb) { c = e / BN; j = e % BN; } else { j = e / BK; c = e % BK; }
-            Bs[buf][c][j] = (CT)rb[s];
-        }
+        for (int s = 0; s < EB; ++s) sb[s * kLdsStepB] = (CT)rb[s];
     };
 
     typedef Mfma<CT> M;
