@@ -175,17 +175,23 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < nk) load(kmin + (kt + 1) * BK);
-#pragma unroll 1
+        // the operands of step kk + 1 are read from LDS while the MFMAs of
+        // step kk run
+        CT a[2][TI], b[2][TJ];
+        auto read_ops = [&](int kk, int h) {
+#pragma unroll
+            for (int i = 0; i < TI; ++i) a[h][i] = As[cur][kk * 4 + lk][wm0 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) b[h][j] = Bs[cur][kk * 4 + lk][wn0 + j * 16 + li];
+        };
+        read_ops(0, 0);
+#pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
-            CT a[TI], b[TJ];
-#pragma unroll
-            for (int i = 0; i < TI; ++i) a[i] = As[cur][kk * 4 + lk][wm0 + i * 16 + li];
-#pragma unroll
-            for (int j = 0; j < TJ; ++j) b[j] = Bs[cur][kk * 4 + lk][wn0 + j * 16 + li];
+            if (kk + 1 < BK / 4) read_ops(kk + 1, (kk + 1) & 1);
 #pragma unroll
             for (int i = 0; i < TI; ++i)
 #pragma unroll
-                for (int j = 0; j < TJ; ++j) acc[i][j] = M::mac(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < TJ; ++j) acc[i][j] = M::mac(a[kk & 1][i], b[kk & 1][j], acc[i][j]);
         }
         if (kt + 1 < nk) store(cur ^ 1);
         __syncthreads();
