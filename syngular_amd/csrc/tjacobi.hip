@@ -244,8 +244,17 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
                 pw2[1].next();
             }
             const int p = pa.p(), q = pa.q(), u = pb.p(), v = pb.q();
-            // this round's operands were final at the last barrier and phase 1
-            // only reads S: load them now, so their latency overlaps phase 1
+            // the rotation's operands are read first (their wave waits for
+            // them alone); this round's update operands were final at the last
+            // barrier and phase 1 only reads S, so they are loaded next and
+            // their latency overlaps the rotation
+            const bool rot_thread = tid < TP / 2;   // here ka == 0 and kb == tid: plane tid is (u, v)
+            double apq = 0.0, app = 0.0, aqq = 0.0;
+            if (rot_thread) {
+                apq = S[u][v];
+                app = S[u][u];
+                aqq = S[v][v];
+            }
             const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
             int wp[2], wq[2];
             double wx[2], wy[2];
@@ -256,8 +265,7 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
                 wx[h] = W[wi][wp[h]];
                 wy[h] = W[wi][wq[h]];
             }
-            if (tid < TP / 2) {   // here ka == 0 and kb == tid: plane tid is (u, v)
-                const double apq = S[u][v], app = S[u][u], aqq = S[v][v];
+            if (rot_thread) {
                 double c = 1.0, sn = 0.0;
                 if (rotates(apq, app, aqq, tol, floor_abs)) {
                     rotation(apq, app, aqq, c, sn);

@@ -658,6 +658,12 @@ def cholqr(A, tolrel=None, mixed=False):
 # Second CholeskyQR pass: when the first pass leaves max |Q1^T Q1 - I| below
 # this, the Cholesky of I + E is taken to first order (error O(n 1e-18)).
 CHOLQR_REFINE_THRESH = 1e-9
+# Mixed form: its products round to FP32, so max|Q^T Q - I| after either
+# pass is ~1e-7 whatever factor the second pass uses.  The first-order factor
+# is exact to O(n max|E|^2), at most 4e-9 relative for n = 4096 and
+# max|E| <= 1e-6: below the FP32 product error, so the full Cholesky of the
+# second pass buys nothing there.
+CHOLQR_REFINE_THRESH_MIXED = 1e-6
 
 
 def cholqr_refine(Q1, tolrel=None, thresh=CHOLQR_REFINE_THRESH, mixed=False):
@@ -684,7 +690,8 @@ def cholqr2(A, tolrel=None, mixed=False):
     second through cholqr_refine).  Returns (Q (m x n), R (n x n upper))
     with A = Q R.  mixed: the Q products on the FP32 matrix cores."""
     Q1, L1, _ = cholqr(A, tolrel, mixed)
-    Q, L2, _ = cholqr_refine(Q1, tolrel, mixed=mixed)
+    Q, L2, _ = cholqr_refine(Q1, tolrel, thresh=CHOLQR_REFINE_THRESH_MIXED if mixed else CHOLQR_REFINE_THRESH,
+                             mixed=mixed)
     R = gemm(L2.t(), L1.t(), tri=4 | 8)  # R = R2 R1 = L2^T L1^T, both upper
     return Q, R
 
