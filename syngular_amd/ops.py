@@ -685,15 +685,76 @@ def cholqr_refine(Q1, tolrel=None, thresh=CHOLQR_REFINE_THRESH, mixed=False):
     return Q, L, bool(full.value)
 
 
+# Adaptive second pass (cholqr2_adaptive): the first pass's loss of
+# orthogonality E = Q1^T Q1 - I is estimated as ||Q1^T Q1 X - X||_F / sqrt(p)
+# over p Gaussian probe columns X (an unbiased estimate of ||E||_F^2 in
+# square; the chance that it falls below a tenth of ||E||_F is under 1e-7
+# for p = 8), at the cost of two skinny products instead of a Gram matrix.
+REORTH_PROBES = 8
+_probe_cache = {}
+
+
+def orthogonality_estimate(Q):
+    """Randomised estimate of ||Q^T Q - I||_F for a row-major m x n Q, from
+    REORTH_PROBES fixed-seed Gaussian columns (deterministic for a given Q)."""
+    m, n = Q.shape
+    key = (n, Q.device)
+    if key not in _probe_cache:
+        g = torch.Generator(device=Q.device)
+        g.manual_seed(0x5EED)
+        _probe_cache[key] = torch.randn(n, REORTH_PROBES, dtype=F64, device=Q.device, generator=g)
+    X = _probe_cache[key]
+    Y = _gemm_ksplit(Q, X)                              # Q X         (m x p)
+    Z = _gemm_ksplit(Q.t(), Y, X.clone(), -1.0)          # Q^T Q X - X (n x p)
+    return float(np.linalg.norm(Z.cpu().numpy())) / REORTH_PROBES ** 0.5
+
+
+def _gemm_ksplit(A, B, C=None, beta=0.0, chunks=16):
+    """C = A @ B + beta C for a skinny B (few columns) and a long K: the K
+    range is cut into `chunks` slices computed as one grouped launch into a
+    workspace, then summed by a one-row GEMM (a plain launch would give each
+    of its few output tiles the whole K loop on one CU)."""
+    m, k = A.shape
+    p = B.shape[1]
+    kc = (k + chunks - 1) // chunks
+    parts = [(j * kc, min(k, (j + 1) * kc)) for j in range(chunks) if j * kc < k]
+    W = torch.empty((len(parts), m, p), dtype=F64, device=A.device)
+    gemm_many([(A[:, a:b], B[a:b, :], W[j], 1.0, 0.0) for j, (a, b) in enumerate(parts)], tile=0)
+    if C is None:
+        C = torch.empty((m, p), dtype=F64, device=A.device)
+        beta = 0.0
+    ones = torch.ones((1, len(parts)), dtype=F64, device=A.device)
+    gemm(ones, W.view(len(parts), m * p), C.view(1, m * p), 1.0, beta)
+    return C
+
+
 def cholqr2(A, tolrel=None, mixed=False):
     """Reduced QR of a tall row-major matrix by two CholeskyQR passes (the
     second through cholqr_refine).  Returns (Q (m x n), R (n x n upper))
     with A = Q R.  mixed: the Q products on the FP32 matrix cores."""
     Q1, L1, _ = cholqr(A, tolrel, mixed)
+    return _second_pass(Q1, L1, tolrel, mixed)
+
+
+def _second_pass(Q1, L1, tolrel, mixed):
     Q, L2, _ = cholqr_refine(Q1, tolrel, thresh=CHOLQR_REFINE_THRESH_MIXED if mixed else CHOLQR_REFINE_THRESH,
                              mixed=mixed)
     R = gemm(L2.t(), L1.t(), tri=4 | 8)  # R = R2 R1 = L2^T L1^T, both upper
     return Q, R
+
+
+def cholqr2_adaptive(A, allowance, tolrel=None):
+    """CholeskyQR whose second pass runs only where it is needed: after the
+    first pass, the loss of orthogonality ||Q1^T Q1 - I||_F is estimated
+    (orthogonality_estimate); at or below `allowance` the first pass is kept.
+    Returns (Q, R, spent) with A = Q R and spent the estimate of the kept
+    first pass (0.0 when the second pass ran).  FP64 only."""
+    Q1, L1, _ = cholqr(A, tolrel)
+    est = orthogonality_estimate(Q1)
+    if est <= allowance:
+        return Q1, L1.t(), est
+    Q, R = _second_pass(Q1, L1, tolrel, False)
+    return Q, R, 0.0
 
 
 def jacobi_tol(length):

@@ -439,6 +439,74 @@ def test_cholqr_refine_paths(cond, expect_full):
     np.testing.assert_allclose(Q2.cpu().numpy() @ R2.cpu().numpy(), A, atol=1e-13)
 
 
+def test_orthogonality_estimate():
+    """The randomised ||Q^T Q - I||_F estimate against the exact value, for a
+    Q with a planted loss of orthogonality Q = U (I + F/2), F symmetric (so
+    Q^T Q - I = F + F^2/4)."""
+    from syngular_amd import ops
+
+    case("orthogonality estimate")
+    rng = np.random.default_rng(5)
+    m, n = 1200, 400
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    for scale in (1e-12, 1e-9, 1e-6):
+        F = rng.standard_normal((n, n)) * scale
+        F = (F + F.T) / 2
+        Q = U @ (np.eye(n) + F / 2)
+        exact = np.linalg.norm(Q.T @ Q - np.eye(n))
+        est = ops.orthogonality_estimate(T(Q))
+        assert exact / 3 < est < 3 * exact, (scale, exact, est)
+        assert est == ops.orthogonality_estimate(T(Q))   # fixed probes: deterministic
+
+
+@pytest.mark.parametrize("cond,expect_skip", [(10.0, True), (1e6, False)])
+def test_cholqr2_adaptive(cond, expect_skip):
+    """Adaptive CholeskyQR: a well-conditioned matrix keeps its first pass
+    (within the allowance), an ill-conditioned one runs the second; A = Q R
+    both ways and Q^T Q - I stays within what was spent."""
+    from syngular_amd import ops
+
+    case(f"cholqr2 adaptive cond={cond}")
+    rng = np.random.default_rng(32)
+    m, n = 900, 300
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (U * np.logspace(0, -np.log10(cond), n)) @ V.T
+    Q, R, spent = ops.cholqr2_adaptive(T(A), 1e-10)
+    Qh, Rh = Q.cpu().numpy(), R.cpu().numpy()
+    assert (spent > 0.0) == expect_skip
+    np.testing.assert_allclose(Qh @ Rh, A, atol=1e-13)
+    assert np.allclose(np.triu(Rh), Rh)
+    E = np.linalg.norm(Qh.T @ Qh - np.eye(n))
+    assert E <= (3 * spent if expect_skip else 1e-13 * n)
+
+
+def test_round_budgeted_left_sweep():
+    """TT-rounding with the budgeted left sweep (its default) against both
+    CholeskyQR passes at every site: retained singular values within 1e-10
+    relative, and against the oracle."""
+    from syngular_amd import rounding
+    from syngular_amd import workloads as W
+
+    case("rounding budgeted left sweep N=12 chi=64 D_w=8")
+    xs, os_ = W.chain_inputs(12, 2, 64, 8, seed=11)
+    X = [T(x) for x in xs]
+    O = [T(o) for o in os_]
+    from syngular_amd import ops
+    prod = ops.apply_many(list(zip(O, X)))
+    r1, r2 = rounding.RoundingReport(), rounding.RoundingReport()
+    rounding.right_svd_sweep(rounding.left_orthonormalize(prod, reorth_budget=rounding.ROUND_REORTH_BUDGET), 64,
+                             0.0, r1)
+    rounding.right_svd_sweep(rounding.left_orthonormalize(prod, reorth_budget=None), 64, 0.0, r2)
+    ref, sv, kept, _ = ornd.apply_and_round(os_, xs, chi_max=64)
+    assert r1.kept == r2.kept == kept[1:]
+    for k in range(11):
+        a, b = r1.svals[k].cpu().numpy(), r2.svals[k].cpu().numpy()
+        c = kept[k + 1]
+        assert np.max(np.abs(a[:c] - b[:c])) <= 1e-10 * b[0]
+        assert np.max(np.abs(a[:c] - sv[k + 1][:c])) <= 1e-10 * sv[k + 1][0]
+
+
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("m,n,k", [(37, 50, 19), (300, 260, 513)])
 def test_gemm_mixed(ta, tb, m, n, k):
