@@ -63,8 +63,8 @@ def parse():
     ap.add_argument("--bond-mpo", type=int, default=16)
     ap.add_argument("--phys", type=int, default=2)
     ap.add_argument("--batch-jobs", type=int, default=4096)
-    ap.add_argument("--serve-jobs", type=int, default=12)
-    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="fp64",
                     help="arithmetic of the served form (the per-chain comparison is always plain FP64); "
                          "the mixed form gains no speed on this Jacobi-bound workload and its FP32 "
