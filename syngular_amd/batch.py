@@ -312,6 +312,34 @@ def cholqr2_table(A, m, n, lda, hold, mixed=False):
     return Q, R
 
 
+# Orthogonality budget of the left sweep in TT-rounding (plain FP64), per
+# chain: the same bound as the single-chain rounding (sum_k ||Q_k^T Q_k - I||_F
+# <= 1e-10 keeps every singular value within 5e-11 relative).
+REORTH_BUDGET = 1e-10
+
+
+def orth_estimate_table(Q, m, n, hold):
+    """Randomised ||Q_i^T Q_i - I||_F of many row-major m x n Q (ld n), from
+    ops.REORTH_PROBES fixed-seed Gaussian columns (the first n_i rows of one
+    shared probe matrix): ||Q^T Q X - X||_F / sqrt(p), the products on the
+    device, the difference and norm of the small n x p results on the host."""
+    p = ops.REORTH_PROBES
+    nmax = int(np.max(n))
+    xh = np.random.default_rng(0x5EED).standard_normal((nmax, p))
+    X = _empty(nmax * p)
+    X.copy_(torch.from_numpy(xh.reshape(-1)))
+    xp = np.uint64(X.data_ptr())
+    yt, Y = arena(m * p)
+    wt, W = arena(n * p)
+    gemm_table(Q, xp, Y, m, p, n, n, p, p)                 # Y = Q X
+    gemm_table(Q, Y, W, n, p, m, n, p, p, ta=1)            # W = Q^T Y
+    hold += [yt, wt, X]
+    off = (W - W[0]) // 8
+    wh = wt.cpu().numpy()
+    return np.array([np.linalg.norm(wh[o:o + int(ni) * p].reshape(int(ni), p) - xh[:int(ni)]) / p ** 0.5
+                     for o, ni in zip(off.astype(np.int64), n)])
+
+
 def jacobi_gram_blocked_table(x, g, rt, s):
     """Two-sided block Jacobi of many symmetric g x g Gram matrices (g > 64),
     one batched call; x is diagonalised in place."""
@@ -496,13 +524,18 @@ class ChainBatch:
         return ChainBatch(self.n, bond, phys, ptr, [t, ops_batch.hold, self.hold])
 
     # ---- left sweep --------------------------------------------------------
-    def left_orthonormalize(self, mixed=False):
+    def left_orthonormalize(self, mixed=False, reorth_budget=None):
         """Left QR sweep over all chains in lock-step (in place on the tables;
         the input arenas are not written).  mixed: CholeskyQR's Q and the
-        absorb products on the FP32 matrix cores."""
+        absorb products on the FP32 matrix cores.  reorth_budget (FP64 only):
+        per chain, the total estimated loss of orthogonality
+        sum_k ||Q_k^T Q_k - I||_F the sweep may keep; CholeskyQR skips its
+        second pass on a problem while its chain stays within it.  None runs
+        both passes everywhere."""
         J, nmax = self.ptr.shape
         ptr, bond = self.ptr.copy(), self.bond.copy()
         hold = list(self.hold)
+        spent = np.zeros(J)
         for k in range(nmax - 1):
             a = np.nonzero(self.n - 1 > k)[0]
             if len(a) == 0:
@@ -521,6 +554,7 @@ class ChainBatch:
                 eye_table(q[wi], m[wi])
             S = ptr[a, k].copy()          # wide: S = L itself (m x r, ld r)
             lds = r.copy()
+            sta = np.zeros(len(a), np.int64)   # 1: S holds L1 and the absorb takes L1^T
             hh = tall & (r < CHOLQR_MIN_COLS)
             if hh.any():
                 # Householder, one work-group per matrix; S = Q^T L  (r x r)
@@ -536,15 +570,36 @@ class ChainBatch:
             if cq.any():
                 # CholeskyQR2 (grouped GEMMs + batched Cholesky leaves)
                 ti = cq.nonzero()[0]
-                Qc, Rc = cholqr2_table(ptr[a[ti], k], m[ti], r[ti], r[ti], hold, mixed)
-                q[ti] = Qc
-                S[ti] = Rc
+                if mixed or reorth_budget is None:
+                    Qc, Rc = cholqr2_table(ptr[a[ti], k], m[ti], r[ti], r[ti], hold, mixed)
+                    q[ti] = Qc
+                    S[ti] = Rc
+                else:
+                    # first pass everywhere; the second only where the chain's
+                    # orthogonality budget cannot take the first pass's loss
+                    mt, rt = m[ti], r[ti]
+                    Q1, L1 = cholqr_table(ptr[a[ti], k], mt, rt, rt, hold)
+                    est = orth_estimate_table(Q1, mt, rt, hold)
+                    keep = est <= reorth_budget - spent[a[ti]]
+                    spent[a[ti][keep]] += est[keep]
+                    q[ti[keep]] = Q1[keep]
+                    S[ti[keep]] = L1[keep]
+                    sta[ti[keep]] = 1
+                    redo = ~keep
+                    if redo.any():
+                        mr, rr = mt[redo], rt[redo]
+                        Q2, L2 = cholqr_table(Q1[redo], mr, rr, rr, hold)
+                        rt_, R = arena(rr * rr)
+                        gemm_table(L2, L1[redo], R, rr, rr, rr, rr, rr, rr, ta=1, tb=1, tri=4 | 8)  # R2 R1
+                        hold.append(rt_)
+                        q[ti[redo]] = Q2
+                        S[ti[redo]] = R
             # absorb: next = S @ core_{k+1}  (new_r x d' b'')
             dn, bn = self.phys[a, k + 1], bond[a, k + 2]
             ncols = dn * bn
             nt, npk = arena(new_r * ncols)
             # S from a QR is upper triangular (tri 4); S = L of a wide unfolding is not
-            gemm_table(S, ptr[a, k + 1], npk, new_r, ncols, r, lds, ncols, ncols, tri=np.where(wide, 0, 4),
+            gemm_table(S, ptr[a, k + 1], npk, new_r, ncols, r, lds, ncols, ncols, ta=sta, tri=np.where(wide, 0, 4),
                        mixed=mixed)
             hold += [qt, nt]
             ptr[a, k] = q
@@ -636,7 +691,8 @@ class ChainBatch:
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
         mixed = precision == "mixed"
-        out = self.left_orthonormalize(mixed).right_svd_sweep(chi_max, cutoff, records, mixed)
+        out = self.left_orthonormalize(mixed, None if mixed else REORTH_BUDGET).right_svd_sweep(
+            chi_max, cutoff, records, mixed)
         if compact:
             out.compact()
         return out

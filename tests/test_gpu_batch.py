@@ -211,3 +211,38 @@ def test_served_lanes_alone_equals_in_mix():
         da, dm = oref.dense_mps(a.cores), oref.dense_mps(m.cores)
         assert np.abs(da - dm).max() <= 1e-12
         assert abs(m.norm2 + m.discarded - 1.0) <= 1e-10
+
+
+def test_batched_orthogonality_budget():
+    """The batched estimate of ||Q^T Q - I||_F against the exact value for
+    planted losses, and the budgeted left sweep against both CholeskyQR
+    passes everywhere: per-bond singular values within 1e-10 relative."""
+    rng = np.random.default_rng(77)
+    qs, exact = [], []
+    for (m, n, scale) in ((300, 120, 1e-12), (500, 200, 1e-9), (260, 130, 1e-6)):
+        U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+        F = rng.standard_normal((n, n)) * scale
+        F = (F + F.T) / 2
+        Q = U @ (np.eye(n) + F / 2)
+        qs.append(torch.from_numpy(Q).to(ops.device()))
+        exact.append(np.linalg.norm(Q.T @ Q - np.eye(n)))
+    hold = []
+    est = B.orth_estimate_table(np.array([q.data_ptr() for q in qs], np.uint64),
+                                np.array([q.shape[0] for q in qs]), np.array([q.shape[1] for q in qs]), hold)
+    torch.cuda.synchronize()
+    for e, x in zip(est, exact):
+        assert x / 3 < e < 3 * x, (e, x)
+
+    jobs = _mixed_jobs(3)
+    X = B.ChainBatch.from_numpy([j[0] for j in jobs])
+    O = B.OperatorBatch.from_numpy([j[1] for j in jobs])
+    chi = np.array([j[2] for j in jobs])
+    P = X.apply(O)
+    ra, rb = [], []
+    A = P.left_orthonormalize(False, B.REORTH_BUDGET).right_svd_sweep(chi, 0.0, ra, False)
+    Bb = P.left_orthonormalize(False, None).right_svd_sweep(chi, 0.0, rb, False)
+    for q, (xs, os_, c) in enumerate(jobs):
+        ref, sv, kept, disc = ornd.apply_and_round(os_, xs, chi_max=c)
+        _check_against_oracle(A.to_numpy()[q], ref, len(xs))
+        n2a, n2b = ornd.norm2(A.to_numpy()[q]), ornd.norm2(Bb.to_numpy()[q])
+        assert abs(n2a - n2b) <= 1e-10 * n2b
