@@ -20,6 +20,8 @@
 #include "common.hpp"
 #include "prof.hpp"
 
+#include <algorithm>
+
 namespace syn {
 
 constexpr int kNb = 64;
@@ -302,11 +304,19 @@ static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, doub
 // dropped columns zeroed (the convention of chol_diag_kernel), by
 //   (L11, X11) = chol_inv(G11);  L21 = G21 X11^T;  G22 -= L21 L21^T;
 //   (L22, X22) = chol_inv(G22);  X21 = -X22 (L21 X11).
-// Every step above the 64 x 64 leaves is a GEMM.  L and X must be zero above
-// the diagonal on entry; T is scratch of (n/2)^2 doubles.
+// Every step above the 64 x 64 leaves is a GEMM; G22 -= L21 L21^T and
+// T = L21 X11 need only L21 and X11 and share one launch.  L and X must be
+// zero above the diagonal on entry; T is scratch of chol_inv_scratch(n)
+// doubles (this level's n2 x n1 product, the second recursion's past it).
 // ws (ws_cap doubles), when given, lets the products split along K; the
 // CholeskyQR callers pass none: on the 8192 x 4096 chain the split saved
 // under 1% of a step there.
+static int64_t chol_inv_scratch(int64_t n) {
+    if (n <= kNb) return 0;
+    const int64_t n1 = ((n / kNb + 1) / 2) * kNb, n2 = n - n1;
+    return std::max(chol_inv_scratch(n1), n2 * n1 + chol_inv_scratch(n2));
+}
+
 static int chol_inv(double *G, int64_t ldg, double *L, int64_t ldl, double *X, int64_t ldx, int64_t n, double *T,
                     const double *scal, int *dropped, double tolrel, hipStream_t s, double *ws = nullptr,
                     int64_t ws_cap = 0) {
@@ -325,16 +335,20 @@ static int chol_inv(double *G, int64_t ldg, double *L, int64_t ldl, double *X, i
     rc = gemm_f64_split_launch(make_gemm(G + n1 * ldg, ldg, 0, X, ldx, 1, L21, ldl, n2, n1, n1, 1.0, 0.0, 8), s, ws,
                                ws_cap);
     if (rc) return rc;
-    // G22 -= L21 L21^T, lower triangle only
-    rc = gemm_f64_split_launch(
-        make_gemm(L21, ldl, 0, L21, ldl, 1, G + n1 * ldg + n1, ldg, n2, n2, n1, -1.0, 1.0, 1), s, ws, ws_cap);
+    // G22 -= L21 L21^T (lower triangle only) and T = L21 X11 (op(B) = X11
+    // lower), side by side; with a split-K workspace they go separately
+    const syn_gemm_item upd = make_gemm(L21, ldl, 0, L21, ldl, 1, G + n1 * ldg + n1, ldg, n2, n2, n1, -1.0, 1.0, 1);
+    const syn_gemm_item tl = make_gemm(L21, ldl, 0, X, ldx, 0, T, n1, n2, n1, n1, 1.0, 0.0, 16);
+    if (ws) {
+        if ((rc = gemm_f64_split_launch(upd, s, ws, ws_cap))) return rc;
+        if ((rc = gemm_f64_split_launch(tl, s, ws, ws_cap))) return rc;
+    } else if ((rc = gemm_f64_pair_launch(upd, tl, s))) {
+        return rc;
+    }
+    rc = chol_inv(G + n1 * ldg + n1, ldg, L + n1 * ldl + n1, ldl, X + n1 * ldx + n1, ldx, n2, T + n2 * n1, scal,
+                  dropped, tolrel, s, ws, ws_cap);
     if (rc) return rc;
-    rc = chol_inv(G + n1 * ldg + n1, ldg, L + n1 * ldl + n1, ldl, X + n1 * ldx + n1, ldx, n2, T, scal, dropped,
-                  tolrel, s, ws, ws_cap);
-    if (rc) return rc;
-    // T = L21 X11 (op(B) = X11 lower);  X21 = -X22 T (op(A) = X22 lower)
-    rc = gemm_f64_split_launch(make_gemm(L21, ldl, 0, X, ldx, 0, T, n1, n2, n1, n1, 1.0, 0.0, 16), s, ws, ws_cap);
-    if (rc) return rc;
+    // X21 = -X22 T (op(A) = X22 lower)
     return gemm_f64_split_launch(
         make_gemm(X + n1 * ldx + n1, ldx, 0, T, n1, 0, X + n1 * ldx, ldx, n2, n1, n2, -1.0, 0.0, 2), s, ws, ws_cap);
 }
@@ -382,11 +396,10 @@ __global__ __launch_bounds__(256) void near_identity_chol_kernel(const double *g
 }  // namespace syn
 
 extern "C" int64_t syn_cholqr_workspace(int64_t n) {
-    // G (n x n), X = inv(L) (n x n), the recursion scratch ((n/2)^2 + n),
-    // scalars, and two n x n partial products of the split-K Gram
+    // G (n x n), X = inv(L) (n x n), the recursion scratch, scalars, and two
+    // n x n partial products of the split-K Gram
     const int64_t nblk = (n + syn::kNb - 1) / syn::kNb;
-    const int64_t half = ((nblk + 1) / 2) * syn::kNb;
-    return 4 * n * n + half * half + nblk * syn::kNb * syn::kNb + 16;
+    return 4 * n * n + syn::chol_inv_scratch(n) + nblk * syn::kNb * syn::kNb + 16;
 }
 
 extern "C" int syn_potrf_f64(double *G, int64_t n, int64_t ldg, double *L, int64_t ldl, double *work, double tolrel,
@@ -412,11 +425,10 @@ extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, doub
     }
     hipStream_t s = (hipStream_t)stream;
     const int64_t nblk = (n + kNb - 1) / kNb;
-    const int64_t half = ((nblk + 1) / 2) * kNb;
     double *G = work;
     double *X = G + n * n;
     double *T = X + n * n;
-    double *scal = T + half * half + nblk * kNb * kNb;
+    double *scal = T + chol_inv_scratch(n) + nblk * kNb * kNb;
     // G = A^T A, lower triangle only (the Cholesky reads nothing else)
     double *splitws = scal + 16;
     int rc = gemm_f64_split_launch(make_gemm(A, lda, 1, A, lda, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,
@@ -468,11 +480,10 @@ extern "C" int syn_cholqr_refine_f64(const double *Q1, int64_t m, int64_t n, int
     }
     hipStream_t s = (hipStream_t)stream;
     const int64_t nblk = (n + kNb - 1) / kNb;
-    const int64_t half = ((nblk + 1) / 2) * kNb;
     double *G = work;
     double *X = G + n * n;
     double *T = X + n * n;
-    double *scal = T + half * half + nblk * kNb * kNb;
+    double *scal = T + chol_inv_scratch(n) + nblk * kNb * kNb;
     // G = Q1^T Q1 (lower triangle) and max |G - I| on it
     double *splitws = scal + 16;
     int rc = gemm_f64_split_launch(make_gemm(Q1, ldq1, 1, Q1, ldq1, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,

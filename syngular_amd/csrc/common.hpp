@@ -57,6 +57,8 @@ __device__ __forceinline__ int find_item(const Item *items, int n, int64_t b) {
 // Host-side helpers shared by the multi-step routines (gemm.hip).
 int gemm_f64_launch(syn_gemm_item P, hipStream_t s, bool mixed = false);
 int gemm_f64_split_launch(syn_gemm_item P, hipStream_t s, double *ws, int64_t ws_cap);
+// two independent FP64 problems in one launch (one tile size for both)
+int gemm_f64_pair_launch(syn_gemm_item P0, syn_gemm_item P1, hipStream_t s);
 syn_gemm_item make_gemm(const double *A, int64_t lda, int ta, const double *B, int64_t ldb, int tb, double *C,
                         int64_t ldc, int64_t m, int64_t n, int64_t k, double alpha, double beta, int tri = 0);
 

@@ -17,6 +17,8 @@
 //   D: reg r of lane l holds D[i = l/16 + 4*r][j = l%16]  (checked against
 //      NumPy on the device: rows 4g+r and g+4r differ unless g == r)
 #include "common.hpp"
+
+#include <algorithm>
 #include "prof.hpp"
 
 namespace syn {
@@ -273,6 +275,20 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_single
     gemm_f64_tile<BM, BN, BK, WM, WN, CT>(P, tile_of_block(blockIdx.x, total, P.tiles_n, P.tri));
 }
 
+// Two independent problems in one launch: work-groups [0, tiles0) take the
+// first, the rest the second (each in its own tile order).  For short
+// dependent chains of small products (the recursive Cholesky-inverse) this
+// saves a launch and lets the two run side by side.
+template <int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_pair(syn_gemm_item P0, int64_t tiles0,
+                                                                             syn_gemm_item P1, int64_t tiles1) {
+    const int64_t bid = blockIdx.x;
+    if (bid < tiles0)
+        gemm_f64_tile<BM, BN, BK, WM, WN, double>(P0, tile_of_block(bid, tiles0, P0.tiles_n, P0.tri));
+    else
+        gemm_f64_tile<BM, BN, BK, WM, WN, double>(P1, tile_of_block(bid - tiles0, tiles1, P1.tiles_n, P1.tri));
+}
+
 // Split-K form of a single problem: work-group b computes K slice
 // b / tiles of tile b % tiles; slice 0 writes C (with beta), slice s > 0
 // writes the partial product into ws + (s - 1) m ws_ld, and
@@ -310,6 +326,41 @@ __global__ __launch_bounds__(256) void gemm_splitk_reduce(double *C, int64_t ldc
 
 // One GEMM launched with its descriptor passed by value (no device table):
 // used by the library's own multi-step routines (Cholesky, CholeskyQR, ...).
+int gemm_f64_pair_launch(syn_gemm_item P0, syn_gemm_item P1, hipStream_t s) {
+    const bool live0 = P0.m > 0 && P0.n > 0 && !(P0.k <= 0 && P0.beta == 1.0);
+    const bool live1 = P1.m > 0 && P1.n > 0 && !(P1.k <= 0 && P1.beta == 1.0);
+    if (!live0 || !live1) {
+        if (live0) return gemm_f64_launch(P0, s);
+        if (live1) return gemm_f64_launch(P1, s);
+        return SYN_OK;
+    }
+    // one tile size for both, chosen by the larger output
+    const int64_t area = std::max(P0.m * P0.n, P1.m * P1.n);
+    const int tile = (area >= (int64_t)256 * 128 * 128) ? 1 : 0;
+    const int64_t bm = tile ? 128 : 64;
+    P0.tiles_n = (P0.n + bm - 1) / bm;
+    P1.tiles_n = (P1.n + bm - 1) / bm;
+    P0.block_start = P1.block_start = 0;
+    const int64_t t0 = ((P0.m + bm - 1) / bm) * P0.tiles_n, t1 = ((P1.m + bm - 1) / bm) * P1.tiles_n;
+    double fl = 0.0;
+    for (const syn_gemm_item *P : {&P0, &P1}) {
+        double f = 2.0 * (double)P->m * (double)P->n * (double)P->k;
+        const int nbits = __builtin_popcount((unsigned)P->tri);
+        if (nbits == 1) f *= 0.5;
+        if (nbits >= 2) f /= 3.0;
+        fl += f;
+    }
+    ProfScope ps(tile ? kTagGemmSingle128 : kTagGemmSingle64, fl, s);
+    if (tile)
+        hipLaunchKernelGGL((gemm_f64_pair<128, 128, 16, 64, 64>), dim3((unsigned)(t0 + t1)), dim3(256), 0, s, P0, t0,
+                           P1, t1);
+    else
+        hipLaunchKernelGGL((gemm_f64_pair<64, 64, 16, 32, 32>), dim3((unsigned)(t0 + t1)), dim3(256), 0, s, P0, t0,
+                           P1, t1);
+    SYN_LAUNCH_CHECK("gemm_f64_pair_launch");
+    return SYN_OK;
+}
+
 int gemm_f64_launch(syn_gemm_item P, hipStream_t s, bool mixed) {
     if (P.m <= 0 || P.n <= 0) return SYN_OK;
     if (P.k <= 0 && P.beta == 1.0) return SYN_OK;
