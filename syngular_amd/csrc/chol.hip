@@ -161,6 +161,8 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
         const double sd = keep[tid] ? sqrt(lo[tid][tid]) : 0.0;
         sdiag[tid] = sd;
         rdiag[tid] = keep[tid] ? 1.0 / sd : 1.0;
+    } else if (tid < kNb) {
+        rdiag[tid] = 1.0;   // rows past nb: zero rows of L~, unit rows of its inverse
     }
     __syncthreads();
     for (int e = tid; e < nb * nb; e += NT) {
@@ -179,50 +181,101 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
         if (!keep[i] && k <= i) lo[i][k] = (i == k) ? 1.0 : 0.0;
     }
     __syncthreads();
-    // Z = inv(L~) by right-looking forward substitution in registers: thread
-    // (ty, tx) holds Z_acc at rows ty+TG*u, columns tx+TG*v; row k of Z is final
-    // once updated by rows < k, its owners divide by L~_kk, publish it through
-    // a double-buffered LDS row and write it out.
-    __shared__ double zrow[2][kNb];
-    double zr[RB][RB];
+    // Z = inv(L~) by blocks.  The four waves first invert the four 16 x 16
+    // diagonal blocks of L~ (wave w: rows and columns 16w .. 16w + 15) by
+    // forward substitution: lane (g, c) holds column c of rows g, g + 4, g + 8,
+    // g + 12, and a finished row reaches the lanes of its columns by a
+    // cross-lane read, so these 16 steps need no work-group barrier.  Two
+    // doubling levels follow on the matrix cores, Z21 = -Z22 (L21 Z11):
+    // 16 -> 32 for both 32-blocks, then 32 -> 64.  Their products T = L21 Z11
+    // are kept in the upper triangle of zs, which Z (lower) does not use, so
+    // the Z operands are masked to their lower triangle where they are read.
+    static_assert(kNb == 64 && NT == 256, "the blocked inverse assumes four waves and 64 x 64 leaves");
+    double(*zs)[kNb + 1] = a;   // the Schur complements are no longer needed
+    const int w = tid >> 6, l = tid & 63;
+    {
+        const int c = l & 15, g = l >> 4, b0 = 16 * w;
+        double zr[4];
 #pragma unroll
-    for (int u = 0; u < RB; ++u)
+        for (int u = 0; u < 4; ++u) zr[u] = (g + 4 * u == c) ? 1.0 : 0.0;
 #pragma unroll
-        for (int v = 0; v < RB; ++v) zr[u][v] = (ty + TG * u == tx + TG * v) ? 1.0 : 0.0;
-    auto publish = [&](int k) {
-        if (k < nb && ty == (k % TG)) {
-            const double inv = rdiag[k];
+        for (int k = 0; k < 16; ++k) {
+            const int gk = k & 3, uk = k >> 2;
+            if (g == gk) zr[uk] *= rdiag[b0 + k];   // row k is final
+            const double zk = __shfl(zr[uk], gk * 16 + c, 64);
 #pragma unroll
-            for (int u = 0; u < RB; ++u)
-                if (ty + TG * u == k) {
-#pragma unroll
-                    for (int v = 0; v < RB; ++v) {
-                        const int c = tx + TG * v;
-                        const double zv = zr[u][v] * inv;
-                        zr[u][v] = zv;
-                        zrow[k & 1][c] = zv;
-                        if (c < nb) zb[k * ldz + c] = (keep[c] && c <= k) ? zv : 0.0;
-                    }
-                }
+            for (int u = 0; u < 4; ++u)
+                if (g + 4 * u > k) zr[u] = fma(-lo[b0 + g + 4 * u][b0 + k], zk, zr[u]);
         }
-    };
-    publish(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) zs[b0 + g + 4 * u][b0 + c] = zr[u];
+    }
     __syncthreads();
-    for (int k = 0; k + 1 < nb; ++k) {
-        const double *zk = zrow[k & 1];
-        double lk[RB], zc[RB];
+    // MFMA f64 16x16x4 operands: A[i = l%16][k = l/16], B[k = l/16][j = l%16];
+    // D register r of lane l is D[l/16 + 4r][l%16]
+    const int li = l & 15, lk = l >> 4;
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    // 16 -> 32: waves 0 and 1 take the 32-blocks at o = 0 and o = 32
+    if (w < 2) {
+        const int o = 32 * w;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int u = 0; u < RB; ++u) lk[u] = lo[ty + TG * u][k];
+        for (int k0 = 0; k0 < 16; k0 += 4) {   // T = L21 Z11
+            const int k = k0 + lk;
+            const double za = lo[o + 16 + li][o + k];
+            const double zb_ = (k >= li) ? zs[o + k][o + li] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(za, zb_, acc, 0, 0, 0);
+        }
 #pragma unroll
-        for (int v = 0; v < RB; ++v) zc[v] = zk[tx + TG * v];
-        // rows <= k were published already and are not read again, so all
-        // rows are updated
+        for (int r = 0; r < 4; ++r) zs[o + lk + 4 * r][o + 16 + li] = acc[r];
+    }
+    __syncthreads();
+    if (w < 2) {
+        const int o = 32 * w;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int u = 0; u < RB; ++u)
+        for (int k0 = 0; k0 < 16; k0 += 4) {   // Z21 = -Z22 T
+            const int k = k0 + lk;
+            const double za = (k <= li) ? zs[o + 16 + li][o + 16 + k] : 0.0;
+            const double zb_ = zs[o + k][o + 16 + li];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(za, zb_, acc, 0, 0, 0);
+        }
 #pragma unroll
-            for (int v = 0; v < RB; ++v) zr[u][v] = fma(-lk[u], zc[v], zr[u][v]);
-        publish(k + 1);
-        __syncthreads();
+        for (int r = 0; r < 4; ++r) zs[o + 16 + lk + 4 * r][o + li] = -acc[r];
+    }
+    __syncthreads();
+    // 32 -> 64: wave w takes output tile (ti, tj) of the 32 x 32 products
+    {
+        const int ti = w >> 1, tj = w & 1;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < 32; k0 += 4) {   // T = L21 Z11
+            const int k = k0 + lk;
+            const double za = lo[32 + 16 * ti + li][k];
+            const double zb_ = (k >= 16 * tj + li) ? zs[k][16 * tj + li] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(za, zb_, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zs[16 * ti + lk + 4 * r][32 + 16 * tj + li] = acc[r];
+    }
+    __syncthreads();
+    {
+        const int ti = w >> 1, tj = w & 1;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < 32; k0 += 4) {   // Z21 = -Z22 T
+            const int k = k0 + lk;
+            const double za = (k <= 16 * ti + li) ? zs[32 + 16 * ti + li][32 + k] : 0.0;
+            const double zb_ = zs[k][32 + 16 * tj + li];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(za, zb_, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zs[32 + 16 * ti + lk + 4 * r][16 * tj + li] = -acc[r];
+    }
+    __syncthreads();
+    for (int e = tid; e < nb * nb; e += NT) {
+        const int k = e / nb, c = e % nb;
+        zb[k * ldz + c] = (keep[c] && c <= k) ? zs[k][c] : 0.0;
     }
 }
 
