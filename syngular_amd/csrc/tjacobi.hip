@@ -172,11 +172,20 @@ __device__ __forceinline__ int tj_find(const syn_gbj_item *items, int n, int64_t
     return lo;
 }
 
-// Per-problem scratch layout (doubles): W tiles (half * TP^2), the rotation
-// flags of the pairs (half ints), the rotation floor, then the sort scratch.
+// Per-problem scratch layout (doubles): two sets of W tiles (half * TP^2
+// each) and of rotation flags (half ints each), selected by step parity so a
+// step's pair solves can write theirs while the previous step's update reads
+// its own; the rotation floor; the sort scratch (rt copy, s); a second
+// nv x nv buffer for the Gram matrix (the fused steps read one buffer and
+// write the other).
 template <int TB>
-__device__ __forceinline__ int *tj_flags(const syn_gbj_item &P, int64_t half) {
-    return (int *)(P.w + half * (2 * TB) * (2 * TB));
+__device__ __forceinline__ double *tj_wset(const syn_gbj_item &P, int64_t half, int par) {
+    return P.w + par * half * (2 * TB) * (2 * TB);
+}
+
+template <int TB>
+__device__ __forceinline__ int *tj_flags(const syn_gbj_item &P, int64_t half, int par = 0) {
+    return (int *)(P.w + 2 * half * (2 * TB) * (2 * TB)) + par * half;
 }
 
 // Absolute rotation floor eps * max_i G_ii (set by the init kernel): Gram
@@ -185,32 +194,28 @@ __device__ __forceinline__ int *tj_flags(const syn_gbj_item &P, int64_t half) {
 // (a null space at noise level) a purely relative test would never settle.
 template <int TB>
 __host__ __device__ __forceinline__ int64_t tj_floor_off(int64_t half) {
-    return half * (2 * TB) * (2 * TB) + (half + 1) / 2;
+    return 2 * half * (2 * TB) * (2 * TB) + half;
 }
 
+// the second Gram buffer (ld nv), after the sort scratch
 template <int TB>
-__global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *__restrict__ items, int n,
-                                                           int64_t step, double tol, int inner_sweeps) {
+__host__ __device__ __forceinline__ int64_t tj_g2_off(int64_t half, int64_t nv) {
+    return tj_floor_off<TB>(half) + 1 + nv * nv + nv + 4;
+}
+
+// The pair solve on a tile S already in LDS: W <- I, the rotation test of
+// the tile, then inner_sweeps round-robin sweeps of two-sided rotations;
+// writes W to Wout and the pair's flag (1: rotated) and counts a rotating
+// pair in P.counter.  Every thread of the work-group calls it.
+template <int TB>
+__device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDims &D, int64_t pw,
+                                              double (*S)[2 * TB + 1], double (*W)[2 * TB + 1], v2d *cs,
+                                              int &do_rot, int &any_rot, double tol, int inner_sweeps, double *Wout,
+                                              int *flags) {
     constexpr int TP = 2 * TB, NT = TB * TB;
-    __shared__ double S[TP][TP + 1];
-    __shared__ double W[TP][TP + 1];
-    __shared__ v2d cs[TP / 2];   // (c, s) of each plane of the round
-    __shared__ int rows[TP];
-    __shared__ int do_rot, any_rot;
-    const int64_t bid = blockIdx.x;
-    const syn_gbj_item &P = items[tj_find<true>(items, n, bid)];
-    const TjDims D = tj_dims<TB>(P.nv);
-    const int64_t pw = bid - P.solve_start;
-    const int64_t st = step % (D.np - 1);
     const int tid = threadIdx.x;
-    if (tid < TP) rows[tid] = tj_row<TB>(st, pw, D.np, D.nblk, P.nv, tid);
+    for (int e = tid; e < TP * TP; e += NT) W[e / TP][e % TP] = (e / TP == e % TP) ? 1.0 : 0.0;
     if (tid == 0) do_rot = 0;
-    __syncthreads();
-    for (int e = tid; e < TP * TP; e += NT) {
-        const int i = e / TP, k = e % TP, ri = rows[i], rk = rows[k];
-        S[i][k] = (ri >= 0 && rk >= 0) ? P.g[(int64_t)ri * P.ldg + rk] : 0.0;
-        W[i][k] = (i == k) ? 1.0 : 0.0;
-    }
     __syncthreads();
     const double floor_abs = P.w[tj_floor_off<TB>(D.half)];
     for (int e = tid; e < TP * TP; e += NT) {
@@ -218,7 +223,6 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
         if (i < k && rotates(S[i][k], S[i][i], S[k][k], tol, floor_abs)) do_rot = 1;
     }
     __syncthreads();
-    int *flags = tj_flags<TB>(P, D.half);
     if (!do_rot) {
         if (tid == 0) flags[pw] = 0;
         return;
@@ -307,12 +311,37 @@ __global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *_
         __syncthreads();
         if (!any) break;
     }
-    double *Wout = P.w + pw * TP * TP;
     for (int e = tid; e < TP * TP; e += NT) Wout[e] = W[e / TP][e % TP];
     if (tid == 0) {
         flags[pw] = 1;
         atomicAdd(P.counter, 1);
     }
+}
+
+
+template <int TB>
+__global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *__restrict__ items, int n,
+                                                           int64_t step, double tol, int inner_sweeps) {
+    constexpr int TP = 2 * TB, NT = TB * TB;
+    __shared__ double S[TP][TP + 1];
+    __shared__ double W[TP][TP + 1];
+    __shared__ v2d cs[TP / 2];   // (c, s) of each plane of the round
+    __shared__ int rows[TP];
+    __shared__ int do_rot, any_rot;
+    const int64_t bid = blockIdx.x;
+    const syn_gbj_item &P = items[tj_find<true>(items, n, bid)];
+    const TjDims D = tj_dims<TB>(P.nv);
+    const int64_t pw = bid - P.solve_start;
+    const int64_t st = step % (D.np - 1);
+    const int tid = threadIdx.x;
+    if (tid < TP) rows[tid] = tj_row<TB>(st, pw, D.np, D.nblk, P.nv, tid);
+    __syncthreads();
+    for (int e = tid; e < TP * TP; e += NT) {
+        const int i = e / TP, k = e % TP, ri = rows[i], rk = rows[k];
+        S[i][k] = (ri >= 0 && rk >= 0) ? P.g[(int64_t)ri * P.ldg + rk] : 0.0;
+    }
+    tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, inner_sweeps, tj_wset<TB>(P, D.half, 0) + pw * TP * TP,
+                      tj_flags<TB>(P, D.half, 0));
 }
 
 // MFMA f64 16x16x4 layout: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
@@ -425,6 +454,284 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
     }
 }
 
+// Pair of step `step` - 1 holding block x, and the half (0: first 16 rows,
+// 1: second) it sits in: the inverse of tj_pair's round-robin map.
+__device__ __forceinline__ void tj_owner(int64_t x, int64_t sp, int64_t np, int64_t half, int64_t &w, int &h) {
+    const int64_t np1 = np - 1;
+    if (x == np - 1) {
+        w = 0;
+        h = 1;
+    } else if (x == sp) {
+        w = 0;
+        h = 0;
+    } else {
+        const int64_t w1 = ((x - sp) % np1 + np1) % np1;
+        if (w1 >= 1 && w1 <= half - 1) {
+            w = w1;
+            h = 0;
+        } else {
+            w = ((sp - x) % np1 + np1) % np1;
+            h = 1;
+        }
+    }
+}
+
+// One fused step: the tile updates of step - 1 (has_update) and the pair
+// solves of step (has_solve) in one launch, so the updates' work-groups run
+// beside the latency-bound solves.  The Gram matrix is read from buffer `cur`
+// (0: P.g, 1: the workspace copy) and the updates write the other buffer,
+// every tile (unrotated ones are copied), so no buffer is read and written in
+// the same launch.  A solve builds its tile of G_step = W^T G_(step-1) W from
+// buffer `cur` and the previous step's rotations: for its blocks a and b,
+// with V = diag(W_A(:, half of a), W_B(:, half of b)) over the rows of their
+// pairs A and B at step - 1, S = V^T G_old V (two small MFMA products).
+// Solves come first in the grid (they are the long ones).  W tiles and
+// flags alternate between two sets by step parity.
+template <int TB>
+__global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *__restrict__ items, int n,
+                                                          int64_t step, int has_update, int64_t solve_total,
+                                                          int cur, double tol, int inner_sweeps,
+                                                          unsigned long long *bytes) {
+    constexpr int TP = 2 * TB, NT = TB * TB;
+    static_assert(TB == 16 && NT == kTUpdThreads, "fused steps: 16-row blocks, one block size for both roles");
+    constexpr int kS = TP * (TP + 1), kV = 2 * TP * (TP + 1), kY = TP * (2 * TP + 1), kX = TP * (kTCol + 1);
+    constexpr int kSolve = 2 * kS + kV + kY, kUpd = 3 * kS + kX;
+    __shared__ double smem[kSolve > kUpd ? kSolve : kUpd];
+    __shared__ v2d cs[TP / 2];
+    __shared__ int rows[TP], rb_[TP];
+    __shared__ int do_rot, any_rot;
+    const int64_t bid = blockIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    auto gbuf = [](const syn_gbj_item &P, int which, int64_t half, int64_t &ld) -> double * {
+        if (which == 0) {
+            ld = P.ldg;
+            return P.g;
+        }
+        ld = P.nv;
+        return P.w + tj_g2_off<TB>(half, P.nv);
+    };
+    if (bid < solve_total) {
+        // ---- pair solve of `step` ------------------------------------------
+        double(*S)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem);
+        double(*W)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + kS);
+        double(*V)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + 2 * kS);
+        double(*Y)[2 * TP + 1] = reinterpret_cast<double(*)[2 * TP + 1]>(smem + 2 * kS + kV);
+        const syn_gbj_item &P = items[tj_find<true>(items, n, bid)];
+        const TjDims D = tj_dims<TB>(P.nv);
+        const int64_t pw = bid - P.solve_start;
+        const int64_t st = step % (D.np - 1);
+        int64_t ldc;
+        const double *Gc = gbuf(P, cur, D.half, ldc);
+        if (!has_update) {
+            if (tid < TP) rows[tid] = tj_row<TB>(st, pw, D.np, D.nblk, P.nv, tid);
+            __syncthreads();
+            for (int e = tid; e < TP * TP; e += NT) {
+                const int i = e / TP, k = e % TP, ri = rows[i], rk = rows[k];
+                S[i][k] = (ri >= 0 && rk >= 0) ? Gc[(int64_t)ri * ldc + rk] : 0.0;
+            }
+        } else {
+            const int64_t sp = (step - 1) % (D.np - 1);
+            int64_t ba, bb, wA, wB;
+            int hA, hB;
+            tj_pair(st, pw, D.np, ba, bb);
+            tj_owner(ba, sp, D.np, D.half, wA, hA);
+            tj_owner(bb, sp, D.np, D.half, wB, hB);
+            // the old rows are computed by every thread for its own loads (no
+            // barrier before the first global reads)
+            auto orow_of = [&](int k) -> int {
+                return k < TP ? tj_row<TB>(sp, wA, D.np, D.nblk, P.nv, k) : tj_row<TB>(sp, wB, D.np, D.nblk, P.nv, k - TP);
+            };
+            // G_old operands of this wave's two Y tiles, loaded first so their
+            // latency overlaps the V build below
+            double gpre[2][TP / 4];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = wv + 4 * h, tr = q >> 2, tc = q & 3;
+                const int rc = orow_of(16 * tc + li);
+#pragma unroll
+                for (int k0 = 0; k0 < TP; k0 += 4) {
+                    const int rk = orow_of(tr * TP + k0 + lk);
+                    gpre[h][k0 / 4] = (rk >= 0 && rc >= 0) ? Gc[(int64_t)rk * ldc + rc] : 0.0;
+                }
+            }
+            const int pp = (int)((step - 1) & 1);
+            const int *fl = tj_flags<TB>(P, D.half, pp);
+            const int fA = fl[wA], fB = fl[wB];
+            const double *WA = tj_wset<TB>(P, D.half, pp) + wA * TP * TP;
+            const double *WB = tj_wset<TB>(P, D.half, pp) + wB * TP * TP;
+            // V (2TP x TP): columns c < TB are column hA TB + c of W_A on rows
+            // 0 .. TP-1, columns c >= TB column hB TB + c - TB of W_B on rows
+            // TP .. 2TP-1 (identity columns for a pair that did not rotate)
+            for (int e = tid; e < 2 * TP * TP; e += NT) {
+                const int j = e / TP, c = e % TP;
+                double v = 0.0;
+                if (c < TB && j < TP) {
+                    const int col = hA * TB + c;
+                    v = fA ? WA[j * TP + col] : (j == col ? 1.0 : 0.0);
+                } else if (c >= TB && j >= TP) {
+                    const int col = hB * TB + c - TB, jj = j - TP;
+                    v = fB ? WB[jj * TP + col] : (jj == col ? 1.0 : 0.0);
+                }
+                V[j][c] = v;
+            }
+            __syncthreads();
+            // Y = V^T G_old (TP x 2TP): 2 x 4 tiles of 16, two per wave; the
+            // rows of tile row tr come from V's column block tr, nonzero on
+            // rows tr TP .. tr TP + TP - 1 only
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = wv + 4 * h, tr = q >> 2, tc = q & 3;
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < TP; k0 += 4) {
+                    const int k = tr * TP + k0 + lk;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(V[k][16 * tr + li], gpre[h][k0 / 4], acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Y[16 * tr + lk + 4 * r][16 * tc + li] = acc[r];
+            }
+            __syncthreads();
+            // S = Y V (TP x TP): one 16 x 16 tile per wave; column block tj of
+            // V is nonzero on rows tj TP .. tj TP + TP - 1 only
+            {
+                const int ti = wv >> 1, tj = wv & 1;
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < TP; k0 += 4) {
+                    const int k = tj * TP + k0 + lk;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Y[16 * ti + li][k], V[k][16 * tj + li], acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) S[16 * ti + lk + 4 * r][16 * tj + li] = acc[r];
+            }
+        }
+        const int ps = (int)(step & 1);
+        tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, inner_sweeps,
+                          tj_wset<TB>(P, D.half, ps) + pw * TP * TP, tj_flags<TB>(P, D.half, ps));
+        return;
+    }
+    if (!has_update) return;
+    // ---- tile update of step - 1: buffer cur -> buffer cur ^ 1 ----------------
+    constexpr unsigned long long kTileBytes = (unsigned long long)TP * TP * sizeof(double);
+    constexpr unsigned long long kSlabBytes = 2ull * TP * kTCol * sizeof(double);
+    double(*Wa)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem);
+    double(*Wb)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + kS);
+    double(*Tt)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + 2 * kS);
+    double(*X)[kTCol + 1] = reinterpret_cast<double(*)[kTCol + 1]>(smem + 3 * kS);
+    int *ra = rows, *rb = rb_;
+    const int64_t ub = bid - solve_total;
+    const syn_gbj_item &P = items[tj_find<false>(items, n, ub)];
+    const TjDims D = tj_dims<TB>(P.nv);
+    const int64_t su = step - 1, st = su % (D.np - 1);
+    const int pu = (int)(su & 1);
+    const int64_t t = ub - P.tile_start;
+    const int *flags = tj_flags<TB>(P, D.half, pu);
+    const double *wset = tj_wset<TB>(P, D.half, pu);
+    int64_t ldc, ldn;
+    const double *Gc = gbuf(P, cur, D.half, ldc);
+    double *Gn = gbuf(P, cur ^ 1, D.half, ldn);
+    if (t < D.half * D.half) {
+        // G tile (a, b), a <= b, and its transpose (b, a)
+        const int64_t a = t / D.half, b = t % D.half;
+        if (a > b) return;
+        const int fa = flags[a], fb = flags[b];
+        if (tid < TP) {
+            ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
+            rb[tid] = tj_row<TB>(st, b, D.np, D.nblk, P.nv, tid);
+        }
+        __syncthreads();
+        for (int e = tid; e < TP * TP; e += NT) {
+            const int i = e / TP, k = e % TP, r = ra[i], c = rb[k];
+            X[i][k] = (r >= 0 && c >= 0) ? Gc[(int64_t)r * ldc + c] : 0.0;
+        }
+        if (fa || fb) {
+            if (tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);
+            const double *wa = wset + a * TP * TP, *wb = wset + b * TP * TP;
+            for (int e = tid; e < TP * TP; e += NT) {
+                const int i = e / TP, k = e % TP;
+                Wa[i][k] = fa ? wa[e] : (i == k ? 1.0 : 0.0);
+                Wb[i][k] = fb ? wb[e] : (i == k ? 1.0 : 0.0);
+            }
+            __syncthreads();
+            {   // T = Wa^T X, one 16 x 16 tile per wave
+                const int ti = wv >> 1, tj = wv & 1;
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < TP; k0 += 4)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li],
+                                                                acc, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Tt[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
+            }
+            __syncthreads();
+            {   // G_ab = T Wb, into X
+                const int ti = wv >> 1, tj = wv & 1;
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < TP; k0 += 4)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tt[ti * 16 + li][k0 + lk], Wb[k0 + lk][tj * 16 + li],
+                                                                acc, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < TP * TP; e += NT) {
+            const int i = e / TP, k = e % TP;
+            const int gr = ra[i], gc = rb[k];
+            if (gr >= 0 && gc >= 0) Gn[(int64_t)gr * ldn + gc] = X[i][k];
+            if (a != b) {
+                const int tr = rb[i], tc = ra[k];
+                if (tr >= 0 && tc >= 0) Gn[(int64_t)tr * ldn + tc] = X[k][i];
+            }
+        }
+        return;
+    }
+    // rt slab (a, columns [c0, c0 + kTCol)), in place
+    const int64_t u = t - D.half * D.half;
+    const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
+    if (!flags[a]) return;
+    if (tid == 0) atomicAdd(bytes, kSlabBytes);
+    if (tid < TP) ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
+    __syncthreads();
+    const double *wa = wset + a * TP * TP;
+    for (int e = tid; e < TP * TP; e += NT) Wa[e / TP][e % TP] = wa[e];
+    for (int e = tid; e < TP * kTCol; e += NT) {
+        const int i = e / kTCol, j = e % kTCol, r = ra[i];
+        X[i][j] = (r >= 0 && c0 + j < P.nv) ? P.rt[(int64_t)r * P.ldr + c0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int q = wv; q < (TP / 16) * (kTCol / 16); q += NT / 64) {
+        const int ti = q / (kTCol / 16), tj = q % (kTCol / 16);
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < TP; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc, 0, 0,
+                                                        0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = ti * 16 + lk + 4 * r, j = tj * 16 + li;
+            const int gr = ra[i];
+            if (gr >= 0 && c0 + j < P.nv) P.rt[(int64_t)gr * P.ldr + c0 + j] = acc[r];
+        }
+    }
+}
+
+// Copy the Gram matrix back from the workspace buffer (ld nv) into P.g for
+// the problems whose last fused step left it there (which[i] != 0).
+template <int TB>
+__global__ __launch_bounds__(256) void tjf_copyback_kernel(const syn_gbj_item *__restrict__ items,
+                                                           const int32_t *__restrict__ which) {
+    const syn_gbj_item &P = items[blockIdx.x];
+    if (!which[blockIdx.x]) return;
+    const double *g2 = P.w + tj_g2_off<TB>(tj_dims<TB>(P.nv).half, P.nv);
+    const int64_t nel = P.nv * P.nv;
+    for (int64_t e = (int64_t)blockIdx.y * 256 + threadIdx.x; e < nel; e += (int64_t)gridDim.y * 256) {
+        const int64_t i = e / P.nv, j = e - i * P.nv;
+        P.g[i * P.ldg + j] = g2[e];
+    }
+}
+
 template <int TB>
 __global__ __launch_bounds__(256) void tj_init_kernel(const syn_gbj_item *__restrict__ items) {
     __shared__ double red[4];
@@ -481,11 +788,12 @@ __global__ __launch_bounds__(256) void tj_zero_kernel(int32_t *c, int n) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) c[i] = 0;
 }
 
-// per-problem workspace: W tiles, flags, floor, then the sort scratch (rt copy, s)
+// per-problem workspace: W tiles and flags (two sets), floor, the sort
+// scratch (rt copy, s) and the second Gram buffer
 template <int TB>
 static int64_t tj_work(int64_t nv) {
     const int64_t half = tj_dims<TB>(nv).half;
-    return tj_floor_off<TB>(half) + 1 + nv * nv + nv + 4;
+    return tj_g2_off<TB>(half, nv) + nv * nv;
 }
 
 static int64_t tj_work_max(int64_t nv) {
@@ -495,6 +803,9 @@ static int64_t tj_work_max(int64_t nv) {
 
 static int g_tj_inner = 1;
 static int g_tj_block = 0;   // 0: automatic (16-row blocks: measured at least as fast up to 2048)
+#ifndef SYN_TJ_FUSED
+#define SYN_TJ_FUSED 1
+#endif
 
 template <int TB>
 static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
@@ -527,6 +838,11 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     SYN_LAUNCH_CHECK("gram byte counter");
 
     std::vector<int32_t> hc(n_items);
+    // fused steps (16-row blocks): buffer holding each problem's Gram matrix
+    // (0: P.g, 1: the workspace copy); all problems of a sweep share it
+    constexpr bool fused = TB == 16 && SYN_TJ_FUSED;
+    std::vector<int32_t> gpar(n_items, 0);
+    int cur = 0;
     std::vector<int> idx;
     for (int i = 0; i < n_items; ++i)
         if (all[i].active) idx.push_back(i);
@@ -546,7 +862,19 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
         }
         if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, counters, n_items);
-        for (int64_t step = 0; step < max_np - 1; ++step) {
+        for (int64_t step = 0; fused && step < max_np; ++step) {
+            // solves of step (all but the last launch) beside the updates of
+            // step - 1 (all but the first)
+            const bool hs = step < max_np - 1, hu = step > 0;
+            const int64_t grid = (hs ? ss : 0) + (hu ? ts : 0);
+            ProfScope ps(kTagGramSolve, 0.0, st);
+            hipLaunchKernelGGL(tjf_step_kernel<16>, dim3((unsigned)grid), dim3(256), 0, st, items, (int)act.size(),
+                               step, hu ? 1 : 0, hs ? ss : (int64_t)0, cur, tol, g_tj_inner, dbytes);
+            if (hu) cur ^= 1;
+        }
+        if (fused)
+            for (int i : idx) gpar[i] = cur;
+        for (int64_t step = 0; !fused && step < max_np - 1; ++step) {
             {
                 ProfScope ps(kTagGramSolve, 0.0, st);
                 hipLaunchKernelGGL(tj_solve_kernel<TB>, dim3((unsigned)ss), dim3(TB * TB), 0, st, items,
@@ -577,6 +905,13 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
 
     // eigenvalues, host argsort per problem, permutation of rt rows and s
     if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
+    if (fused) {
+        // the Gram matrix is diagonalised in place: bring back the ones the
+        // last fused step left in the workspace copy
+        if (int rc = upload(gpar.data(), counters, n_items * sizeof(int32_t), "gram parity")) return rc;
+        hipLaunchKernelGGL(tjf_copyback_kernel<TB>, dim3(n_items, 64), dim3(256), 0, st, items, counters);
+        SYN_LAUNCH_CHECK("gram copy back");
+    }
     hipLaunchKernelGGL(tj_diag_kernel, dim3(n_items), dim3(256), 0, st, items);
     SYN_LAUNCH_CHECK("gram diag");
     std::vector<int64_t> poff(n_items), toff(n_items);
