@@ -456,14 +456,15 @@ int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_
 /* 9/10 Gram block-Jacobi pair solve / tile update, 11/12 mixed GEMM       */
 /* 64/128 tiles, 13/14 split-K GEMM of 128/64 tiles (CholeskyQR Gram       */
 /* matrices and the recursive Cholesky's products, partial-sum pass        */
-/* included).                                                             */
+/* included), 15/16 two independent GEMMs in one launch of 64/128 tiles   */
+/* (the recursive Cholesky's G22 update beside T = L21 X11).              */
 /* syn_prof_note_flops sets the count attached to the next grouped GEMM */
 /* or apply launch (their tables live on the device).                    */
 /* syn_prof_select(mask) limits the events to the tags whose bit is set  */
 /* (all by default; kept until changed): two event packets per launch    */
 /* slow a stream of short kernels down by microseconds each.             */
 /* ------------------------------------------------------------------ */
-#define SYN_PROF_NTAGS 16
+#define SYN_PROF_NTAGS 18
 int syn_prof_begin(void);
 int syn_prof_select(uint32_t mask);
 int syn_prof_note_flops(double flops);

@@ -123,7 +123,7 @@ SIGNATURES = {
     "syn_tune_jacobi_inner_sweeps": (None, [_I32]),
 }
 
-PROF_NTAGS = 16
+PROF_NTAGS = 18
 PROF_TAGS = {
     0: "gemm_f64_single<64,64>",
     1: "gemm_f64_single<128,128>",
@@ -140,6 +140,8 @@ PROF_TAGS = {
     12: "gemm_mixed<128,128>",
     13: "gemm_f64_splitk<128,128>",
     14: "gemm_f64_splitk<64,64>",
+    15: "gemm_f64_pair<64,64>",
+    16: "gemm_f64_pair<128,128>",
 }
 
 

@@ -350,7 +350,7 @@ int gemm_f64_pair_launch(syn_gemm_item P0, syn_gemm_item P1, hipStream_t s) {
         if (nbits >= 2) f /= 3.0;
         fl += f;
     }
-    ProfScope ps(tile ? kTagGemmSingle128 : kTagGemmSingle64, fl, s);
+    ProfScope ps(tile ? kTagGemmPair128 : kTagGemmPair64, fl, s);
     if (tile)
         hipLaunchKernelGGL((gemm_f64_pair<128, 128, 16, 64, 64>), dim3((unsigned)(t0 + t1)), dim3(256), 0, s, P0, t0,
                            P1, t1);

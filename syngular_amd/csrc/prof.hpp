@@ -21,6 +21,8 @@ enum ProfTag {
     kTagGemmMixed128 = 12,
     kTagGemmSplitK128 = 13,
     kTagGemmSplitK64 = 14,
+    kTagGemmPair64 = 15,
+    kTagGemmPair128 = 16,
 };
 
 bool prof_on();
