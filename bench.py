@@ -526,8 +526,10 @@ def serve_form(np, torch, args):
     serving.Server(serving.DeviceExecutor(), hbm_budget=2 ** 34).run(
         W.serving_jobs(2, 1e6, seed=99, n_range=(16, 20), layer_range=(2, 2), chis=(64,)))
     torch.cuda.synchronize()
+    log(f"serve: {len(jobs)} jobs at {args.serve_rate}/s into the server")
     rep = srv.run(jobs)
     torch.cuda.synchronize()
+    log(f"serve: {rep.site_layers} site-layers served in {rep.makespan:.1f} s; per-chain sample next")
     lat = rep.latency_percentiles()
     pyth = max(abs(r.norm2 + r.discarded - 1.0) for r in rep.results)
     # per-chain form: complete circuits of every third job of the same stream,
@@ -536,6 +538,7 @@ def serve_form(np, torch, args):
     dev = ops.device()
     done, t0, boxed = 0, time.perf_counter(), False
     subset = jobs[::3]
+    last_log = time.perf_counter()
     for j in subset:
         cores = [torch.from_numpy(np.eye(2)[0].reshape(1, 2, 1).copy()).to(dev) for _ in range(j["n"])]
         for layer in range(j["layers"]):
@@ -544,6 +547,9 @@ def serve_form(np, torch, args):
             cores = rounding.apply_and_round(mpo, cores, chi_max=j["chi_max"], cutoff=j["cutoff"])
             torch.cuda.synchronize()
             done += j["n"]
+            if time.perf_counter() - last_log > 30.0:
+                last_log = time.perf_counter()
+                log(f"serve: per-chain sample at {done} site-layers")
             if time.perf_counter() - t0 > args.serve_plain_seconds:
                 boxed = True
                 break
@@ -552,6 +558,7 @@ def serve_form(np, torch, args):
     plain = done / (time.perf_counter() - t0)
     # oracle on the two cheapest jobs (n * layers), full circuits
     from oracle import rounding as ornd
+    log(f"serve: per-chain form {plain:.1f} site-layers/s; oracle check of the two cheapest jobs")
     cheap = sorted(jobs, key=lambda j: j["n"] * j["layers"] * j["chi_max"])[:2]
     srv2 = serving.Server(serving.DeviceExecutor(precision=prec), hbm_budget=2 ** 36, keep_states=True)
     got = {r.id: r for r in srv2.run([dict(j, arrival=0.0) for j in cheap]).results}
