@@ -464,6 +464,7 @@ def batch_form(np, torch, args):
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
         res[prec] = (nsites / min(times), min(times), Zp)
+        log(f"batch: {prec} {nsites / min(times):.1f} sites/s")
     batched, Z = res["fp64"][0], res["fp64"][2]
     # per-chain form on a strided sample of the same jobs
     dev = ops.device()
@@ -483,6 +484,7 @@ def batch_form(np, torch, args):
         if plain_time > args.plain_seconds:
             break
     plain = done_sites / plain_time
+    log(f"batch: per-chain form {plain:.1f} sites/s; oracle check of every 512th job")
     # oracle error on a few jobs (every 512th)
     from oracle import reference as oref
     from oracle import rounding as ornd
