@@ -243,7 +243,7 @@ int syn_colscale_f64(const syn_colscale_item *items, int n_items, int64_t total_
 /*     x_in = rt^T * x_out   with   x_out rows orthogonal.               */
 /* With gram = 1 the input is a symmetric Gram matrix G = C^T C and      */
 /* s[i] = sqrt(||x_i||) are the singular values of C (rt = V^T).         */
-/* No reference call site (north-star addition: SVD rounding).           */
+/* No reference call site (an addition: the SVD form of the rounding).  */
 /* ------------------------------------------------------------------ */
 typedef struct {
     double *x;
@@ -263,7 +263,7 @@ int syn_jacobi_f64(const syn_jacobi_item *items, int n_items, double tol, int ma
 /* eigenvectors (rt = V^T, G = V diag(lambda) V^T) sorted by decreasing    */
 /* eigenvalue and s[i] = sqrt(max(lambda_i, 0)), the singular values of C */
 /* when G = C^T C.  Items outside the contract get s = NaN.  No reference */
-/* call site (north-star addition: SVD rounding, small bonds).           */
+/* call site (an addition: the SVD form of the rounding, small bonds).  */
 int syn_jacobi_gram_f64(const syn_jacobi_item *items, int n_items, int64_t nv_max, double tol, int max_sweeps,
                         void *stream);
 
@@ -308,14 +308,17 @@ int64_t syn_jacobi_blocked_many_workspace(const int64_t *nv, const int64_t *len,
 /* rounding.  Per problem: g (nv x nv, ldg) is diagonalised IN PLACE;      */
 /* on exit rt (nv x nv, ldr) holds the eigenvectors as rows sorted by     */
 /* decreasing eigenvalue (G_in = rt^T diag(lambda) rt) and s[i] =          */
-/* sqrt(max(lambda_i, 0)).  Blocks of 16 or 32 rows; every step is one launch */
-/* of pair solves (LDS, one wave each) and one launch of MFMA tile         */
-/* updates covering all problems still rotating.  `items` is a DEVICE     */
+/* sqrt(max(lambda_i, 0)).  Blocks of 16 or 32 rows; with 16 every step  */
+/* is one launch holding its pair solves (LDS, one work-group each) beside */
+/* the previous step's MFMA tile updates, for all problems still rotating */
+/* (32: a launch of each).  A problem stops after a sweep that rotated    */
+/* nothing or when no off-diagonal entry passes the rotation test at the  */
+/* end of a sweep (the next sweep would rotate nothing).  `items` is a DEVICE */
 /* table of n_items entries and `items_host` the same table on the host;  */
 /* w, solve_start, tile_start, counter and active are filled by the       */
 /* library.  work: syn_jacobi_gram_blocked_workspace(nv, n) doubles;      */
 /* counters: n_items ints; *sweeps (host, may be null) gets the sweeps.   */
-/* No reference call site (north-star addition: SVD rounding).           */
+/* No reference call site (an addition: the SVD form of the rounding).  */
 typedef struct {
     double *g;
     int64_t ldg;

@@ -751,6 +751,34 @@ __global__ __launch_bounds__(256) void tj_init_kernel(const syn_gbj_item *__rest
             __DBL_EPSILON__ * fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// End-of-sweep test: chk[i] = 1 when some off-diagonal entry of problem i's
+// Gram matrix (buffer `cur`: 0 P.g, 1 the workspace copy) passes the
+// rotation test, i.e. when the next sweep would rotate.  A sweep's first
+// step reads its tiles straight from G, and later tiles stay exact copies of
+// G while nothing has rotated, so "no entry passes" means the next sweep
+// rotates nothing.  Both triangles are tested: a pair solve tests G_pq or
+// G_qp depending on its tile order, and G is symmetric only to rounding.
+template <int TB>
+__global__ __launch_bounds__(256) void tj_check_kernel(const syn_gbj_item *__restrict__ items, int cur, double tol,
+                                                       const int32_t *counters, int32_t *chk) {
+    const syn_gbj_item &P = items[blockIdx.x];
+    const int64_t half = tj_dims<TB>(P.nv).half;
+    int64_t ld = P.ldg;
+    const double *G = P.g;
+    if (cur) {
+        ld = P.nv;
+        G = P.w + tj_g2_off<TB>(half, P.nv);
+    }
+    const double floor_abs = P.w[tj_floor_off<TB>(half)];
+    bool hit = false;
+    const int64_t nel = P.nv * P.nv;
+    for (int64_t e = (int64_t)blockIdx.y * 256 + threadIdx.x; e < nel && !hit; e += (int64_t)gridDim.y * 256) {
+        const int64_t i = e / P.nv, k = e - i * P.nv;
+        if (i != k && rotates(G[i * ld + k], G[i * ld + i], G[k * ld + k], tol, floor_abs)) hit = true;
+    }
+    if (hit) chk[P.counter - counters] = 1;
+}
+
 __global__ __launch_bounds__(256) void tj_diag_kernel(const syn_gbj_item *__restrict__ items) {
     const syn_gbj_item &P = items[blockIdx.x];
     for (int64_t i = threadIdx.x; i < P.nv; i += 256) P.s[i] = sqrt(fmax(P.g[i * P.ldg + i], 0.0));
@@ -837,7 +865,10 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, (int32_t *)dbytes, 2);
     SYN_LAUNCH_CHECK("gram byte counter");
 
-    std::vector<int32_t> hc(n_items);
+    // end-of-sweep test flags, in the workspace's spare tail after the byte
+    // counter (at least tot_nv + 15 doubles are left there)
+    int32_t *chk = (int32_t *)(dbytes + 1);
+    std::vector<int32_t> hc(n_items), hk(n_items);
     // fused steps (16-row blocks): buffer holding each problem's Gram matrix
     // (0: P.g, 1: the workspace copy); all problems of a sweep share it
     constexpr bool fused = TB == 16 && SYN_TJ_FUSED;
@@ -887,14 +918,24 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
             }
         }
         SYN_LAUNCH_CHECK("gram step");
+        // a problem goes on when its sweep rotated and the end-of-sweep test
+        // finds an entry that would still rotate: this saves the last sweep,
+        // which only confirmed convergence, with the same result
+        hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, chk, n_items);
+        hipLaunchKernelGGL(tj_check_kernel<TB>, dim3((unsigned)act.size(), 16), dim3(256), 0, st, items,
+                           fused ? cur : 0, tol, counters, chk);
+        SYN_LAUNCH_CHECK("gram sweep test");
         if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,
                                               st),
                                "gram counters"))
             return rc;
+        if (int rc = hip_check(hipMemcpyAsync(hk.data(), chk, n_items * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+                               "gram sweep test flags"))
+            return rc;
         if (int rc = hip_check(hipStreamSynchronize(st), "gram sync")) return rc;
         std::vector<int> next;
         for (int i : idx)
-            if (hc[i] != 0) next.push_back(i);
+            if (hc[i] != 0 && hk[i] != 0) next.push_back(i);
         idx.swap(next);
         if (idx.empty()) {
             ++used;

@@ -1,4 +1,4 @@
-"""SVD rounding of an MPS on the device (the north-star form of ``>>``).
+"""SVD rounding of an MPS on the device (an SVD form of ``>>`` added beside the reference's QR form).
 
 The reference's live ``>>`` is QR truncation (tensor/matrix_product_state.py:432-468);
 this module adds the textbook TT-rounding (Oseledets 2011, Alg. 2):
