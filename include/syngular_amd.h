@@ -460,7 +460,8 @@ int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_
 /* 64/128 tiles, 13/14 split-K GEMM of 128/64 tiles (CholeskyQR Gram       */
 /* matrices and the recursive Cholesky's products, partial-sum pass        */
 /* included), 15/16 two independent GEMMs in one launch of 64/128 tiles   */
-/* (the recursive Cholesky's G22 update beside T = L21 X11).              */
+/* (the recursive Cholesky's G22 update beside T = L21 X11), 17 one      */
+/* recursive-Cholesky node of 64 < n <= 128 (two leaves, three products). */
 /* syn_prof_note_flops sets the count attached to the next grouped GEMM */
 /* or apply launch (their tables live on the device).                    */
 /* syn_prof_select(mask) limits the events to the tags whose bit is set  */

@@ -142,6 +142,7 @@ PROF_TAGS = {
     14: "gemm_f64_splitk<64,64>",
     15: "gemm_f64_pair<64,64>",
     16: "gemm_f64_pair<128,128>",
+    17: "chol_node_kernel",
 }
 
 

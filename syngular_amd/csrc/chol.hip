@@ -58,13 +58,14 @@ __global__ __launch_bounds__(256) void maxdiag_kernel(const double *g, int64_t n
 // L11 (lower, zeros above) into L and Z = inv(L~11) with dropped columns
 // zeroed into zb (nb x nb, ld nb).  L~11 equals L11 with every dropped row
 // replaced by the unit row, so that a panel solve X L11^T = B over the kept
-// columns is X = B Z^T with zero columns at the dropped positions.
+// columns is X = B Z^T with zero columns at the dropped positions.  a and lo
+// are two kNb x (kNb + 1) LDS tiles of the caller (free again on return):
+// the Schur complements, then L and L~ (dropped rows -> unit rows).
 template <int TG>
 __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, double *L, int64_t ldl, int64_t n,
                                                int64_t j0, double *zb, int64_t ldz, const double *maxdiag,
-                                               double tolrel, int *dropped) {
-    __shared__ double a[kNb][kNb + 1];   // Schur complements (lower triangle)
-    __shared__ double lo[kNb][kNb + 1];  // L, then L~ (dropped rows -> unit rows)
+                                               double tolrel, int *dropped, double (*a)[kNb + 1],
+                                               double (*lo)[kNb + 1]) {
     __shared__ int keep[kNb];
     const int nb = (int)((n - j0) < kNb ? (n - j0) : kNb);
     constexpr int RB = kNb / TG, NT = TG * TG;
@@ -289,13 +290,110 @@ __global__ __launch_bounds__(kCholThreads) void chol_diag_kernel(const double *G
                                                                  int64_t ldl, int64_t n, int64_t j0, double *zb,
                                                                  int64_t ldz, const double *maxdiag, double tolrel,
                                                                  int *dropped) {
-    chol_diag_body<kCholTG>(G, ldg, L, ldl, n, j0, zb, ldz, maxdiag, tolrel, dropped);
+    __shared__ double a[kNb][kNb + 1], lo[kNb][kNb + 1];
+    chol_diag_body<kCholTG>(G, ldg, L, ldl, n, j0, zb, ldz, maxdiag, tolrel, dropped, a, lo);
 }
 
 // One leaf (n <= 64) per work-group, for many problems at once.
 __global__ __launch_bounds__(kCholThreads) void chol_leaf_many_kernel(const syn_chol_item *__restrict__ items) {
+    __shared__ double a[kNb][kNb + 1], lo[kNb][kNb + 1];
     const syn_chol_item &P = items[blockIdx.x];
-    chol_diag_body<kCholTG>(P.g, P.ldg, P.l, P.ldl, P.n, 0, P.x, P.ldx, P.maxdiag, P.tolrel, P.dropped);
+    chol_diag_body<kCholTG>(P.g, P.ldg, P.l, P.ldl, P.n, 0, P.x, P.ldx, P.maxdiag, P.tolrel, P.dropped, a, lo);
+}
+
+// C (m x nn, ldc) = alpha op(A) op(B) + beta C for 1 <= m, nn, kk <= 64
+// inside one work-group of kCholThreads: A (m x kk, lda) and op(B) (kk x nn;
+// B^T when bt, B then stored nn x kk) are staged whole into the LDS tiles As
+// and Bs, zero-padded to 64, and wave w computes rows 16w .. 16w + 15 with
+// f64 16x16x4 MFMA.  Every global load of a thread is issued before the first
+// is used (addresses clamped into the operand, padding selected to zero
+// afterwards), so their latencies overlap.  lower: only entries j <= i are
+// written.  The caller's writes of A, B and C must be visible (a work-group
+// barrier); ends with one.
+__device__ __forceinline__ void node_gemm(const double *A, int64_t lda, const double *B, int64_t ldb, bool bt,
+                                          double *C, int64_t ldc, int m, int nn, int kk, double alpha, double beta,
+                                          bool lower, double (*As)[kNb + 1], double (*Bs)[kNb + 1]) {
+    static_assert(kNb == 64 && kCholThreads == 256, "the node products assume 64 x 64 tiles and four waves");
+    constexpr int Q = kNb * kNb / kCholThreads;
+    const int tid = threadIdx.x;
+    // B's stored shape: rows x cols = nn x kk (bt) or kk x nn
+    const int br = bt ? nn : kk, bc = bt ? kk : nn;
+    double va[Q], vb[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = tid + kCholThreads * q, i = e >> 6, k = e & 63;   // consecutive threads: consecutive columns
+        va[q] = A[(int64_t)min(i, m - 1) * lda + min(k, kk - 1)];
+        vb[q] = B[(int64_t)min(i, br - 1) * ldb + min(k, bc - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e = tid + kCholThreads * q, i = e >> 6, k = e & 63;
+        As[i][k] = (i < m && k < kk) ? va[q] : 0.0;
+        const double b = (i < br && k < bc) ? vb[q] : 0.0;
+        if (bt)
+            Bs[k][i] = b;
+        else
+            Bs[i][k] = b;
+    }
+    // MFMA f64 16x16x4 operands: A[i = l%16][k = l/16], B[k = l/16][j = l%16];
+    // D register r of lane l is D[l/16 + 4r][l%16]
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    const int w = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
+    // C's old values (beta != 0) are loaded now, under the products
+    double cv[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * w + lk + 4 * r, j = 16 * t + li;
+            cv[t][r] = (beta != 0.0) ? C[(int64_t)min(i, m - 1) * ldc + min(j, nn - 1)] : 0.0;
+        }
+    __syncthreads();
+    v4d acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    // the padding is zero, so the full 64 of K adds nothing past kk
+#pragma unroll
+    for (int k0 = 0; k0 < kNb; k0 += 4) {
+        const double av = As[16 * w + li][k0 + lk];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Bs[k0 + lk][16 * t + li], acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * w + lk + 4 * r, j = 16 * t + li;
+            if (i < m && j < nn && (!lower || j <= i)) {
+                const double v = alpha * acc[t][r];
+                C[(int64_t)i * ldc + j] = (beta != 0.0) ? fma(beta, cv[t][r], v) : v;
+            }
+        }
+    __syncthreads();
+}
+
+// One node of the recursive Cholesky-inverse with 64 < n <= 128 (split
+// 64 + (n - 64)) in one work-group: the leaf of G11, L21 = G21 X11^T,
+// G22 -= L21 L21^T (lower), T = L21 X11 (T: (n - 64) x 64, ld 64), the leaf of
+// G22 and X21 = -X22 T, the five launches of the recursion at that size.
+// Each stage reads what the one before wrote to global memory after a
+// work-group barrier; the leaves and the products share the two LDS tiles.
+__global__ __launch_bounds__(kCholThreads) void chol_node_kernel(double *G, int64_t ldg, double *L, int64_t ldl,
+                                                                 double *X, int64_t ldx, int64_t n, double *T,
+                                                                 const double *maxdiag, double tolrel,
+                                                                 int *dropped) {
+    __shared__ double a[kNb][kNb + 1], lo[kNb][kNb + 1];
+    const int n2 = (int)(n - kNb);
+    double *G22 = G + kNb * ldg + kNb, *L21 = L + kNb * ldl, *X21 = X + kNb * ldx;
+    chol_diag_body<kCholTG>(G, ldg, L, ldl, kNb, 0, X, ldx, maxdiag, tolrel, dropped, a, lo);
+    __syncthreads();
+    node_gemm(G + kNb * ldg, ldg, X, ldx, true, L21, ldl, n2, kNb, kNb, 1.0, 0.0, false, a, lo);
+    node_gemm(L21, ldl, L21, ldl, true, G22, ldg, n2, n2, kNb, -1.0, 1.0, true, a, lo);
+    node_gemm(L21, ldl, X, ldx, false, T, kNb, n2, kNb, kNb, 1.0, 0.0, false, a, lo);
+    chol_diag_body<kCholTG>(G22, ldg, L21 + kNb, ldl, n2, 0, X21 + kNb, ldx, maxdiag, tolrel, dropped, a, lo);
+    __syncthreads();
+    node_gemm(X21 + kNb, ldx, T, kNb, false, X21, ldx, n2, kNb, n2, -1.0, 0.0, false, a, lo);
 }
 
 __global__ __launch_bounds__(256) void diagmax_many_kernel(const syn_diagmax_item *__restrict__ items) {
@@ -364,6 +462,11 @@ static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, doub
 // ws (ws_cap doubles), when given, lets the products split along K; the
 // CholeskyQR callers pass none: on the 8192 x 4096 chain the split saved
 // under 1% of a step there.
+// nodes with 64 < n <= 128 in one launch (chol_node_kernel)
+#ifndef SYN_CHOL_NODE
+#define SYN_CHOL_NODE 1
+#endif
+
 static int64_t chol_inv_scratch(int64_t n) {
     if (n <= kNb) return 0;
     const int64_t n1 = ((n / kNb + 1) / 2) * kNb, n2 = n - n1;
@@ -378,6 +481,13 @@ static int chol_inv(double *G, int64_t ldg, double *L, int64_t ldl, double *X, i
         hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(kCholThreads), 0, s, G, ldg, L, ldl, n, (int64_t)0, X, ldx, scal,
                            tolrel, dropped);
         SYN_LAUNCH_CHECK("chol_diag");
+        return SYN_OK;
+    }
+    if (SYN_CHOL_NODE && n <= 2 * kNb) {
+        ProfScope ps(kTagCholNode, (double)n * n * n * (2.0 / 3.0), s);
+        hipLaunchKernelGGL(chol_node_kernel, dim3(1), dim3(kCholThreads), 0, s, G, ldg, L, ldl, X, ldx, n, T, scal,
+                           tolrel, dropped);
+        SYN_LAUNCH_CHECK("chol_node");
         return SYN_OK;
     }
     const int64_t n1 = ((n / kNb + 1) / 2) * kNb, n2 = n - n1;

@@ -23,6 +23,7 @@ enum ProfTag {
     kTagGemmSplitK64 = 14,
     kTagGemmPair64 = 15,
     kTagGemmPair128 = 16,
+    kTagCholNode = 17,
 };
 
 bool prof_on();

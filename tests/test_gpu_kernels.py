@@ -313,6 +313,35 @@ def test_cholqr_rank_deficient_drops():
     np.testing.assert_allclose(q[:, kept].T @ q[:, kept], np.eye(40), atol=1e-12)
 
 
+@pytest.mark.parametrize("n,dep", [(65, ()), (100, ()), (128, ()), (100, range(70, 100)),
+                                   (128, tuple(range(40, 52)) + tuple(range(110, 128)))])
+def test_cholqr_node_sizes(n, dep):
+    """Unfoldings of 65..128 columns, where the recursive Cholesky-inverse is
+    one node (two 64-column leaves and the products between them); dep:
+    columns made combinations of earlier ones, which the factorisation drops
+    (in the second leaf, or in both)."""
+    from syngular_amd import ops
+
+    case(f"cholqr node n={n} dropped={len(dep)}")
+    rng = np.random.default_rng(n + len(dep))
+    A0 = rng.standard_normal((400, n)) @ np.diag(np.logspace(0, -2, n))
+    for j in dep:
+        A0[:, j] = A0[:, :j] @ rng.standard_normal(j) / np.sqrt(j)
+    Q, L, dropped = ops.cholqr(T(A0))
+    assert int(dropped.item()) == len(dep)
+    Q2, L2, d2 = ops.cholqr(Q.clone())
+    q = Q2.cpu().numpy()
+    r = (L.cpu().numpy() @ L2.cpu().numpy()).T
+    np.testing.assert_allclose(q @ r, A0, atol=1e-10 * np.abs(A0).max())
+    kept = np.linalg.norm(q, axis=0) > 0.5
+    assert kept.sum() == n - len(dep)
+    np.testing.assert_allclose(q[:, kept].T @ q[:, kept], np.eye(n - len(dep)), atol=1e-12)
+    if not dep:
+        # full rank: the factor matches LAPACK's R up to the signs of its rows
+        _, rn = np.linalg.qr(A0)
+        np.testing.assert_allclose(np.abs(r), np.abs(rn), atol=1e-10 * np.abs(rn).max())
+
+
 @pytest.mark.parametrize("nv,ln,gram", [(17, 40, False), (100, 130, False), (300, 300, True), (512, 512, True)])
 def test_jacobi_blocked(nv, ln, gram):
     from syngular_amd import ops
