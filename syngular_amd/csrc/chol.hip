@@ -26,18 +26,33 @@ namespace syn {
 
 constexpr int kNb = 64;
 
-// compile-time integer and an unrolled loop over 0 .. N-1 handing f the
-// index as a constant
-template <int V>
-struct IntC {
-    static constexpr int value = V;
-};
-template <int N, int I = 0, typename F>
-__device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (I < N) {
-        f(IntC<I>{});
-        static_for<N, I + 1>(f);
-    }
+// v from lane `lane` (uniform) of the wave, as two 32-bit readlanes
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo32 = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi32 = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi32 << 32) | (long long)(unsigned)lo32);
+}
+
+// 1 / x for normal positive x from the hardware estimate (v_rcp_f64) and two
+// Newton-Raphson steps r <- r + r (1 - x r), each squaring the relative
+// error, to within a unit or two in the last place: shorter than the IEEE
+// division sequence on a panel step's critical path
+__device__ __forceinline__ double recip_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Work-group barrier that orders LDS only: global stores issued before it
+// stay in flight.  For phases whose global writes no thread of the
+// work-group reads back (a plain __syncthreads waits for them to land).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __global__ __launch_bounds__(256) void maxdiag_kernel(const double *g, int64_t n, int64_t ld, double *out,
@@ -68,8 +83,8 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
                                                double (*lo)[kNb + 1]) {
     __shared__ int keep[kNb];
     const int nb = (int)((n - j0) < kNb ? (n - j0) : kNb);
-    constexpr int RB = kNb / TG, NT = TG * TG;
-    const int tid = threadIdx.x, tx = tid % TG, ty = tid / TG;
+    constexpr int NT = TG * TG;
+    const int tid = threadIdx.x;
     const double thr = tolrel * (*maxdiag);
     {
         double v[kNb * kNb / NT];
@@ -85,75 +100,71 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
             lo[i][k] = 0.0;
         }
     }
-    __syncthreads();
-    // right-looking with the Schur complement in registers: thread (ty, tx)
-    // owns rows ty+TG*u and columns tx+TG*v (u, v < RB).  Column j is
-    // published through a double-buffered LDS column by its owners, so one
-    // step costs 2*RB broadcast reads, RB*RB register FMAs and one barrier.
-    __shared__ double col[2][kNb];
-    double r[RB][RB];
-#pragma unroll
-    for (int u = 0; u < RB; ++u)
-#pragma unroll
-        for (int v = 0; v < RB; ++v) r[u][v] = a[ty + TG * u][tx + TG * v];
-    if (tid < kNb) col[0][tid] = a[tid][0];
-    __syncthreads();
+    lds_barrier();
+    // Blocked right-looking factorisation in four panels of 16 columns.  Wave
+    // 0 factors a panel in registers (lane r holds row r of the panel, the
+    // pivot and the column entries reach every lane by readlane: no barrier
+    // and no LDS round trip inside a panel); then the four waves subtract
+    // its rank-16 product from the trailing lower tiles of a on the matrix
+    // cores.  lo receives the unscaled columns (pivot on the diagonal) and
+    // rdk the pivots' reciprocals (0 for dropped and padding columns), as the
+    // column-by-column form leaves them.
+    __shared__ double rdk[kNb];
     int ndrop = 0;
-    // the loop keeps the unscaled columns in lo; the square roots and
-    // divisions of L = col / sqrt(pivot) run afterwards for all columns at
-    // once, so a step's critical path holds only the pivot's reciprocal.
-    // Step j with VN = the register column (tx + TG * VN) of column j + 1:
-    auto col_step = [&](int j, auto vn_c) {
-        constexpr int VN = decltype(vn_c)::value;
-        const double *cj = col[j & 1];
-        // every operand of the step is read before the pivot's reciprocal
-        double ci[RB], ck[RB];
-        const double d = cj[j];
-#pragma unroll
-        for (int u = 0; u < RB; ++u) ci[u] = cj[ty + TG * u];
-#pragma unroll
-        for (int v = 0; v < RB; ++v) ck[v] = cj[tx + TG * v];
-        const bool kp = d > thr;
-        ndrop += !kp;
-        const double rd = kp ? 1.0 / d : 0.0;
-#pragma unroll
-        for (int u = 0; u < RB; ++u) ci[u] *= rd;
-        // every register entry is updated: only the trailing lower triangle
-        // (i >= k > j) is read again, and it takes valid operands only (rows
-        // and columns > j of column j); entries above the diagonal or of
-        // eliminated columns hold values that no later step reads
-#pragma unroll
-        for (int u = 0; u < RB; ++u)
-#pragma unroll
-            for (int v = 0; v < RB; ++v) r[u][v] = fma(-ci[u], ck[v], r[u][v]);
-        // publish column j+1 (owners: tx == (j+1) % TG, register column VN)
-        if constexpr (VN < RB) {
-            const int jn = j + 1;
-            if (jn < nb && tx == (jn % TG)) {
-#pragma unroll
-                for (int u = 0; u < RB; ++u) col[jn & 1][ty + TG * u] = r[u][VN];
-            }
-        }
-        if (tid >= j && tid < nb) lo[tid][j] = cj[tid];
-        if (tid == 0) keep[j] = kp;
-        __syncthreads();
-    };
-    // columns in blocks of TG: column j + 1 sits in register column j / TG
-    // except after the last column of a block, so both are compile-time
-    // constants (no selects over the register block)
-    static_for<RB>([&](auto jb_c) {
-        constexpr int JB = decltype(jb_c)::value;
-        // kept rolled: each launch runs the code once, so an unrolled
-        // column sequence would be fetched from L2 instruction by instruction
+    {
+        typedef double v4d __attribute__((ext_vector_type(4)));
+        const int w = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
 #pragma unroll 1
-        for (int jj = 0; jj + 1 < TG; ++jj) {
-            const int j = JB * TG + jj;
-            if (j >= nb) return;   // nb is uniform over the work-group
-            col_step(j, IntC<JB>{});
+        for (int k = 0; 16 * k < nb; ++k) {
+            const int c0 = 16 * k;
+            if (w == 0) {
+                double pv[16];
+#pragma unroll
+                for (int m = 0; m < 16; ++m) pv[m] = a[l][c0 + m];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int jc = c0 + j;
+                    const double d = readlane_f64(pv[j], jc);
+                    const bool valid = jc < nb, kp = valid && d > thr;
+                    ndrop += valid && !kp;
+                    const double rd = kp ? recip_nr(d) : 0.0;
+                    if (valid && l >= jc && l < nb) lo[l][jc] = pv[j];
+                    if (l == 0) {
+                        rdk[jc] = rd;
+                        if (valid) keep[jc] = kp;
+                    }
+                    const double sr = pv[j] * rd;
+#pragma unroll
+                    for (int m = j + 1; m < 16; ++m) pv[m] = fma(-sr, readlane_f64(pv[j], c0 + m), pv[m]);
+                }
+            }
+            lds_barrier();
+            // trailing tiles (I, J), k < J <= I < 4, numbered row by row
+            const int nt = (3 - k) * (4 - k) / 2;
+            for (int t = w; t < nt; t += 4) {
+                int I = k + 1, J = k + 1, q = t;
+                while (q > I - k - 1) {
+                    q -= I - k;
+                    ++I;
+                }
+                J = k + 1 + q;
+                v4d acc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[r] = a[16 * I + lk + 4 * r][16 * J + li];
+#pragma unroll
+                for (int k0 = 0; k0 < 16; k0 += 4) {
+                    const int kc = c0 + k0 + lk;
+                    // A[i][kk] = -s_(16I+i, kc), B[kk][m] = c_(16J+m, kc)
+                    const double av = -(lo[16 * I + li][kc] * rdk[kc]);
+                    const double bv = lo[16 * J + li][kc];
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[16 * I + lk + 4 * r][16 * J + li] = acc[r];
+            }
+            lds_barrier();
         }
-        const int j = JB * TG + TG - 1;
-        if (j < nb) col_step(j, IntC<JB + 1>{});
-    });
+    }
     if (tid == 0 && ndrop) atomicAdd(dropped, ndrop);
     // L_jj = sqrt(d_j), L_ij = col_ij / L_jj for kept pivots, zero columns for
     // dropped ones; 1 / L~_jj for the inversion below (1 on dropped rows)
@@ -165,23 +176,22 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
     } else if (tid < kNb) {
         rdiag[tid] = 1.0;   // rows past nb: zero rows of L~, unit rows of its inverse
     }
-    __syncthreads();
-    for (int e = tid; e < nb * nb; e += NT) {
-        const int i = e / nb, k = e % nb;
-        if (k <= i) lo[i][k] = (i == k) ? sdiag[k] : (keep[k] ? lo[i][k] / sdiag[k] : 0.0);
+    lds_barrier();
+    // L (kept columns scaled by the pivot's reciprocal square root, dropped
+    // ones zero) to global memory, and lo turned into L~ (dropped rows ->
+    // unit rows): each thread rewrites the entries it has just read, so no
+    // barrier separates the two
+#pragma unroll
+    for (int q = 0; q < kNb * kNb / NT; ++q) {
+        const int e = tid + NT * q, i = e >> 6, k = e & 63;
+        if (i < nb && k < nb) {
+            double v = 0.0;
+            if (k <= i) v = (i == k) ? sdiag[k] : (keep[k] ? lo[i][k] * rdiag[k] : 0.0);
+            L[(j0 + i) * ldl + (j0 + k)] = v;
+            if (k <= i) lo[i][k] = keep[i] ? v : (i == k ? 1.0 : 0.0);
+        }
     }
-    __syncthreads();
-    // write L, then turn dropped rows of lo into unit rows (L~)
-    for (int e = tid; e < nb * nb; e += NT) {
-        const int i = e / nb, k = e % nb;
-        L[(j0 + i) * ldl + (j0 + k)] = (k <= i) ? lo[i][k] : 0.0;
-    }
-    __syncthreads();
-    for (int e = tid; e < nb * nb; e += NT) {
-        const int i = e / nb, k = e % nb;
-        if (!keep[i] && k <= i) lo[i][k] = (i == k) ? 1.0 : 0.0;
-    }
-    __syncthreads();
+    lds_barrier();
     // Z = inv(L~) by blocks.  The four waves first invert the four 16 x 16
     // diagonal blocks of L~ (wave w: rows and columns 16w .. 16w + 15) by
     // forward substitution: lane (g, c) holds column c of rows g, g + 4, g + 8,
@@ -211,7 +221,7 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
 #pragma unroll
         for (int u = 0; u < 4; ++u) zs[b0 + g + 4 * u][b0 + c] = zr[u];
     }
-    __syncthreads();
+    lds_barrier();
     // MFMA f64 16x16x4 operands: A[i = l%16][k = l/16], B[k = l/16][j = l%16];
     // D register r of lane l is D[l/16 + 4r][l%16]
     const int li = l & 15, lk = l >> 4;
@@ -230,7 +240,7 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
 #pragma unroll
         for (int r = 0; r < 4; ++r) zs[o + lk + 4 * r][o + 16 + li] = acc[r];
     }
-    __syncthreads();
+    lds_barrier();
     if (w < 2) {
         const int o = 32 * w;
         v4d acc = {0.0, 0.0, 0.0, 0.0};
@@ -244,7 +254,7 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
 #pragma unroll
         for (int r = 0; r < 4; ++r) zs[o + 16 + lk + 4 * r][o + li] = -acc[r];
     }
-    __syncthreads();
+    lds_barrier();
     // 32 -> 64: wave w takes output tile (ti, tj) of the 32 x 32 products
     {
         const int ti = w >> 1, tj = w & 1;
@@ -259,7 +269,7 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
 #pragma unroll
         for (int r = 0; r < 4; ++r) zs[16 * ti + lk + 4 * r][32 + 16 * tj + li] = acc[r];
     }
-    __syncthreads();
+    lds_barrier();
     {
         const int ti = w >> 1, tj = w & 1;
         v4d acc = {0.0, 0.0, 0.0, 0.0};
@@ -273,16 +283,17 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
 #pragma unroll
         for (int r = 0; r < 4; ++r) zs[32 + 16 * ti + lk + 4 * r][16 * tj + li] = -acc[r];
     }
-    __syncthreads();
-    for (int e = tid; e < nb * nb; e += NT) {
-        const int k = e / nb, c = e % nb;
-        zb[k * ldz + c] = (keep[c] && c <= k) ? zs[k][c] : 0.0;
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < kNb * kNb / NT; ++q) {
+        const int e = tid + NT * q, k = e >> 6, c = e & 63;
+        if (k < nb && c < nb) zb[k * ldz + c] = (keep[c] && c <= k) ? zs[k][c] : 0.0;
     }
 }
 
-// 16 x 16 threads with 4 x 4 register blocks: the four waves spread each
-// step's FMAs over the four SIMDs (one wave with 8 x 8 blocks measured twice
-// as slow: the barrier it saves costs less than the serialised FMAs).
+// 16 x 16 threads (four waves): wave 0 factors each 16-column panel, the
+// four waves share the trailing tile updates, the blocked inverse and the
+// loads and stores.
 constexpr int kCholTG = 16;
 constexpr int kCholThreads = kCholTG * kCholTG;
 
