@@ -562,15 +562,25 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
             // V (2TP x TP): columns c < TB are column hA TB + c of W_A on rows
             // 0 .. TP-1, columns c >= TB column hB TB + c - TB of W_B on rows
             // TP .. 2TP-1 (identity columns for a pair that did not rotate)
-            for (int e = tid; e < 2 * TP * TP; e += NT) {
-                const int j = e / TP, c = e % TP;
+            // every W load of a thread is issued before the first is used
+            // (from the tile its column block needs; unused rows load too)
+            constexpr int QV = 2 * TP * TP / NT;
+            double wpre[QV];
+#pragma unroll
+            for (int q = 0; q < QV; ++q) {
+                const int e = tid + NT * q, j = e / TP, c = e % TP;
+                wpre[q] = (c < TB) ? WA[(j % TP) * TP + hA * TB + c] : WB[(j % TP) * TP + hB * TB + c - TB];
+            }
+#pragma unroll
+            for (int q = 0; q < QV; ++q) {
+                const int e = tid + NT * q, j = e / TP, c = e % TP;
                 double v = 0.0;
                 if (c < TB && j < TP) {
                     const int col = hA * TB + c;
-                    v = fA ? WA[j * TP + col] : (j == col ? 1.0 : 0.0);
+                    v = fA ? wpre[q] : (j == col ? 1.0 : 0.0);
                 } else if (c >= TB && j >= TP) {
                     const int col = hB * TB + c - TB, jj = j - TP;
-                    v = fB ? WB[jj * TP + col] : (jj == col ? 1.0 : 0.0);
+                    v = fB ? wpre[q] : (jj == col ? 1.0 : 0.0);
                 }
                 V[j][c] = v;
             }
@@ -640,17 +650,30 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
             rb[tid] = tj_row<TB>(st, b, D.np, D.nblk, P.nv, tid);
         }
         __syncthreads();
-        for (int e = tid; e < TP * TP; e += NT) {
-            const int i = e / TP, k = e % TP, r = ra[i], c = rb[k];
-            X[i][k] = (r >= 0 && c >= 0) ? Gc[(int64_t)r * ldc + c] : 0.0;
+        // the G tile and the two W tiles are loaded before the first value
+        // is used (padding rows from row 0, selected to zero afterwards)
+        constexpr int QX = TP * TP / NT;
+        const double *wa = wset + a * TP * TP, *wb = wset + b * TP * TP;
+        double xv[QX], wva[QX], wvb[QX];
+#pragma unroll
+        for (int q = 0; q < QX; ++q) {
+            const int e = tid + NT * q, i = e / TP, k = e % TP;
+            xv[q] = Gc[(int64_t)max(ra[i], 0) * ldc + max(rb[k], 0)];
+            wva[q] = wa[e];
+            wvb[q] = wb[e];
+        }
+#pragma unroll
+        for (int q = 0; q < QX; ++q) {
+            const int e = tid + NT * q, i = e / TP, k = e % TP;
+            X[i][k] = (ra[i] >= 0 && rb[k] >= 0) ? xv[q] : 0.0;
         }
         if (fa || fb) {
             if (tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);
-            const double *wa = wset + a * TP * TP, *wb = wset + b * TP * TP;
-            for (int e = tid; e < TP * TP; e += NT) {
-                const int i = e / TP, k = e % TP;
-                Wa[i][k] = fa ? wa[e] : (i == k ? 1.0 : 0.0);
-                Wb[i][k] = fb ? wb[e] : (i == k ? 1.0 : 0.0);
+#pragma unroll
+            for (int q = 0; q < QX; ++q) {
+                const int e = tid + NT * q, i = e / TP, k = e % TP;
+                Wa[i][k] = fa ? wva[q] : (i == k ? 1.0 : 0.0);
+                Wb[i][k] = fb ? wvb[q] : (i == k ? 1.0 : 0.0);
             }
             __syncthreads();
             {   // T = Wa^T X, one 16 x 16 tile per wave
@@ -694,11 +717,27 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
     if (tid == 0) atomicAdd(bytes, kSlabBytes);
     if (tid < TP) ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
     __syncthreads();
+    // W and the rt slab are loaded before the first value is used (padding
+    // rows and columns from row 0 / column nv - 1, selected to zero)
     const double *wa = wset + a * TP * TP;
-    for (int e = tid; e < TP * TP; e += NT) Wa[e / TP][e % TP] = wa[e];
-    for (int e = tid; e < TP * kTCol; e += NT) {
-        const int i = e / kTCol, j = e % kTCol, r = ra[i];
-        X[i][j] = (r >= 0 && c0 + j < P.nv) ? P.rt[(int64_t)r * P.ldr + c0 + j] : 0.0;
+    constexpr int QW = TP * TP / NT, QS = TP * kTCol / NT;
+    double wq[QW], sv[QS];
+#pragma unroll
+    for (int q = 0; q < QW; ++q) wq[q] = wa[tid + NT * q];
+#pragma unroll
+    for (int q = 0; q < QS; ++q) {
+        const int e = tid + NT * q, i = e / kTCol, j = e % kTCol;
+        sv[q] = P.rt[(int64_t)max(ra[i], 0) * P.ldr + min(c0 + j, P.nv - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < QW; ++q) {
+        const int e = tid + NT * q;
+        Wa[e / TP][e % TP] = wq[q];
+    }
+#pragma unroll
+    for (int q = 0; q < QS; ++q) {
+        const int e = tid + NT * q, i = e / kTCol, j = e % kTCol;
+        X[i][j] = (ra[i] >= 0 && c0 + j < P.nv) ? sv[q] : 0.0;
     }
     __syncthreads();
     for (int q = wv; q < (TP / 16) * (kTCol / 16); q += NT / 64) {
