@@ -79,7 +79,31 @@ struct RrPlane {
     }
     __device__ __forceinline__ int p() const { return a < b ? a : b; }
     __device__ __forceinline__ int q() const { return a < b ? b : a; }
+    static constexpr int kRounds = TP - 1;
 };
+
+// Plane k of the cross-block order over a pair tile: p = k in the first
+// block and, in round r, q = TP/2 + (k + r) mod TP/2 in the second, so the
+// TP/2 rounds meet every (first block, second block) plane once and no
+// plane within a block.
+template <int TP>
+struct XPlane {
+    int a, b;
+    __device__ __forceinline__ void start(int k) {
+        a = k;
+        b = TP / 2 + k;
+    }
+    __device__ __forceinline__ void next() { b = b + 1 == TP ? TP / 2 : b + 1; }
+    __device__ __forceinline__ int p() const { return a; }
+    __device__ __forceinline__ int q() const { return b; }
+    static constexpr int kRounds = TP / 2;
+};
+
+// Steps after the first of each round-robin pass run the cross-block order
+// only (0: every step runs the full round-robin sweep of its pair tile)
+#ifndef SYN_TJ_CROSS
+#define SYN_TJ_CROSS 1
+#endif
 
 // Rotation test of plane (p, q): |G_pq| > max(tol sqrt(|G_pp G_qq|), floor),
 // in squares (no square root on the critical path of a round).
@@ -211,7 +235,7 @@ template <int TB>
 __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDims &D, int64_t pw,
                                               double (*S)[2 * TB + 1], double (*W)[2 * TB + 1], v2d *cs,
                                               int &do_rot, int &any_rot, double tol, int inner_sweeps, double *Wout,
-                                              int *flags) {
+                                              int *flags, bool cross = false) {
     constexpr int TP = 2 * TB, NT = TB * TB;
     const int tid = threadIdx.x;
     for (int e = tid; e < TP * TP; e += NT) W[e / TP][e % TP] = (e / TP == e % TP) ? 1.0 : 0.0;
@@ -220,7 +244,8 @@ __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDim
     const double floor_abs = P.w[tj_floor_off<TB>(D.half)];
     for (int e = tid; e < TP * TP; e += NT) {
         const int i = e / TP, k = e % TP;
-        if (i < k && rotates(S[i][k], S[i][i], S[k][k], tol, floor_abs)) do_rot = 1;
+        if (i < k && (!cross || (i < TB && k >= TB)) && rotates(S[i][k], S[i][i], S[k][k], tol, floor_abs))
+            do_rot = 1;
     }
     __syncthreads();
     if (!do_rot) {
@@ -235,77 +260,86 @@ __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDim
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
         if (tid == 0) any_rot = 0;
         __syncthreads();
-        RrPlane<TP> pa, pb, pw2[2];
-        pa.start(ka);
-        pb.start(kb);
-        pw2[0].start(wk);
-        pw2[1].start(wk + TP / 4);
-        for (int r = 0; r < TP - 1; ++r) {
-            if (r) {
-                pa.next();
-                pb.next();
-                pw2[0].next();
-                pw2[1].next();
-            }
-            const int p = pa.p(), q = pa.q(), u = pb.p(), v = pb.q();
-            // the rotation's operands are read first (their wave waits for
-            // them alone); this round's update operands were final at the last
-            // barrier and phase 1 only reads S, so they are loaded next and
-            // their latency overlaps the rotation
-            const bool rot_thread = tid < TP / 2;   // here ka == 0 and kb == tid: plane tid is (u, v)
-            double apq = 0.0, app = 0.0, aqq = 0.0;
-            if (rot_thread) {
-                apq = S[u][v];
-                app = S[u][u];
-                aqq = S[v][v];
-            }
-            const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
-            int wp[2], wq[2];
-            double wx[2], wy[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                wp[h] = pw2[h].p();
-                wq[h] = pw2[h].q();
-                wx[h] = W[wi][wp[h]];
-                wy[h] = W[wi][wq[h]];
-            }
-            if (rot_thread) {
-                double c = 1.0, sn = 0.0;
-                if (rotates(apq, app, aqq, tol, floor_abs)) {
-                    rotation(apq, app, aqq, c, sn);
-                    any_rot = 1;
+        // one pass of the plane order Pl (round-robin: every plane of the
+        // tile; cross-block: the planes between the two blocks)
+        auto rounds = [&](auto tag) {
+            using Pl = decltype(tag);
+            Pl pa, pb, pw2[2];
+            pa.start(ka);
+            pb.start(kb);
+            pw2[0].start(wk);
+            pw2[1].start(wk + TP / 4);
+            for (int r = 0; r < Pl::kRounds; ++r) {
+                if (r) {
+                    pa.next();
+                    pb.next();
+                    pw2[0].next();
+                    pw2[1].next();
                 }
-                cs[tid] = v2d{c, sn};
-            }
-            __syncthreads();
-            // every rotation this thread applies, read in one batch of 16-byte
-            // loads (one LDS round trip instead of one per use)
-            const v2d csa = cs[ka], csb = cs[kb], csw[2] = {cs[wk], cs[wk + TP / 4]};
-            {
-                const double ca = csa.x, sa = csa.y, cb = csb.x, sb = csb.y;
-                if (sa != 0.0 || sb != 0.0) {
-                    const double rpu = ca * spu - sa * squ, rpv = ca * spv - sa * sqv;
-                    const double rqu = sa * spu + ca * squ, rqv = sa * spv + ca * sqv;
-                    S[p][u] = cb * rpu - sb * rpv;
-                    S[p][v] = sb * rpu + cb * rpv;
-                    S[q][u] = cb * rqu - sb * rqv;
-                    S[q][v] = sb * rqu + cb * rqv;
-                    if (ka == kb) {  // the annihilated entry of this plane
-                        S[p][q] = 0.0;
-                        S[q][p] = 0.0;
+                const int p = pa.p(), q = pa.q(), u = pb.p(), v = pb.q();
+                // the rotation's operands are read first (their wave waits for
+                // them alone); this round's update operands were final at the last
+                // barrier and phase 1 only reads S, so they are loaded next and
+                // their latency overlaps the rotation
+                const bool rot_thread = tid < TP / 2;   // here ka == 0 and kb == tid: plane tid is (u, v)
+                double apq = 0.0, app = 0.0, aqq = 0.0;
+                if (rot_thread) {
+                    apq = S[u][v];
+                    app = S[u][u];
+                    aqq = S[v][v];
+                }
+                const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
+                int wp[2], wq[2];
+                double wx[2], wy[2];
+    #pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    wp[h] = pw2[h].p();
+                    wq[h] = pw2[h].q();
+                    wx[h] = W[wi][wp[h]];
+                    wy[h] = W[wi][wq[h]];
+                }
+                if (rot_thread) {
+                    double c = 1.0, sn = 0.0;
+                    if (rotates(apq, app, aqq, tol, floor_abs)) {
+                        rotation(apq, app, aqq, c, sn);
+                        any_rot = 1;
+                    }
+                    cs[tid] = v2d{c, sn};
+                }
+                __syncthreads();
+                // every rotation this thread applies, read in one batch of 16-byte
+                // loads (one LDS round trip instead of one per use)
+                const v2d csa = cs[ka], csb = cs[kb], csw[2] = {cs[wk], cs[wk + TP / 4]};
+                {
+                    const double ca = csa.x, sa = csa.y, cb = csb.x, sb = csb.y;
+                    if (sa != 0.0 || sb != 0.0) {
+                        const double rpu = ca * spu - sa * squ, rpv = ca * spv - sa * sqv;
+                        const double rqu = sa * spu + ca * squ, rqv = sa * spv + ca * sqv;
+                        S[p][u] = cb * rpu - sb * rpv;
+                        S[p][v] = sb * rpu + cb * rpv;
+                        S[q][u] = cb * rqu - sb * rqv;
+                        S[q][v] = sb * rqu + cb * rqv;
+                        if (ka == kb) {  // the annihilated entry of this plane
+                            S[p][q] = 0.0;
+                            S[q][p] = 0.0;
+                        }
                     }
                 }
-            }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {  // W <- W J (columns of plane wk + h TP/4)
-                const double c = csw[h].x, sn = csw[h].y;
-                if (sn != 0.0) {
-                    W[wi][wp[h]] = c * wx[h] - sn * wy[h];
-                    W[wi][wq[h]] = sn * wx[h] + c * wy[h];
+    #pragma unroll
+                for (int h = 0; h < 2; ++h) {  // W <- W J (columns of plane wk + h TP/4)
+                    const double c = csw[h].x, sn = csw[h].y;
+                    if (sn != 0.0) {
+                        W[wi][wp[h]] = c * wx[h] - sn * wy[h];
+                        W[wi][wq[h]] = sn * wx[h] + c * wy[h];
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
-        }
+        };
+        if (cross)
+            rounds(XPlane<TP>{});
+        else
+            rounds(RrPlane<TP>{});
         // any_rot was last written before the final barrier of the sweep
         const int any = any_rot;
         __syncthreads();
@@ -616,8 +650,12 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
             }
         }
         const int ps = (int)(step & 1);
+        // the first step of a pass meets every block and treats the planes
+        // within blocks; the others only the planes between their two blocks,
+        // so a pass treats every plane of G once
         tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, inner_sweeps,
-                          tj_wset<TB>(P, D.half, ps) + pw * TP * TP, tj_flags<TB>(P, D.half, ps));
+                          tj_wset<TB>(P, D.half, ps) + pw * TP * TP, tj_flags<TB>(P, D.half, ps),
+                          SYN_TJ_CROSS && st != 0);
         return;
     }
     if (!has_update) return;
