@@ -39,13 +39,17 @@ constexpr int kTCol = 64;            // columns of an rt slab
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
 
+// Blocks (a, b) of pair w at a step of the round-robin order over np blocks.
+// 32-bit arithmetic (a 64-bit remainder is a long instruction sequence):
+// step < np - 1 and w < np / 2 at every call, and np is far below 2^31.
 __device__ __forceinline__ void tj_pair(int64_t step, int64_t w, int64_t np, int64_t &a, int64_t &b) {
-    if (w == 0) {
-        a = step;
-        b = np - 1;
+    const int s = (int)step, ww = (int)w, m = (int)np - 1;
+    if (ww == 0) {
+        a = s;
+        b = m;
     } else {
-        a = (step + w) % (np - 1);
-        b = (step - w + np - 1) % (np - 1);
+        a = (s + ww) % m;
+        b = (s - ww + m) % m;
     }
 }
 
@@ -490,21 +494,22 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
 
 // Pair of step `step` - 1 holding block x, and the half (0: first 16 rows,
 // 1: second) it sits in: the inverse of tj_pair's round-robin map.
+// 32-bit arithmetic as in tj_pair (x, sp < np, np far below 2^31).
 __device__ __forceinline__ void tj_owner(int64_t x, int64_t sp, int64_t np, int64_t half, int64_t &w, int &h) {
-    const int64_t np1 = np - 1;
-    if (x == np - 1) {
+    const int xi = (int)x, si = (int)sp, np1 = (int)np - 1;
+    if (xi == np1) {
         w = 0;
         h = 1;
-    } else if (x == sp) {
+    } else if (xi == si) {
         w = 0;
         h = 0;
     } else {
-        const int64_t w1 = ((x - sp) % np1 + np1) % np1;
-        if (w1 >= 1 && w1 <= half - 1) {
+        const int w1 = ((xi - si) % np1 + np1) % np1;
+        if (w1 >= 1 && w1 <= (int)half - 1) {
             w = w1;
             h = 0;
         } else {
-            w = ((sp - x) % np1 + np1) % np1;
+            w = ((si - xi) % np1 + np1) % np1;
             h = 1;
         }
     }
@@ -572,8 +577,14 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
             tj_owner(bb, sp, D.np, D.half, wB, hB);
             // the old rows are computed by every thread for its own loads (no
             // barrier before the first global reads)
-            auto orow_of = [&](int k) -> int {
-                return k < TP ? tj_row<TB>(sp, wA, D.np, D.nblk, P.nv, k) : tj_row<TB>(sp, wB, D.np, D.nblk, P.nv, k - TP);
+            int64_t oa[2], ob[2];   // blocks of the old pairs wA and wB
+            tj_pair(sp, wA, D.np, oa[0], ob[0]);
+            tj_pair(sp, wB, D.np, oa[1], ob[1]);
+            auto orow_of = [&](int k) -> int {   // tj_row of pair wA (k < TP) or wB
+                const int h = k >= TP, i = k - h * TP;
+                const int64_t blk = i < TB ? oa[h] : ob[h];
+                const int64_t r = blk * TB + (i % TB);
+                return (blk < D.nblk && r < P.nv) ? (int)r : -1;
             };
             // G_old operands of this wave's two Y tiles, loaded first so their
             // latency overlaps the V build below
