@@ -26,14 +26,6 @@ namespace syn {
 
 constexpr int kNb = 64;
 
-// v from lane `lane` (uniform) of the wave, as two 32-bit readlanes
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo32 = __builtin_amdgcn_readlane((int)b, lane);
-    const int hi32 = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi32 << 32) | (long long)(unsigned)lo32);
-}
-
 // 1 / x for normal positive x from the hardware estimate (v_rcp_f64) and two
 // Newton-Raphson steps r <- r + r (1 - x r), each squaring the relative
 // error, to within a unit or two in the last place: shorter than the IEEE
@@ -103,13 +95,13 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
     lds_barrier();
     // Blocked right-looking factorisation in four panels of 16 columns.  Wave
     // 0 factors a panel in registers (lane r holds row r of the panel, the
-    // pivot and the column entries reach every lane by readlane: no barrier
-    // and no LDS round trip inside a panel); then the four waves subtract
+    // pivot and the column entries reach every lane as LDS broadcasts: no
+    // work-group barrier inside a panel); then the four waves subtract
     // its rank-16 product from the trailing lower tiles of a on the matrix
     // cores.  lo receives the unscaled columns (pivot on the diagonal) and
     // rdk the pivots' reciprocals (0 for dropped and padding columns), as the
     // column-by-column form leaves them.
-    __shared__ double rdk[kNb];
+    __shared__ double rdk[kNb], colb[kNb];
     int ndrop = 0;
     {
         typedef double v4d __attribute__((ext_vector_type(4)));
@@ -124,7 +116,13 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     const int jc = c0 + j;
-                    const double d = readlane_f64(pv[j], jc);
+                    // column j reaches every lane through LDS (one wave: its
+                    // LDS operations run in order, no barrier needed)
+                    colb[l] = pv[j];
+                    const double d = colb[jc];
+                    double cm[16];
+#pragma unroll
+                    for (int m = j + 1; m < 16; ++m) cm[m] = colb[c0 + m];
                     const bool valid = jc < nb, kp = valid && d > thr;
                     ndrop += valid && !kp;
                     const double rd = kp ? recip_nr(d) : 0.0;
@@ -135,7 +133,7 @@ __device__ __forceinline__ void chol_diag_body(const double *G, int64_t ldg, dou
                     }
                     const double sr = pv[j] * rd;
 #pragma unroll
-                    for (int m = j + 1; m < 16; ++m) pv[m] = fma(-sr, readlane_f64(pv[j], c0 + m), pv[m]);
+                    for (int m = j + 1; m < 16; ++m) pv[m] = fma(-sr, cm[m], pv[m]);
                 }
             }
             lds_barrier();

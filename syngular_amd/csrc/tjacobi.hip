@@ -1010,7 +1010,7 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
         // finds an entry that would still rotate: this saves the last sweep,
         // which only confirmed convergence, with the same result
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, chk, n_items);
-        hipLaunchKernelGGL(tj_check_kernel<TB>, dim3((unsigned)act.size(), 16), dim3(256), 0, st, items,
+        hipLaunchKernelGGL(tj_check_kernel<TB>, dim3((unsigned)act.size(), 64), dim3(256), 0, st, items,
                            fused ? cur : 0, tol, counters, chk);
         SYN_LAUNCH_CHECK("gram sweep test");
         if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,
