@@ -86,23 +86,6 @@ struct RrPlane {
     static constexpr int kRounds = TP - 1;
 };
 
-// Plane k of the cross-block order over a pair tile: p = k in the first
-// block and, in round r, q = TP/2 + (k + r) mod TP/2 in the second, so the
-// TP/2 rounds meet every (first block, second block) plane once and no
-// plane within a block.
-template <int TP>
-struct XPlane {
-    int a, b;
-    __device__ __forceinline__ void start(int k) {
-        a = k;
-        b = TP / 2 + k;
-    }
-    __device__ __forceinline__ void next() { b = b + 1 == TP ? TP / 2 : b + 1; }
-    __device__ __forceinline__ int p() const { return a; }
-    __device__ __forceinline__ int q() const { return b; }
-    static constexpr int kRounds = TP / 2;
-};
-
 // Steps after the first of each round-robin pass run the cross-block order
 // only (0: every step runs the full round-robin sweep of its pair tile)
 #ifndef SYN_TJ_CROSS
@@ -264,15 +247,15 @@ __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDim
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
         if (tid == 0) any_rot = 0;
         __syncthreads();
-        // one pass of the plane order Pl (round-robin: every plane of the
-        // tile; cross-block: the planes between the two blocks)
-        auto rounds = [&](auto tag) {
+        // one pass of the round-robin plane order Pl over every plane of the tile
+        auto rounds = [&](auto tag) __attribute__((always_inline)) {
             using Pl = decltype(tag);
             Pl pa, pb, pw2[2];
             pa.start(ka);
             pb.start(kb);
             pw2[0].start(wk);
             pw2[1].start(wk + TP / 4);
+#pragma unroll 1
             for (int r = 0; r < Pl::kRounds; ++r) {
                 if (r) {
                     pa.next();
@@ -295,7 +278,7 @@ __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDim
                 const double spu = S[p][u], spv = S[p][v], squ = S[q][u], sqv = S[q][v];
                 int wp[2], wq[2];
                 double wx[2], wy[2];
-    #pragma unroll
+#pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     wp[h] = pw2[h].p();
                     wq[h] = pw2[h].q();
@@ -329,7 +312,7 @@ __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDim
                         }
                     }
                 }
-    #pragma unroll
+#pragma unroll
                 for (int h = 0; h < 2; ++h) {  // W <- W J (columns of plane wk + h TP/4)
                     const double c = csw[h].x, sn = csw[h].y;
                     if (sn != 0.0) {
@@ -340,8 +323,75 @@ __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDim
                 __syncthreads();
             }
         };
+        // The cross-block pass (TB rounds): plane k joins p = k of the first
+        // block with q = TB + (k + r) mod TB of the second in round r, so the
+        // rounds meet every (first block, second block) plane once.  Here p =
+        // ka, u = kb and the W columns wk, wk + TB/2 stay fixed: S[ka][kb] and
+        // W[wi][wk], W[wi][wk + TB/2] are touched by this thread alone and live
+        // in registers for the pass (the diagonal owners also store theirs to
+        // LDS each round, where the rotation threads read it); written back at
+        // the end.
+        auto cross_rounds = [&]() __attribute__((always_inline)) {
+            constexpr int M = TB - 1;
+            const bool rot_thread = tid < TB;   // ka == 0, kb == tid: plane tid
+            double spu = S[ka][kb];
+            double wx0 = W[wi][wk], wx1 = W[wi][wk + TB / 2];
+#pragma unroll 1
+            for (int r = 0; r < TB; ++r) {
+                const int q = TB + ((ka + r) & M), v = TB + ((kb + r) & M);
+                const int wq0 = TB + ((wk + r) & M), wq1 = TB + ((wk + TB / 2 + r) & M);
+                double apq = 0.0, app = 0.0, aqq = 0.0;
+                if (rot_thread) {
+                    apq = S[kb][v];
+                    app = S[kb][kb];
+                    aqq = S[v][v];
+                }
+                const double spv = S[ka][v], squ = S[q][kb], sqv = S[q][v];
+                const double wy0 = W[wi][wq0], wy1 = W[wi][wq1];
+                if (rot_thread) {
+                    double c = 1.0, sn = 0.0;
+                    if (rotates(apq, app, aqq, tol, floor_abs)) {
+                        rotation(apq, app, aqq, c, sn);
+                        any_rot = 1;
+                    }
+                    cs[tid] = v2d{c, sn};
+                }
+                __syncthreads();
+                const v2d csa = cs[ka], csb = cs[kb], csw0 = cs[wk], csw1 = cs[wk + TB / 2];
+                {
+                    const double ca = csa.x, sa = csa.y, cb = csb.x, sb = csb.y;
+                    if (sa != 0.0 || sb != 0.0) {
+                        const double rpu = ca * spu - sa * squ, rpv = ca * spv - sa * sqv;
+                        const double rqu = sa * spu + ca * squ, rqv = sa * spv + ca * sqv;
+                        spu = cb * rpu - sb * rpv;
+                        S[ka][v] = sb * rpu + cb * rpv;
+                        S[q][kb] = cb * rqu - sb * rqv;
+                        S[q][v] = sb * rqu + cb * rqv;
+                        if (ka == kb) {  // the diagonal entry, and the annihilated one
+                            S[ka][kb] = spu;
+                            S[ka][q] = 0.0;
+                            S[q][ka] = 0.0;
+                        }
+                    }
+                }
+                if (csw0.y != 0.0) {  // W <- W J (columns of planes wk, wk + TB/2)
+                    const double x = wx0;
+                    wx0 = csw0.x * x - csw0.y * wy0;
+                    W[wi][wq0] = csw0.y * x + csw0.x * wy0;
+                }
+                if (csw1.y != 0.0) {
+                    const double x = wx1;
+                    wx1 = csw1.x * x - csw1.y * wy1;
+                    W[wi][wq1] = csw1.y * x + csw1.x * wy1;
+                }
+                __syncthreads();
+            }
+            S[ka][kb] = spu;
+            W[wi][wk] = wx0;
+            W[wi][wk + TB / 2] = wx1;
+        };
         if (cross)
-            rounds(XPlane<TP>{});
+            cross_rounds();
         else
             rounds(RrPlane<TP>{});
         // any_rot was last written before the final barrier of the sweep
@@ -577,28 +627,33 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
             tj_owner(bb, sp, D.np, D.half, wB, hB);
             // the old rows are computed by every thread for its own loads (no
             // barrier before the first global reads)
-            int64_t oa[2], ob[2];   // blocks of the old pairs wA and wB
-            tj_pair(sp, wA, D.np, oa[0], ob[0]);
-            tj_pair(sp, wB, D.np, oa[1], ob[1]);
-            auto orow_of = [&](int k) -> int {   // tj_row of pair wA (k < TP) or wB
-                const int h = k >= TP, i = k - h * TP;
-                const int64_t blk = i < TB ? oa[h] : ob[h];
-                const int64_t r = blk * TB + (i % TB);
-                return (blk < D.nblk && r < P.nv) ? (int)r : -1;
-            };
-            // G_old operands of this wave's two Y tiles, loaded first so their
-            // latency overlaps the V build below
+            int64_t oa0, ob0, oa1, ob1;   // blocks of the old pairs wA and wB
+            tj_pair(sp, wA, D.np, oa0, ob0);
+            tj_pair(sp, wB, D.np, oa1, ob1);
+            // G_old operands of this wave's two Y tiles (tile row h: the rows of
+            // old pair h, wA or wB; tile column wv: entries 16 wv .. 16 wv + 15
+            // of the two old pairs' 2 TP), loaded first so their latency
+            // overlaps the V build below; padding loads row or column 0 and is
+            // selected to zero afterwards
+            const int rc = tj_row<TB>(sp, wv >= 2 ? wB : wA, D.np, D.nblk, P.nv, 16 * (wv & 1) + li);
+            int rkv[2][TP / 4];
             double gpre[2][TP / 4];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int q = wv + 4 * h, tr = q >> 2, tc = q & 3;
-                const int rc = orow_of(16 * tc + li);
 #pragma unroll
                 for (int k0 = 0; k0 < TP; k0 += 4) {
-                    const int rk = orow_of(tr * TP + k0 + lk);
-                    gpre[h][k0 / 4] = (rk >= 0 && rc >= 0) ? Gc[(int64_t)rk * ldc + rc] : 0.0;
+                    const int64_t blk = k0 < TB ? (h ? oa1 : oa0) : (h ? ob1 : ob0);
+                    const int64_t r = blk * TB + ((k0 + lk) % TB);
+                    const int rk = (blk < D.nblk && r < P.nv) ? (int)r : -1;
+                    rkv[h][k0 / 4] = rk;
+                    gpre[h][k0 / 4] = Gc[(int64_t)max(rk, 0) * ldc + max(rc, 0)];
                 }
             }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int k0 = 0; k0 < TP; k0 += 4)
+                    if (rkv[h][k0 / 4] < 0 || rc < 0) gpre[h][k0 / 4] = 0.0;
             const int pp = (int)((step - 1) & 1);
             const int *fl = tj_flags<TB>(P, D.half, pp);
             const int fA = fl[wA], fB = fl[wB];
