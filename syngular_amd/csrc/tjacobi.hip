@@ -7,20 +7,26 @@
 //
 //   solve   one small work-group per pair: the kTP x kTP diagonal tile of the
 //           pair (rows and columns of both blocks) is loaded into LDS and
-//           rotated by cyclic two-sided Jacobi (the scheme of the LDS Gram
-//           kernel), accumulating the orthogonal W of the pair;
+//           rotated by two-sided Jacobi, accumulating the orthogonal W of
+//           the pair: a full round-robin sweep of the tile at the first step
+//           of a pass (which meets every block), only the planes between
+//           the pair's two blocks at the others, so a pass treats every
+//           plane of G once;
 //   update  one work-group per tile: G_ab <- W_a^T G_ab W_b for every pair of
 //           pairs (a, b) and rt_a <- W_a^T rt_a for every 64-column slab of
 //           the accumulated eigenvector rows, on the f64 MFMA.
 //
 // Both launches of a step spread over many work-groups (the update over
 // (g/kTP)^2 + (g/kTP)(g/64) tiles), and several problems share every launch.
-// A pair whose tile has no |G_ik| > max(tol sqrt(|G_ii G_kk|), eps max_j G_jj)
-// is not rotated; the sweep loop stops after a sweep in which no pair was
-// rotated, so on exit every off-diagonal entry of G meets that test, G is
-// diagonal to it and G_in = rt^T diag(G_out) rt.  This is the two-sided
-// block Jacobi method (Hansen; Van Loan; Becka, Oksa & Vajtersic) in its
-// parallel round-robin form.
+// With 16-row blocks a step is one launch: the solves of step s beside the
+// updates of step s - 1.  A pair whose tile has no |G_ik| > max(tol
+// sqrt(|G_ii G_kk|), eps max_j G_jj) among the planes it treats is not
+// rotated; a problem stops after a pass that rotated nothing or when no
+// off-diagonal entry of G passes that test at the end of a pass, so on exit
+// every off-diagonal entry of G meets it, G is diagonal to it and
+// G_in = rt^T diag(G_out) rt.  This is the two-sided block Jacobi method
+// (Hansen; Van Loan; Becka, Oksa & Vajtersic) in its parallel round-robin
+// form, with the planes in the cyclic-by-blocks order.
 #include "common.hpp"
 #include "prof.hpp"
 
