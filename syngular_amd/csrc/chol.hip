@@ -568,10 +568,10 @@ __global__ __launch_bounds__(256) void near_identity_chol_kernel(const double *g
 }  // namespace syn
 
 extern "C" int64_t syn_cholqr_workspace(int64_t n) {
-    // G (n x n), X = inv(L) (n x n), the recursion scratch, scalars, and two
-    // n x n partial products of the split-K Gram
+    // G (n x n), X = inv(L) (n x n), the recursion scratch, scalars, and
+    // three n x n partial products of the split-K Gram (up to four K slices)
     const int64_t nblk = (n + syn::kNb - 1) / syn::kNb;
-    return 4 * n * n + syn::chol_inv_scratch(n) + nblk * syn::kNb * syn::kNb + 16;
+    return 5 * n * n + syn::chol_inv_scratch(n) + nblk * syn::kNb * syn::kNb + 16;
 }
 
 extern "C" int syn_potrf_f64(double *G, int64_t n, int64_t ldg, double *L, int64_t ldl, double *work, double tolrel,
@@ -604,7 +604,7 @@ extern "C" int syn_cholqr_f64(double *A, int64_t m, int64_t n, int64_t lda, doub
     // G = A^T A, lower triangle only (the Cholesky reads nothing else)
     double *splitws = scal + 16;
     int rc = gemm_f64_split_launch(make_gemm(A, lda, 1, A, lda, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,
-                                   2 * n * n);
+                                   3 * n * n);
     if (rc) return rc;
     if ((rc = zero(L, n, n, ldl, s))) return rc;
     if ((rc = zero(X, n, n, n, s))) return rc;
@@ -659,7 +659,7 @@ extern "C" int syn_cholqr_refine_f64(const double *Q1, int64_t m, int64_t n, int
     // G = Q1^T Q1 (lower triangle) and max |G - I| on it
     double *splitws = scal + 16;
     int rc = gemm_f64_split_launch(make_gemm(Q1, ldq1, 1, Q1, ldq1, 0, G, n, n, n, m, 1.0, 0.0, 1), s, splitws,
-                                   2 * n * n);
+                                   3 * n * n);
     if (rc) return rc;
     if ((rc = hip_check(hipMemsetAsync(scal + 1, 0, sizeof(double), s), "refine emax reset"))) return rc;
     {
