@@ -1,39 +1,42 @@
 #!/usr/bin/env python3
 """apply+round benchmark of the matrix-product core on one MI355X.
 
-Headline workload (one "step", the JSON `value`): the MPO x MPS product of
-an N=32-site chain (d = 2, MPS bond chi = 256, MPO bond D_w = 16, product bond
-4096) followed by TT-rounding back to chi = 256 (left CholeskyQR2 sweep,
-right Gram + Jacobi SVD sweep), plain FP64, inputs resident in HBM.
-Value = sites/s = 32 * steps / elapsed.
+The metric is apply+round sites/s of the mixed form against the plain FP64
+form, with p50/p99 latency, at a stated relative error.  One JSON line:
 
-Beside the JSON contract fields the line carries:
-  roofline      the work kernel with the largest share of GPU time (from an
-                untimed census step in which every kernel launch is timed by
-                HIP events), its algorithmic rate in the timed steps (events
-                recorded by the library on the launch stream around its
-                launches only) against the MI355X peak;
-  pipeline      the census step's per-kernel launches, milliseconds and
-                rates;
-  cpu_baseline  the NumPy FP64 oracle on a bounded sample of the same chain
-                (its last three sites), timed on this host;
-  parity        errors against the oracle on a reduced chain and the
-                Pythagoras identity at full size;
-  forms         chain_mixed: the same chain in the mixed form (products on
-                       the FP32 matrix cores; Gram matrices, Cholesky, Jacobi
-                       and truncation in FP64) with its error vs the oracle;
-                the many-chain workloads run by the grouped engine:
-                batch: 4096 independent chains (N ~ U[8,48], chi ~ U{8..128})
-                       with one layer of Haar 2-site gates each, rounded back
-                       to chi, all in lock-step grouped launches (FP64 and
-                       mixed);
-                serve: a Poisson stream of brickwork-circuit jobs (N ~
-                       U[16,128], layers ~ U[4,32], chi_max ~ U{64..512},
-                       cutoff 1e-10) under the continuous-batching server
-                       (FP64 by default, --serve-precision mixed);
-                each beside the per-chain FP64 form (one chain at a time
-                through the same kernels) on the same jobs, with p50/p99
-                latency and the error against the oracle.
+Headline (`value`, one "step"): the batching workload -- 4096 independent
+MPS (N ~ U[8,48], d = 2, chi ~ U{8,16,32,64,128}, Gaussian N(0, 1/chi) cores
+drawn on the device) each through one layer of Haar-random real 2-site gates
+on the pairs (0,1), (2,3), ... applied as an MPO of bond 4, then TT-rounded
+back to its own chi, all chains in lock-step grouped launches, in the mixed
+form (contractions on the FP32 matrix cores; Gram matrices, Cholesky, Jacobi
+rotations and truncation in FP64), inputs resident in HBM.
+value = sites of the batch * steps / elapsed.  Beside it:
+  plain_fp64_sites_per_s  the plain FP64 per-chain form (one chain at a time
+                          through the same kernels) on a strided sample of the
+                          same jobs; speedup_vs_plain_fp64 = value / that;
+  batched_fp64_sites_per_s the grouped form in plain FP64;
+  error_vs_fp64_oracle    norms and overlaps <Z|X> of checked jobs against
+                          the NumPy FP64 oracle (relative to |<Z|X>| and to
+                          ||Z|| ||X||), for the mixed and the FP64 batch;
+  roofline                the work kernel with the largest share of GPU time
+                          (census step with every launch timed by HIP events),
+                          its algorithmic rate over the timed steps;
+  cpu_baseline            the oracle on a bounded subsample of the jobs.
+forms:
+  serve_cfg4    a Poisson stream of brickwork-circuit jobs (N ~ U[16,128],
+                layers ~ U[4,32], chi_max ~ U{64..512}, cutoff 1e-10, from
+                |0...0>) under the continuous-batching server, against the
+                per-chain form on the same jobs, p50/p99 latency, oracle
+                errors, and a sequential oracle replay on the host;
+  chain_cfg1    the chain N=32, d=2, chi=256, D_w=16 (product bond 4096) in
+                plain FP64 and mixed, its own roofline, parity and host
+                baseline;
+  precision_cfg2 the chain N=64, d=4, chi=1024, D_w=8 (product bond 8192),
+                cutoff 1e-8: one plain FP64 and one mixed step, the mixed
+                error against plain, full-size identities, and the plain form
+                against the oracle on a reduced chain;
+  api_paths     the drop-in paths beside the headline against the oracle.
 """
 from __future__ import annotations
 
@@ -46,7 +49,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"fp64_mfma_tflops": 78.6, "hbm_gbs": 8000.0}
+PEAKS = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
+METRIC = "apply+round sites/s (mixed vs plain FP64) + p50/p99 latency at stated rel err"
+FORMS = ("batch", "serve", "chain", "precision", "api")
 
 
 def log(msg):
@@ -56,35 +61,31 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed steps of the headline batch")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--sites", type=int, default=32)
-    ap.add_argument("--chi", type=int, default=256)
-    ap.add_argument("--bond-mpo", type=int, default=16)
-    ap.add_argument("--phys", type=int, default=2)
     ap.add_argument("--batch-jobs", type=int, default=4096)
+    ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--plain-seconds", type=float, default=15.0,
+                    help="time box of the per-chain plain FP64 sample of the batch")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="time box of each host oracle baseline")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="fp64",
                     help="arithmetic of the served form (the per-chain comparison is always plain FP64); "
-                         "the mixed form gains no speed on this Jacobi-bound workload and its FP32 "
-                         "accumulation grows the overlap error over long circuits")
-    ap.add_argument("--plain-seconds", type=float, default=20.0,
-                    help="time budget of the per-chain comparison of the batch form")
-    ap.add_argument("--serve-plain-seconds", type=float, default=60.0,
-                    help="time budget of the per-chain comparison of the served form")
+                         "FP32 products add ~1e-7 per layer, which long circuits accumulate past 1e-6")
+    ap.add_argument("--serve-plain-seconds", type=float, default=30.0,
+                    help="time box of the per-chain comparison of the served form")
+    ap.add_argument("--chain-steps", type=int, default=5, help="timed steps of the cfg1 chain form")
+    ap.add_argument("--only", choices=FORMS, help="run one form only (profiling)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="run the headline batch eagerly (no HIP graph for its apply and left sweep)")
+    for f in FORMS[1:]:
+        ap.add_argument(f"--no-{f}", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-api", action="store_true", help="skip the timings of the drop-in API paths")
-    ap.add_argument("--precision-form", action="store_true",
-                    help="only the precision workload (N=64, d=4, chi=1024, D_w=8, cutoff 1e-8): one plain FP64 "
-                         "step and one mixed step, their times and the mixed form's error (a few minutes)")
-    ap.add_argument("--chain-only", action="store_true", help="only the headline chain (for profiling)")
+    ap.add_argument("--precision-form", action="store_true", help="same as --only precision")
     ap.add_argument("--cholqr-probe", action="store_true",
                     help="only the copy probe and one CholeskyQR pass on an 8192 x 4096 unfolding (the chain's "
-                         "left-sweep shape, whose GEMMs are the dominant kernel), for short counter runs")
+                         "left-sweep shape), for short counter runs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (syn_copy_f64) three times: the known-size "
                          "reference that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run")
@@ -98,45 +99,264 @@ def main():
         print("bench.py measures one GPU only: syngular_amd has no multi-GPU path "
               f"(asked for --gpus {args.gpus}, WORLD_SIZE={world}).", file=sys.stderr)
         sys.exit(2)
-    if args.chain_only:
-        args.no_batch = args.no_serve = args.no_cpu_baseline = args.no_parity = args.no_api = True
+    if args.precision_form:
+        args.only = "precision"
 
     import numpy as np
     import torch
 
-    from syngular_amd import linalg, ops, rounding
-    from syngular_amd import workloads as W
+    from syngular_amd import ops
 
-    dev = ops.device()
+    ops.device()
     if args.copy_probe or args.cholqr_probe:
         copy_probe(np, torch)
-    if args.precision_form:
-        res = precision_form(np, torch)
-        print(json.dumps({"precision_form": res}), flush=True)
-        return
     if args.cholqr_probe:
-        A = torch.randn(8192, 4096, dtype=torch.float64, device=dev) / 64.0
+        A = torch.randn(8192, 4096, dtype=torch.float64, device=ops.device()) / 64.0
         ops.cholqr(A)
         torch.cuda.synchronize()
         log("cholqr probe: one CholeskyQR pass on 8192 x 4096")
         return
-    n, d, chi, dw = args.sites, args.phys, args.chi, args.bond_mpo
-    xs, os_ = W.chain_inputs(n, d, chi, dw, seed=2024)
-    X = [torch.from_numpy(x).to(dev) for x in xs]
-    O = [torch.from_numpy(o).to(dev) for o in os_]
-    log(f"workload: N={n} d={d} chi={chi} D_w={dw} (product bond {chi * dw}), fp64")
 
-    def step(report=None):
-        return rounding.apply_and_round(O, X, chi_max=chi, cutoff=0.0, report=report)
+    def want(f):
+        return args.only == f if args.only else not getattr(args, f"no_{f}", False)
 
+    head = batch_headline(np, torch, args) if want("batch") else None
+    forms = {}
+    if want("serve"):
+        forms["serve_cfg4"] = serve_form(np, torch, args)
+        log(f"serve form: {forms['serve_cfg4']}")
+    if want("chain"):
+        forms["chain_cfg1"] = chain_form(np, torch, args)
+        log(f"chain form: {forms['chain_cfg1']}")
+    if want("precision"):
+        forms["precision_cfg2"] = precision_form(np, torch)
+        log(f"precision form: {forms['precision_cfg2']}")
+    if want("api"):
+        forms["api_paths"] = api_form(np, torch)
+        log(f"api paths: {forms['api_paths']}")
+    if head is None:
+        print(json.dumps({"forms": forms}), flush=True)
+        return
+    line = {
+        "metric": METRIC,
+        "value": head.pop("value"),
+        "unit": "sites/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head.pop("ms_per_step"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 contractions + f64 Gram/Cholesky/Jacobi/truncation (mixed)",
+        "data": "synthetic (seeded Gaussian cores drawn on the device, Haar-random orthogonal 2-site gates)",
+        "config": head.pop("config"),
+    }
+    line.update(head)
+    served = forms.get("serve_cfg4")
+    if served:
+        line["served_cfg4"] = {k: served[k] for k in ("sites_per_s", "plain_fp64_per_chain_sites_per_s",
+                                                     "speedup_vs_plain", "p50_latency_s", "p99_latency_s",
+                                                     "precision", "norm2_rel_err_max", "overlap_rel_err_max")}
+    line["forms"] = forms or None
+    print(json.dumps(line), flush=True)
+
+
+# --------------------------------------------------------------------------
+# headline: the batching workload in the mixed form
+# --------------------------------------------------------------------------
+def batch_headline(np, torch, args):
+    from oracle import reference as oref
+    from oracle import rounding as ornd
+    from syngular_amd import autotune, ops, rounding
+    from syngular_amd import batch as B
+    from syngular_amd import workloads as W
+
+    t0 = time.perf_counter()
+    ns, chis, gates = W.batching_layout(args.batch_jobs, seed=args.batch_seed)
+    X = B.ChainBatch.gaussian(ns, chis, 2, seed=args.batch_seed)
+    Op = B.gate_layer(ns, np.zeros(len(ns), np.int64), gates)
+    torch.cuda.synchronize()
+    nsites = int(ns.sum())
+    goff = np.concatenate([[0], np.cumsum([W.npairs(int(v), 0) for v in ns])])
+    log(f"batch: {len(ns)} chains, {nsites} sites, {X.hbm_bytes() / 2 ** 30:.1f} GiB of cores, "
+        f"built in {time.perf_counter() - t0:.1f} s")
+
+    def step(prec="mixed"):
+        return B.apply_and_round(X, Op, chis, precision=prec)
+
+    # tile autotuning over the GEMM shape classes this (chi, d, D_w) mix
+    # produces: one recording step, then both tiles timed per class
+    t1 = time.perf_counter()
+    autotune.collect()
+    step()
+    step("fp64")
+    torch.cuda.synchronize()
+    autotune.tune()
+    tuned = autotune.export()
+    log(f"batch autotune: {len(tuned)} shape classes in {time.perf_counter() - t1:.1f} s, "
+        f"{sum(e['tile'] == '128x128' for e in tuned)} take 128x128")
     for i in range(args.warmup):
-        t0 = time.perf_counter()
+        t1 = time.perf_counter()
         step()
         torch.cuda.synchronize()
-        log(f"warmup {i}: {time.perf_counter() - t0:.3f} s")
+        log(f"batch warmup {i}: {time.perf_counter() - t1:.3f} s")
+    torch.cuda.synchronize()
+    ops.profile_begin()
+    t1 = time.perf_counter()
+    Z = step()
+    torch.cuda.synchronize()
+    census_ms = 1e3 * (time.perf_counter() - t1)
+    census = ops.profile_end()
+    dom = dominant_work_kernel(census)
+    log(f"batch census step (all kernels timed): {census_ms:.0f} ms; dominant work kernel {dom}")
+    del Z
+    # the apply and the left sweep (no host reads in the mixed form) run as
+    # one captured HIP graph per step, the right sweep eagerly
+    cap = None
+    if not args.no_graph:
+        t1 = time.perf_counter()
+        cap = B.CapturedApplyLeft(X, Op, mixed=True)
+        torch.cuda.synchronize()
+        log(f"batch: apply + left sweep captured in a HIP graph ({len(cap.staging)} table copies) in "
+            f"{time.perf_counter() - t1:.1f} s")
 
-    # census step (untimed): every instrumented kernel bracketed by events,
-    # for the per-kernel breakdown and the choice of the dominant work kernel
+    def timed_step():
+        return cap.round(chis) if cap is not None else step()
+
+    timed_step()
+    torch.cuda.synchronize()
+    ops.profile_begin([dom])
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        Z = timed_step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    timed = ops.profile_end()
+    value = nsites * args.steps / elapsed
+    ms = 1e3 * elapsed / args.steps
+    log(f"batch mixed: {args.steps} steps in {elapsed:.2f} s, {ms:.0f} ms/step, {value:.0f} sites/s")
+    measured_in = "the timed steps"
+    if dom not in timed:
+        # the dominant kernel runs inside the captured graph, where launches
+        # carry no events: time it over as many eager steps instead
+        ops.profile_begin([dom])
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        timed = ops.profile_end()
+        measured_in = f"{args.steps} eager steps (inside the timed steps it runs in the captured graph)"
+    roof = kernel_roofline(census, timed, dom, args.steps)
+    roof["measured_in"] = measured_in
+    kept = [Z.bond[j, 1:int(ns[j])] for j in range(len(ns))]
+    textbook = sum(W.apply_round_flops([1] + [int(chis[j])] * (int(ns[j]) - 1) + [1], _gate_bonds(int(ns[j])), 2,
+                                       list(kept[j])) for j in range(len(ns)))
+    roof["textbook_tflops_per_step"] = round(textbook / (ms * 1e-3) / 1e12, 3)
+    roof["textbook_flops_per_step"] = textbook
+    log(f"batch dominant kernel {roof['kernel']}: {roof['achieved']} {roof['unit']} ({roof['frac']:.1%}); "
+        f"textbook {roof['textbook_tflops_per_step']} TFLOP/s")
+    Zm = Z
+    # the grouped form in plain FP64
+    t1 = time.perf_counter()
+    Zp = step("fp64")
+    torch.cuda.synchronize()
+    fp64_rate = nsites / (time.perf_counter() - t1)
+    log(f"batch fp64: {fp64_rate:.0f} sites/s")
+
+    def job_inputs(j):
+        xs = X.to_numpy([j])[0]
+        return xs, W.gate_mpo_cores(int(ns[j]), 0, gates[goff[j]:goff[j + 1]])
+
+    # per-chain plain FP64 form on a strided sample of the same jobs
+    dev = ops.device()
+    done, ptime, used = 0, 0.0, []
+    for j in range(0, len(ns), max(1, len(ns) // 256)):
+        xs, os_ = job_inputs(j)
+        Xs = [torch.from_numpy(c).to(dev) for c in xs]
+        Os = [torch.from_numpy(c).to(dev) for c in os_]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rounding.apply_and_round(Os, Xs, chi_max=int(chis[j]))
+        torch.cuda.synchronize()
+        ptime += time.perf_counter() - t1
+        done += int(ns[j])
+        used.append(j)
+        if ptime > args.plain_seconds:
+            break
+    plain = done / ptime
+    log(f"batch: per-chain plain FP64 form {plain:.0f} sites/s on {len(used)} jobs; oracle check next")
+    # oracle errors on every 512th job
+    check = list(range(0, len(ns), 512))[:8]
+    errs = {"mixed": [], "fp64": []}
+    outs = {"mixed": Zm.to_numpy(check), "fp64": Zp.to_numpy(check)}
+    for q, j in enumerate(check):
+        xs, os_ = job_inputs(j)
+        ref = ornd.apply_and_round(os_, xs, chi_max=int(chis[j]))[0]
+        n2r, ovr, x2 = ornd.norm2(ref), oref.overlap(ref, xs), ornd.norm2(xs)
+        for prec in errs:
+            c = outs[prec][q]
+            ov = oref.overlap(c, xs)
+            errs[prec].append((abs(ornd.norm2(c) - n2r) / n2r, abs(ov - ovr) / abs(ovr),
+                               abs(ov - ovr) / np.sqrt(n2r * x2)))
+    err = {prec: {"jobs": len(v), "norm2_rel_err_max": max(e[0] for e in v),
+                  "overlap_rel_err_max": max(e[1] for e in v),
+                  "overlap_err_over_norms_max": max(e[2] for e in v)} for prec, v in errs.items()}
+    log(f"batch oracle errors: {err}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
+        cdone, t1, cj = 0, time.perf_counter(), []
+        for j in range(7, len(ns), 97):
+            xs, os_ = job_inputs(j)
+            ornd.apply_and_round(os_, xs, chi_max=int(chis[j]))
+            cdone += int(ns[j])
+            cj.append(j)
+            if time.perf_counter() - t1 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t1
+        cpu = {"value": round(cdone / dt, 2), "unit": "sites/s", "cores": int(threads), "kind": "port",
+               "sample": f"NumPy FP64 oracle, apply + TT-rounding of {len(cj)} jobs of the same batch (every 97th "
+                         f"from job 7, {cdone} sites), one after another, {dt:.1f} s"}
+        log(f"batch cpu baseline: {cpu}")
+    return {"value": round(value, 1), "ms_per_step": round(ms, 1),
+            "config": {"workload": f"batching: {len(ns)} independent MPS, N~U[8,48], d=2, chi~U{{8,16,32,64,128}}, "
+                                   f"one layer of Haar O(4) 2-site gates (MPO bond 4) on pairs (0,1),(2,3),..., "
+                                   f"rounded back to chi; grouped lock-step launches, mixed precision",
+                       "jobs": len(ns), "sites": nsites, "phys_dim": 2, "mpo_bond": 4, "form": "batched mixed",
+                       "hip_graph": "apply + left sweep captured, right sweep eager" if cap is not None else "none"},
+            "plain_fp64_sites_per_s": round(plain, 1),
+            "plain_sample": f"{done} sites of {len(used)} strided jobs, one chain at a time, plain FP64",
+            "speedup_vs_plain_fp64": round(value / plain, 2),
+            "batched_fp64_sites_per_s": round(fp64_rate, 1),
+            "error_vs_fp64_oracle": err,
+            "roofline": roof, "cpu_baseline": cpu, "autotune": tuned,
+            "pipeline": {"census_step_ms": round(census_ms, 1), "census_kernels": census_table(census)}}
+
+
+def _gate_bonds(n):
+    """MPO bonds of one gate layer on the pairs (0,1), (2,3), ...: 4 inside a
+    pair, 1 across."""
+    return [1] + [4 if k % 2 == 1 and k < n - (n % 2) else 1 for k in range(1, n)] + [1]
+
+
+# --------------------------------------------------------------------------
+# cfg1 chain form
+# --------------------------------------------------------------------------
+def chain_form(np, torch, args):
+    from syngular_amd import linalg, ops, rounding
+    from syngular_amd import workloads as W
+
+    n, d, chi, dw = 32, 2, 256, 16
+    xs, os_ = W.chain_inputs(n, d, chi, dw, seed=2024)
+    dev = ops.device()
+    X = [torch.from_numpy(x).to(dev) for x in xs]
+    O = [torch.from_numpy(o).to(dev) for o in os_]
+    log(f"chain: N={n} d={d} chi={chi} D_w={dw} (product bond {chi * dw}), fp64")
+
+    def step(report=None, precision="fp64"):
+        return rounding.apply_and_round(O, X, chi_max=chi, cutoff=0.0, report=report, precision=precision)
+
+    step()
     torch.cuda.synchronize()
     ops.profile_begin()
     t0 = time.perf_counter()
@@ -145,78 +365,33 @@ def main():
     census_ms = 1e3 * (time.perf_counter() - t0)
     census = ops.profile_end()
     dom = dominant_work_kernel(census)
-    log(f"census step (all kernels timed by events): {census_ms:.1f} ms; dominant work kernel {dom}")
-
-    # timed steps: events around the dominant kernel's launches only (two
-    # event packets per launch would slow the thousands of short Jacobi and
-    # Cholesky launches of a step by a measurable amount)
-    torch.cuda.synchronize()
     ops.profile_begin([dom])
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for _ in range(args.chain_steps):
         rep = rounding.RoundingReport()
         Z = step(rep)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     timed = ops.profile_end()
-    ms_per_step = 1e3 * elapsed / args.steps
-    sites_per_s = n * args.steps / elapsed
-    log(f"timed {args.steps} steps: {elapsed:.3f} s, {ms_per_step:.1f} ms/step, {sites_per_s:.2f} sites/s")
-
-    roof, pipeline = roofline(census, timed, dom, args.steps, ms_per_step, census_ms, W, n, chi, dw, d, rep)
-    log(f"dominant kernel {roof['kernel']}: {roof['achieved']} {roof['unit']} ({roof['frac']:.1%} of peak)")
-
-    parity = None
-    if not args.no_parity:
-        parity = parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep)
-        log(f"parity: {parity}")
+    ms = 1e3 * elapsed / args.chain_steps
+    rate = n * args.chain_steps / elapsed
+    log(f"chain fp64: {ms:.1f} ms/step, {rate:.2f} sites/s")
+    roof, pipeline = roofline(census, timed, dom, args.chain_steps, ms, census_ms, W, n, chi, dw, d, rep)
+    parity = parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep)
+    log(f"chain parity: {parity}")
     del Z, rep
-
-    forms = {}
-    if not args.chain_only:
-        forms["chain_mixed"] = chain_mixed_form(np, torch, args, O, X, sites_per_s)
-        log(f"chain mixed form: {forms['chain_mixed']}")
-    if not args.no_batch:
-        forms["batch"] = batch_form(np, torch, args)
-        log(f"batch form: {forms['batch']}")
-    if not args.no_serve:
-        forms["serve"] = serve_form(np, torch, args)
-        log(f"serve form: {forms['serve']}")
-    if not args.no_api:
-        forms["api_paths"] = api_form(np, torch)
-        log(f"api paths: {forms['api_paths']}")
-
-    cpu = None
+    out = {"workload": f"chain N={n} d={d} chi={chi} D_w={dw} (product bond {chi * dw}), round to {chi}, plain FP64",
+           "sites_per_s": round(rate, 3), "ms_per_step": round(ms, 2), "steps": args.chain_steps,
+           "roofline": roof, "parity": parity, "pipeline": pipeline}
+    out["mixed"] = chain_mixed_form(np, torch, args, O, X, rate)
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(np, xs, os_, chi)
-        log(f"cpu baseline: {cpu}")
-
-    line = {
-        "metric": "apply+round sites/s (mixed vs plain FP64) + p50/p99 latency at stated rel err",
-        "value": round(sites_per_s, 3),
-        "unit": "sites/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded Gaussian cores, Haar-random orthogonal 2-site gates)",
-        "config": {"workload": f"MPO x MPS apply + SVD rounding, chain N={n} d={d} chi={chi} D_w={dw} "
-                               f"(product bond {chi * dw}), plain FP64",
-                   "form": "plain-fp64", "sites": n, "phys_dim": d, "chi": chi, "mpo_bond": dw,
-                   "round_to": chi, "cutoff": 0.0},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "parity": parity,
-        "pipeline": pipeline,
-        "forms": forms or None,
-    }
-    print(json.dumps(line), flush=True)
+        out["cpu_baseline"] = cpu_baseline(np, xs, os_, chi)
+    return out
 
 
+# --------------------------------------------------------------------------
+# shared measurement helpers
+# --------------------------------------------------------------------------
 # Kernels bound by latency (serial rotation rounds in LDS, 64-column
 # Cholesky leaves, one-work-group factorisations): no flop or byte count
 # describes them, so they get no roofline; bench.py names the largest of
@@ -232,12 +407,17 @@ def dominant_work_kernel(prof):
     return max(work.items(), key=lambda kv: kv[1][1])[0]
 
 
-def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d, rep):
-    """Dominant work kernel of the chain (the largest share of GPU time among
-    kernels with an algorithmic work count, from the census step where every
-    kernel is timed): algorithmic work (flops for the GEMMs, bytes for the
-    memory-bound kernels) over its event time in the timed steps, where
-    only its launches carry events."""
+def _peak(kernel):
+    """MI355X dense peak of the unit a work kernel runs on: the FP32 matrix
+    cores for the mixed GEMMs, the FP64 matrix cores for the rest."""
+    return PEAKS["fp32_mfma_tflops"] if "mixed" in kernel else PEAKS["fp64_mfma_tflops"]
+
+
+def kernel_roofline(census, timed, dom, steps):
+    """Roofline entry of the dominant work kernel: algorithmic work (flops
+    for the GEMMs, bytes for the memory-bound kernels) over its event time in
+    the timed steps, where only its launches carry events; plus its share of
+    the census step's GPU time and the largest latency-bound kernel."""
     total_ms = sum(v[1] for v in census.values())
     dl, dms, dwork = timed[dom]
     if dom in BYTE_KERNELS:
@@ -245,8 +425,7 @@ def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
     else:
         achieved = dwork / (dms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAKS["fp64_mfma_tflops"],
-                "unit": "TFLOP/s"}
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": _peak(dom), "unit": "TFLOP/s"}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
     roof["traffic"] = measured_traffic(dom)
     roof["kernel"] = dom
@@ -261,15 +440,24 @@ def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d
                                               "launches_per_step": ll,
                                               "avg_launch_us": round(1e3 * lms / ll, 2),
                                               "bound": "latency (serial rounds in LDS; no flop/byte roofline)"}
-    kept = list(rep.kept)
-    textbook = W.apply_round_flops(*W.chain_shapes(n, chi, dw), d, kept)
+    return roof
+
+
+def census_table(census):
+    return {k: {"launches": v[0], "ms": round(v[1], 2),
+                "rate": round(v[2] / (v[1] * 1e-3) / (1e9 if k in BYTE_KERNELS else 1e12), 3),
+                "rate_unit": "GB/s" if k in BYTE_KERNELS else "TFLOP/s"}
+            for k, v in sorted(census.items(), key=lambda kv: -kv[1][1])}
+
+
+def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d, rep):
+    """The chain's roofline entry (kernel_roofline) and its pipeline: the
+    textbook flops of the step and the census step's per-kernel table."""
+    roof = kernel_roofline(census, timed, dom, steps)
+    textbook = W.apply_round_flops(*W.chain_shapes(n, chi, dw), d, list(rep.kept))
     pipeline = {"textbook_flops_per_step": textbook,
                 "textbook_tflops": round(textbook / (ms_per_step * 1e-3) / 1e12, 3),
-                "census_step_ms": round(census_ms, 1),
-                "census_kernels": {k: {"launches": v[0], "ms": round(v[1], 2),
-                                       "rate": round(v[2] / (v[1] * 1e-3) / (1e9 if k in BYTE_KERNELS else 1e12), 3),
-                                       "rate_unit": "GB/s" if k in BYTE_KERNELS else "TFLOP/s"}
-                                   for k, v in sorted(census.items(), key=lambda kv: -kv[1][1])}}
+                "census_step_ms": round(census_ms, 1), "census_kernels": census_table(census)}
     return roof, pipeline
 
 
@@ -293,11 +481,29 @@ def measured_traffic(kernel):
 BYTE_KERNELS = ("apply_kernel", "gram_update_kernel")
 
 
+def sval_errors(np, got, sv, kept):
+    """Retained singular values per bond against the oracle's: the largest
+    per-value error |ds_i| / s_i and the largest |ds_i| / s_max."""
+    per, rel_max = 0.0, 0.0
+    for k in range(1, len(sv)):
+        g = got[k - 1].cpu().numpy()[: kept[k]]
+        w = sv[k][: kept[k]]
+        per = max(per, float(np.max(np.abs(g - w) / w)))
+        rel_max = max(rel_max, float(np.max(np.abs(g - w)) / w[0]))
+    return {"svals_per_value_rel_err": per, "svals_err_over_smax": rel_max}
+
+
+def overlap_errors(np, ov, ovr, n2r, x2):
+    """|<Z|X> - <Z|X>_ref| relative to |<Z|X>_ref| and to ||Z|| ||X||."""
+    return {"overlap_rel_err": abs(ov - ovr) / abs(ovr), "overlap_err_over_norms": abs(ov - ovr) / np.sqrt(n2r * x2)}
+
+
 def parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep):
     """(1) Reduced chain (N=12, d=2, chi=64, D_w=8) against the NumPy oracle:
-    retained singular values per bond (max |ds| / s_max), final norm and the
-    overlap <Z|X> (relative to ||Z|| ||X||).  (2) Full size, GPU only:
-    Pythagoras ||OX||^2 - ||Z||^2 = sum_k w_k (relative to ||OX||^2)."""
+    retained singular values per bond (value by value, and relative to
+    s_max), final norm and the overlap <Z|X> (relative to |<Z|X>| and to
+    ||Z|| ||X||).  (2) Full size, GPU only: Pythagoras ||OX||^2 - ||Z||^2 =
+    sum_k w_k (relative to ||OX||^2)."""
     from oracle import reference as oref
     from oracle import rounding as ornd
 
@@ -309,18 +515,14 @@ def parity_checks(np, torch, ops, linalg, rounding, W, O, X, Z, rep):
     r = rounding.RoundingReport()
     cores = rounding.apply_and_round(Os, Xs, chi_max=64, cutoff=0.0, report=r)
     ref, sv, kept, disc = ornd.apply_and_round(os_, xs, chi_max=64)
-    err_s = 0.0
-    for k in range(1, 12):
-        g = r.svals[k - 1].cpu().numpy()[: kept[k]]
-        err_s = max(err_s, float(np.max(np.abs(g - sv[k][: kept[k]])) / sv[k][0]))
     n2 = float(linalg.overlap(cores, cores).item())
     n2r = ornd.norm2(ref)
     ov = float(linalg.overlap(cores, Xs).item())
     ovr = oref.overlap(ref, xs)
     out["reduced_chain"] = "N=12 d=2 chi=64 D_w=8"
-    out["svals_max_rel_err"] = err_s
+    out.update(sval_errors(np, r.svals, sv, kept))
     out["norm2_rel_err"] = abs(n2 - n2r) / n2r
-    out["overlap_rel_err"] = abs(ov - ovr) / np.sqrt(n2r * ornd.norm2(xs))
+    out.update(overlap_errors(np, ov, ovr, n2r, ornd.norm2(xs)))
     out["kept_match"] = r.kept == kept[1:]
     # full size, GPU only
     P = ops.apply_many(list(zip(O, X)))
@@ -342,11 +544,11 @@ def chain_mixed_form(np, torch, args, O, X, plain_sites_per_s):
     from syngular_amd import workloads as W
 
     n = len(X)
-    rounding.apply_and_round(O, X, chi_max=args.chi, precision="mixed")
+    rounding.apply_and_round(O, X, chi_max=256, precision="mixed")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rounding.apply_and_round(O, X, chi_max=args.chi, precision="mixed")
+    for _ in range(args.chain_steps):
+        rounding.apply_and_round(O, X, chi_max=256, precision="mixed")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     xs, os_ = W.chain_inputs(12, 2, 64, 8, seed=11)
@@ -358,14 +560,13 @@ def chain_mixed_form(np, torch, args, O, X, plain_sites_per_s):
     ref, sv, kept, disc = ornd.apply_and_round(os_, xs, chi_max=64)
     n2, n2r = float(linalg.overlap(cores, cores).item()), ornd.norm2(ref)
     ov, ovr = float(linalg.overlap(cores, Xs).item()), oref.overlap(ref, xs)
-    err_s = max(float(np.max(np.abs(r.svals[k - 1].cpu().numpy()[: kept[k]] - sv[k][: kept[k]])) / sv[k][0])
-                for k in range(1, 12))
-    value = n * args.steps / dt
-    return {"sites_per_s": round(value, 3), "ms_per_step": round(1e3 * dt / args.steps, 2),
-            "speedup_vs_plain": round(value / plain_sites_per_s, 3),
-            "error_vs_fp64_oracle": {"reduced_chain": "N=12 d=2 chi=64 D_w=8", "norm2_rel_err": abs(n2 - n2r) / n2r,
-                                     "overlap_rel_err": abs(ov - ovr) / float(np.sqrt(n2r * ornd.norm2(xs))),
-                                     "svals_max_rel_err": err_s, "kept_match": r.kept == kept[1:]}}
+    value = n * args.chain_steps / dt
+    err = {"reduced_chain": "N=12 d=2 chi=64 D_w=8", "norm2_rel_err": abs(n2 - n2r) / n2r,
+           "kept_match": r.kept == kept[1:]}
+    err.update(overlap_errors(np, ov, ovr, n2r, ornd.norm2(xs)))
+    err.update(sval_errors(np, r.svals, sv, kept))
+    return {"sites_per_s": round(value, 3), "ms_per_step": round(1e3 * dt / args.chain_steps, 2),
+            "speedup_vs_plain": round(value / plain_sites_per_s, 3), "error_vs_fp64_oracle": err}
 
 
 def precision_form(np, torch):
@@ -375,8 +576,12 @@ def precision_form(np, torch):
     mixed form (products on the FP32 matrix cores; Gram matrices, Cholesky,
     Jacobi and truncation in FP64), inputs resident.  The mixed result is
     compared with the plain FP64 result of the same inputs (norm, overlap,
-    retained singular values per bond); the plain form itself is compared
-    with the NumPy FP64 oracle on a reduced chain of the same d and MPO bond
+    retained singular values per bond, value by value).  The plain result is
+    checked at full size through identities that need no oracle: Pythagoras
+    ||OX||^2 - ||Z||^2 = sum_k w_k (||OX||^2 by transfer matrices over the
+    product cores, built one site at a time), right-orthonormal cores 1..N-1
+    and the keep rule on every bond's reported spectrum; and against the
+    NumPy FP64 oracle on a reduced chain of the same d, MPO bond and cutoff
     (N=8, chi=64), since the oracle cannot hold the full size."""
     from oracle import reference as oref
     from oracle import rounding as ornd
@@ -408,17 +613,43 @@ def precision_form(np, torch):
     n2p = float(linalg.overlap(zp, zp).item())
     n2m = float(linalg.overlap(zm, zm).item())
     ov = float(linalg.overlap(zm, zp).item())
-    err_s = 0.0
+    per, smax = 0.0, 0.0
     for sp, sm, kp in zip(reps["fp64"].svals, reps["mixed"].svals, reps["fp64"].kept):
         a, b = sp.cpu().numpy()[:kp], sm.cpu().numpy()[:kp]
-        err_s = max(err_s, float(np.max(np.abs(a - b)) / a[0]))
+        per = max(per, float(np.max(np.abs(a - b) / a)))
+        smax = max(smax, float(np.max(np.abs(a - b)) / a[0]))
     out["mixed"]["speedup_vs_fp64"] = round(out["fp64"]["seconds_per_step"] / out["mixed"]["seconds_per_step"], 3)
     out["mixed"]["error_vs_fp64_gpu"] = {"norm2_rel_err": abs(n2m - n2p) / n2p,
                                          "overlap_rel_err": abs(ov - n2p) / n2p,
-                                         "svals_max_rel_err": err_s,
+                                         "svals_per_value_rel_err": per, "svals_err_over_smax": smax,
                                          "kept_match": reps["fp64"].kept == reps["mixed"].kept}
-    out["kept_bonds_fp64"] = [min(reps["fp64"].kept), max(reps["fp64"].kept)]
-    del res, zp, zm
+    kept_p = reps["fp64"].kept
+    out["kept_bonds_fp64"] = [min(kept_p), max(kept_p)]
+    del res, zm
+    # full-size identities of the plain result
+    log("precision form: full-size identities (Pythagoras, orthonormality, keep rule)")
+    e = torch.ones((1, 1), dtype=torch.float64, device=dev)
+    for o, x in zip(O, X):
+        # E' = sum_s P_s^T E P_s, with P_s = P[:, s, :]: T = E P (l x d r), then
+        # E' = (P as (l d) x r)^T (T as (l d) x r)
+        P = ops.apply_many([(o, x)])[0]
+        l, r = P.shape[0], P.shape[2]
+        T = ops.gemm(e, P.reshape(l, -1))
+        e = ops.gemm(P.reshape(-1, r).t(), T.reshape(-1, r))
+        del P, T
+    full2 = float(e.reshape(-1)[0].item())
+    wsum = sum(float(t.item()) for t in reps["fp64"].discarded)
+    orth = 0.0
+    for c in zp[1:]:
+        R = c.reshape(c.shape[0], -1)
+        G = ops.gemm(R, R.t())
+        orth = max(orth, float((G - torch.eye(G.shape[0], dtype=G.dtype, device=G.device)).abs().max().item()))
+    keep_ok = all(ornd.keep_count(s.cpu().numpy(), chi, cut) == c for s, c in zip(reps["fp64"].svals, kept_p))
+    out["fp64_full_size"] = {"pythagoras_rel_err": abs(full2 - n2p - wsum) / full2,
+                             "discarded_weight_rel": wsum / full2, "right_orthonormality_max_err": orth,
+                             "keep_rule_holds_every_bond": keep_ok}
+    log(f"precision form: {out['fp64_full_size']}")
+    del zp, e
     # the plain form against the oracle, reduced chain of the same d and D_w
     xs, os_ = W.chain_inputs(8, d, 64, dw, seed=65)
     Xs = [torch.from_numpy(x).to(dev) for x in xs]
@@ -428,89 +659,12 @@ def precision_form(np, torch):
     ref, sv, kept, _ = ornd.apply_and_round(os_, xs, chi_max=64, cutoff=cut)
     n2, n2r = float(linalg.overlap(cores, cores).item()), ornd.norm2(ref)
     ovg, ovr = float(linalg.overlap(cores, Xs).item()), oref.overlap(ref, xs)
-    es = max(float(np.max(np.abs(r.svals[k - 1].cpu().numpy()[: kept[k]] - sv[k][: kept[k]])) / sv[k][0])
-             for k in range(1, 8))
-    out["fp64_vs_oracle"] = {"reduced_chain": f"N=8 d={d} chi=64 D_w={dw}", "norm2_rel_err": abs(n2 - n2r) / n2r,
-                             "overlap_rel_err": abs(ovg - ovr) / float(np.sqrt(n2r * ornd.norm2(xs))),
-                             "svals_max_rel_err": es, "kept_match": r.kept == kept[1:]}
+    fo = {"reduced_chain": f"N=8 d={d} chi=64 D_w={dw}", "norm2_rel_err": abs(n2 - n2r) / n2r,
+          "kept_match": r.kept == kept[1:]}
+    fo.update(overlap_errors(np, ovg, ovr, n2r, ornd.norm2(xs)))
+    fo.update(sval_errors(np, r.svals, sv, kept))
+    out["fp64_vs_oracle"] = fo
     return out
-
-
-def batch_form(np, torch, args):
-    """4096 chains, one gate layer each, rounded back to their chi: the
-    grouped engine (all chains in lock-step launches) against the per-chain
-    form on a sample of the same jobs; oracle error on a few jobs."""
-    from syngular_amd import batch as B
-    from syngular_amd import ops, rounding
-    from syngular_amd import workloads as W
-
-    t0 = time.perf_counter()
-    jobs = W.batching_jobs(args.batch_jobs, seed=3)
-    nsites = sum(j["n"] for j in jobs)
-    log(f"batch: {len(jobs)} jobs, {nsites} sites, generated in {time.perf_counter() - t0:.1f} s")
-    X = B.ChainBatch.from_numpy([j["cores"] for j in jobs])
-    Op = B.gate_layer([j["n"] for j in jobs], np.zeros(len(jobs), np.int64),
-                      np.concatenate([j["gates"] for j in jobs]))
-    chi = np.array([j["chi"] for j in jobs])
-    torch.cuda.synchronize()
-    B.apply_and_round(X, Op, chi)          # warm-up
-    torch.cuda.synchronize()
-    res = {}
-    for prec in ("fp64", "mixed"):
-        times = []
-        for _ in range(2):
-            t0 = time.perf_counter()
-            Zp = B.apply_and_round(X, Op, chi, precision=prec)
-            torch.cuda.synchronize()
-            times.append(time.perf_counter() - t0)
-        res[prec] = (nsites / min(times), min(times), Zp)
-        log(f"batch: {prec} {nsites / min(times):.1f} sites/s")
-    batched, Z = res["fp64"][0], res["fp64"][2]
-    # per-chain form on a strided sample of the same jobs
-    dev = ops.device()
-    sample = jobs[:: max(1, len(jobs) // 256)]
-    done_sites, plain_time, used = 0, 0.0, 0
-    for j in sample:
-        # inputs go to the device before the clock starts, as for the batch
-        Xs = [torch.from_numpy(c).to(dev) for c in j["cores"]]
-        Os = [torch.from_numpy(c).to(dev) for c in W.gate_mpo_cores(j["n"], 0, j["gates"])]
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        rounding.apply_and_round(Os, Xs, chi_max=j["chi"])
-        torch.cuda.synchronize()
-        plain_time += time.perf_counter() - t1
-        done_sites += j["n"]
-        used += 1
-        if plain_time > args.plain_seconds:
-            break
-    plain = done_sites / plain_time
-    log(f"batch: per-chain form {plain:.1f} sites/s; oracle check of every 512th job")
-    # oracle error on a few jobs (every 512th)
-    from oracle import reference as oref
-    from oracle import rounding as ornd
-    errs, errm = [], []
-    check = list(range(0, len(jobs), 512))[:8]
-    outs = Z.to_numpy(check)
-    outm = res["mixed"][2].to_numpy(check)
-    for q, c, cm in zip(check, outs, outm):
-        j = jobs[q]
-        ref = ornd.apply_and_round(W.gate_mpo_cores(j["n"], 0, j["gates"]), j["cores"], chi_max=j["chi"])[0]
-        n2r, ovr = ornd.norm2(ref), oref.overlap(ref, j["cores"])
-        scale = np.sqrt(n2r * ornd.norm2(j["cores"]))
-        errs.append((abs(ornd.norm2(c) - n2r) / n2r, abs(oref.overlap(c, j["cores"]) - ovr) / scale))
-        errm.append((abs(ornd.norm2(cm) - n2r) / n2r, abs(oref.overlap(cm, j["cores"]) - ovr) / scale))
-    return {"workload": f"{len(jobs)} chains, N~U[8,48], d=2, chi~U{{8,16,32,64,128}}, one layer of Haar "
-                        f"O(4) 2-site gates (MPO bond 4) on pairs (0,1),(2,3),..., rounded to chi; plain FP64",
-            "sites": nsites, "sites_per_s": round(batched, 1), "seconds": round(min(times), 3),
-            "plain_fp64_per_chain_sites_per_s": round(plain, 1),
-            "plain_sample": f"{done_sites} sites of {used} strided jobs, one chain at a time",
-            "speedup_vs_plain": round(batched / plain, 2),
-            "oracle_jobs": len(errs), "norm2_rel_err_max": max(e[0] for e in errs),
-            "overlap_rel_err_max": max(e[1] for e in errs),
-            "mixed": {"sites_per_s": round(res["mixed"][0], 1), "seconds": round(res["mixed"][1], 3),
-                      "speedup_vs_plain": round(res["mixed"][0] / plain, 2),
-                      "norm2_rel_err_max": max(e[0] for e in errm),
-                      "overlap_rel_err_max": max(e[1] for e in errm)}}
 
 
 def serve_form(np, torch, args):
@@ -565,15 +719,40 @@ def serve_form(np, torch, args):
     srv2 = serving.Server(serving.DeviceExecutor(precision=prec), hbm_budget=2 ** 36, keep_states=True)
     got = {r.id: r for r in srv2.run([dict(j, arrival=0.0) for j in cheap]).results}
     errs = []
+    from oracle import reference as oref
     for j in cheap:
         cores = [np.eye(2)[0].reshape(1, 2, 1) for _ in range(j["n"])]
         for layer in range(j["layers"]):
             mpo = W.gate_mpo_cores(j["n"], layer % 2, W.layer_gates(j["seed"], j["id"], layer, j["n"]))
             cores = ornd.apply_and_round(mpo, cores, chi_max=j["chi_max"], cutoff=j["cutoff"])[0]
-        from oracle import reference as oref
         g = got[j["id"]].cores
         n2r = ornd.norm2(cores)
         errs.append((abs(ornd.norm2(g) - n2r) / n2r, abs(oref.overlap(g, cores) - n2r) / n2r))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the same stream replayed on the host by the oracle, one job and one
+        # layer at a time, time-boxed (the early layers of a circuit are the
+        # cheap ones, so the box favours the host)
+        threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
+        cdone, t1, cj, stop = 0, time.perf_counter(), [], False
+        for j in jobs[::4]:
+            cores = [np.eye(2)[0].reshape(1, 2, 1) for _ in range(j["n"])]
+            cj.append(j["id"])
+            for layer in range(j["layers"]):
+                mpo = W.gate_mpo_cores(j["n"], layer % 2, W.layer_gates(j["seed"], j["id"], layer, j["n"]))
+                cores = ornd.apply_and_round(mpo, cores, chi_max=j["chi_max"], cutoff=j["cutoff"])[0]
+                cdone += j["n"]
+                if time.perf_counter() - t1 > args.cpu_seconds:
+                    stop = True
+                    break
+            if stop:
+                break
+        dt = time.perf_counter() - t1
+        cpu = {"value": round(cdone / dt, 2), "unit": "site-layers/s", "cores": int(threads), "kind": "port",
+               "sample": f"NumPy FP64 oracle replaying the circuits of jobs {cj} of the same stream (every 4th) "
+                         f"one layer at a time, {cdone} site-layers in {dt:.1f} s (stopped at the box: later, "
+                         f"larger-bond layers are left out, which favours the host)"}
+        log(f"serve cpu baseline: {cpu}")
     return {"workload": f"Poisson stream of {len(jobs)} jobs at {args.serve_rate}/s: N~U[16,128], layers~U[4,32] "
                         f"brickwork Haar O(4) gates (MPO bond 4), chi_max~U{{64,128,256,512}}, cutoff 1e-10, "
                         f"from |0...0>; {prec} (served, grouped launches)",
@@ -591,7 +770,24 @@ def serve_form(np, torch, args):
             "norm_pythagoras_err_max": pyth,
             "oracle_jobs": [f"N={j['n']} layers={j['layers']} chi_max={j['chi_max']}" for j in cheap],
             "norm2_rel_err_max": max(e[0] for e in errs),
-            "overlap_rel_err_max": max(e[1] for e in errs)}
+            "overlap_rel_err_max": max(e[1] for e in errs),
+            "roofline": served_roofline(),
+            "cpu_baseline": cpu}
+
+
+SERVE_PROFILE = os.path.join(ROOT, "profiles", "serve_stream48_kernel_stats.json")
+
+
+def served_roofline():
+    """Roofline of the served form's dominant work kernel, from the committed
+    rocprofv3 kernel-trace summary of a served run (the server's lanes run in
+    host threads of their own, which the in-process event timer does not
+    follow): algorithmic work per launch over the average launch duration."""
+    try:
+        with open(SERVE_PROFILE) as f:
+            return json.load(f)["roofline"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def api_form(np, torch, small=False):

@@ -45,8 +45,8 @@ int syn_version(void);
 int syn_device_count(int *count);
 /* ABI self-check: writes sizeof() of the item structs below, in declaration
  * order (gemm, apply, mpo_mpo, dsum, qr, jacobi, trunc, rowscale, bjac,
- * eye, copy, diagmax, chol, gbj, lusign, colscale), into out[0..n-1]; returns the
- * number of structs. */
+ * eye, copy, diagmax, chol, gbj, lusign, colscale, vecnorm), into out[0..n-1];
+ * returns the number of structs. */
 int syn_struct_sizes(int64_t *out, int n);
 
 /* ------------------------------------------------------------------ */
@@ -433,6 +433,24 @@ typedef struct {
     double *discarded;
 } syn_trunc_item;
 int syn_truncate_f64(const syn_trunc_item *items, int n_items, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Vector norms: out[j] = ||x[:, j]|| (axis 0, j < cols) or              */
+/* out[i] = ||x[i, :]|| (axis 1, i < rows) of row-major matrices, summed */
+/* in FP64 in a fixed order.  block_start counts ceil(cols / 64) blocks   */
+/* per item for axis 0 and ceil(rows / 4) for axis 1.  The rounding      */
+/* sweep reads the retained singular values off C V = U S (tall C) or    */
+/* U^T C = S V^T (wide C); no reference call site (the reference has no  */
+/* SVD rounding; the values stand beside numpy.linalg.svd's).            */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *x;
+    int64_t rows, cols, ld;
+    int32_t axis;
+    double *out;
+    int64_t block_start;
+} syn_vecnorm_item;
+int syn_vecnorms_f64(const syn_vecnorm_item *items, int n_items, int64_t total_blocks, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Row scaling  X[i, :] *= s[i]  (mode 0)  or  X[i, :] /= s[i]  (mode 1; */

@@ -72,8 +72,13 @@ COLSCALE_ITEM = np.dtype(
     [("src", _P), ("lds", _I), ("dst", _P), ("ldd", _I), ("rows", _I), ("cols", _I), ("s", _P),
      ("trans", np.int32), ("block_start", _I)], align=True)
 
+VECNORM_ITEM = np.dtype(
+    [("x", _P), ("rows", _I), ("cols", _I), ("ld", _I), ("axis", np.int32), ("out", _P), ("block_start", _I)],
+    align=True)
+
 ITEM_DTYPES = [GEMM_ITEM, APPLY_ITEM, MPO_MPO_ITEM, DSUM_ITEM, QR_ITEM, JACOBI_ITEM, TRUNC_ITEM, ROWSCALE_ITEM,
-               BJAC_ITEM, EYE_ITEM, COPY_ITEM, DIAGMAX_ITEM, CHOL_ITEM, GBJ_ITEM, LUSIGN_ITEM, COLSCALE_ITEM]
+               BJAC_ITEM, EYE_ITEM, COPY_ITEM, DIAGMAX_ITEM, CHOL_ITEM, GBJ_ITEM, LUSIGN_ITEM, COLSCALE_ITEM,
+               VECNORM_ITEM]
 
 # every symbol the header declares, with its ctypes signature
 _VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -96,6 +101,7 @@ SIGNATURES = {
     "syn_jacobi_gram_f64": (_I32, [_VP, _I32, _I64, _D, _I32, _VP]),
     "syn_truncate_f64": (_I32, [_VP, _I32, _VP]),
     "syn_rowscale_f64": (_I32, [_VP, _I32, _I64, _VP]),
+    "syn_vecnorms_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_jacobi_blocked_workspace": (_I64, [_I64, _I64]),
     "syn_jacobi_blocked_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _VP, _D, _I32, _I32,
                                       ctypes.POINTER(ctypes.c_int), _VP]),

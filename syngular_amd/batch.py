@@ -29,7 +29,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import _lib, ops
+from . import _lib, autotune, ops
 from ._lib import call
 
 F64 = torch.float64
@@ -108,28 +108,31 @@ def gemm_table(A, B, C, m, n, k, lda, ldb, ldc, ta=0, tb=0, alpha=1.0, beta=0.0,
         return gemm_table(*(bc(v, np.uint64)[sel] for v in (A, B, C)), m[sel], n[sel], k[sel],
                           *(bc(v, np.int64)[sel] for v in (lda, ldb, ldc, ta, tb)),
                           bc(alpha, np.float64)[sel], bc(beta, np.float64)[sel], bc(tri, np.int64)[sel], mixed)
-    area = m * n
-    tile = 0
-    if int(area.sum()) >= 256 * 128 * 128:
-        pad64 = (((m + 63) // 64) * ((n + 63) // 64)).sum() * 64 * 64
-        pad128 = (((m + 127) // 128) * ((n + 127) // 128)).sum() * 128 * 128
-        tile = 1 if pad128 <= 1.15 * pad64 else 0
-    bm = 128 if tile else 64
-    tn = (n + bm - 1) // bm
-    tm = (m + bm - 1) // bm
-    st, total = _starts(tm * tn)
+    # tile configuration per problem from its shape class (autotune), never
+    # from the launch: a chain's sums do not depend on its company
+    tl = autotune.tiles(m, n, k, mixed)
     it = np.zeros(cnt, dtype=_lib.GEMM_ITEM)
     it["A"], it["B"], it["C"] = bc(A, np.uint64), bc(B, np.uint64), bc(C, np.uint64)
     it["m"], it["n"], it["k"] = m, n, k
     it["lda"], it["ldb"], it["ldc"] = bc(lda, np.int64), bc(ldb, np.int64), bc(ldc, np.int64)
     it["alpha"], it["beta"] = bc(alpha, np.float64), bc(beta, np.float64)
     it["transA"], it["transB"] = bc(ta, np.int32), bc(tb, np.int32)
-    it["block_start"], it["tiles_n"] = st, tn
     it["tri"] = bc(tri, np.int32)
-    dev = ops._upload(it)
-    if ops._profiling[0]:
-        call("syn_prof_note_flops", float((2.0 * m * n * k).sum()))
-    call("syn_gemm_mixed_f64" if mixed else "syn_gemm_f64", dev.data_ptr(), cnt, total, tile, ops._stream())
+    for tile in (0, 1):
+        sel = np.nonzero(tl == tile)[0]
+        if len(sel) == 0:
+            continue
+        part = it[sel] if len(sel) < cnt else it
+        bm = 128 if tile else 64
+        tn = (n[sel] + bm - 1) // bm
+        tm = (m[sel] + bm - 1) // bm
+        st, total = _starts(tm * tn)
+        part["block_start"], part["tiles_n"] = st, tn
+        dev = ops._upload(part)
+        if ops._profiling[0]:
+            call("syn_prof_note_flops", float((2.0 * m[sel] * n[sel] * k[sel]).sum()))
+        call("syn_gemm_mixed_f64" if mixed else "syn_gemm_f64", dev.data_ptr(), len(sel), total, tile,
+             ops._stream())
 
 
 def _launch_blocks(fn, it, nel):
@@ -160,6 +163,23 @@ def rowscale_table(x, rows, cols, ld, s, mode):
     it = np.zeros(len(x), dtype=_lib.ROWSCALE_ITEM)
     it["x"], it["rows"], it["cols"], it["ld"], it["s"], it["mode"] = x, rows, cols, ld, s, mode
     _launch_blocks("syn_rowscale_f64", it, np.asarray(rows, dtype=np.int64) * np.asarray(cols, dtype=np.int64))
+
+
+def vecnorm_table(x, rows, cols, ld, axis, out):
+    """out_i = column norms (axis 0) or row norms (axis 1) of every
+    row-major rows x cols matrix x_i (leading dimension ld)."""
+    cnt = len(x)
+    if cnt == 0:
+        return
+    rows, cols = (np.asarray(v, dtype=np.int64) for v in (rows, cols))
+    axis = np.broadcast_to(np.asarray(axis, dtype=np.int32), (cnt,))
+    it = np.zeros(cnt, dtype=_lib.VECNORM_ITEM)
+    it["x"], it["rows"], it["cols"], it["ld"], it["axis"], it["out"] = x, rows, cols, ld, axis, out
+    st, total = _starts(np.where(axis == 0, (cols + 63) // 64, (rows + 3) // 4))
+    it["block_start"] = st
+    if total:
+        dev = ops._upload(it)
+        call("syn_vecnorms_f64", dev.data_ptr(), cnt, total, ops._stream())
 
 
 def apply_table(x, o, p, chil, din, chir, dl, dout, dr):
@@ -440,6 +460,39 @@ class ChainBatch:
                 i += 1
         return cls(n, bond, phys, ptr, [t])
 
+    @classmethod
+    def gaussian(cls, n, chi, d=2, seed=0):
+        """Chains of n[j] sites with bonds (1, chi_j, ..., chi_j, 1) and
+        cores ~ N(0, 1/chi_j), drawn on the device (torch's generator, seeded)
+        so that thousands of chains need no host copy.  The arena holds the
+        chains grouped by chi (one scaling per group); the tables keep the
+        given chain order."""
+        n = np.asarray(n, dtype=np.int64)
+        chi = np.asarray(chi, dtype=np.int64)
+        J = len(n)
+        nmax = int(n.max()) if J else 0
+        k = np.arange(nmax + 1)[None, :]
+        bond = np.where(k == 0, 1, np.where(k < n[:, None], chi[:, None], np.where(k == n[:, None], 1, 0)))
+        mask = np.arange(nmax)[None, :] < n[:, None]
+        phys = np.where(mask, d, 0).astype(np.int64)
+        order = np.argsort(chi, kind="stable")
+        jj, kk = np.nonzero(mask[order])
+        jj = order[jj]
+        sizes = bond[jj, kk] * d * bond[jj, kk + 1]
+        t, addr = arena(sizes)
+        g = torch.Generator(device=t.device)
+        g.manual_seed(int(seed))
+        t.normal_(generator=g)
+        base = np.uint64(t.data_ptr())
+        for c in np.unique(chi):
+            sel = np.nonzero(chi[jj] == c)[0]
+            lo = int((addr[sel[0]] - base) // np.uint64(8))
+            hi = int((addr[sel[-1]] - base) // np.uint64(8)) + int(sizes[sel[-1]])
+            t[lo:hi].mul_(1.0 / np.sqrt(float(c)))
+        ptr = np.zeros((J, nmax), dtype=np.uint64)
+        ptr[jj, kk] = addr
+        return cls(n, bond.astype(np.int64), phys, ptr, [t])
+
     def core_shape(self, j, k):
         return (int(self.bond[j, k]), int(self.phys[j, k]), int(self.bond[j, k + 1]))
 
@@ -656,6 +709,9 @@ class ChainBatch:
                 mt, mpp = arena(l[ti] * cc[ti])
                 # M = C V = C (V^T)^T  (l x cc)
                 gemm_table(C[ti], rtp[ti], mpp, l[ti], cc[ti], ncol[ti], ncol[ti], g[ti], cc[ti], tb=1, mixed=mixed)
+                if not mixed:
+                    # retained s_i = column norms of U S (exact to eps s_max / s_i)
+                    vecnorm_table(mpp, l[ti], cc[ti], cc[ti], 0, sp[ti])
                 Mp[ti] = mpp
                 hold_step.append(mt)
             if wide.any():
@@ -663,6 +719,9 @@ class ChainBatch:
                 nt, npp = arena(cc[wi] * ncol[wi])
                 # newc = U^T C  (cc x ncol), then / s ; U^T *= s
                 gemm_table(rtp[wi], C[wi], npp, cc[wi], ncol[wi], l[wi], g[wi], ncol[wi], ncol[wi], mixed=mixed)
+                if not mixed:
+                    # retained s_i = row norms of S V^T, before the scaling uses them
+                    vecnorm_table(npp, cc[wi], ncol[wi], ncol[wi], 1, sp[wi])
                 rowscale_table(np.concatenate([npp, rtp[wi]]), np.concatenate([cc[wi], cc[wi]]),
                                np.concatenate([ncol[wi], l[wi]]), np.concatenate([ncol[wi], g[wi]]),
                                np.concatenate([sp[wi], sp[wi]]),
@@ -693,6 +752,52 @@ class ChainBatch:
         mixed = precision == "mixed"
         out = self.left_orthonormalize(mixed, None if mixed else REORTH_BUDGET).right_svd_sweep(
             chi_max, cutoff, records, mixed)
+        if compact:
+            out.compact()
+        return out
+
+
+class CapturedApplyLeft:
+    """The materialised apply and the left sweep of one batch, captured once
+    in a HIP graph and replayed per step.
+
+    That part of a layer reads nothing back to the host when the CholeskyQR
+    takes both passes (the mixed form always does; the plain form's
+    orthogonality budget reads estimates back and is not captured): every
+    kernel, item-table copy and arena of the sweep is fixed by the shapes and
+    addresses of the inputs.  So one capture replaces the host's work of
+    building and uploading the tables of every lock-step step (numpy over
+    all chains per launch) by a graph launch.  The inputs X and Op must stay
+    where they are (their addresses are in the tables); ``out`` (the
+    left-orthonormal batch, in the graph's memory pool) is overwritten by
+    every replay, and the right sweep only reads it.  The right sweep stays
+    eager: its next shapes depend on the kept ranks decided on the device,
+    and the block Jacobi tests convergence from the host."""
+
+    def __init__(self, X, Op, mixed=True):
+        self.mixed = bool(mixed)
+        self.inputs = (X, Op)
+        # one eager run first: kernel attributes and allocator pools settle
+        X.apply(Op).left_orthonormalize(self.mixed, None)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        self.staging = []
+        ops._staging_hold[0] = self.staging
+        try:
+            with torch.cuda.graph(self.graph):
+                self.out = X.apply(Op).left_orthonormalize(self.mixed, None)
+        finally:
+            ops._staging_hold[0] = None
+
+    def __call__(self):
+        """Replay on the current stream; returns the left-orthonormal batch."""
+        self.graph.replay()
+        return self.out
+
+    def round(self, chi_max, cutoff=0.0, records=None, compact=True):
+        """One layer: replay, then the eager right sweep (TT-rounding of the
+        captured batch to chi_max with the cutoff)."""
+        out = self().right_svd_sweep(chi_max, cutoff, records, self.mixed)
         if compact:
             out.compact()
         return out

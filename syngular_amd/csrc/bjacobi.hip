@@ -453,10 +453,15 @@ extern "C" int syn_jacobi_blocked_many_f64(syn_bjac_item *items, const syn_bjac_
         all[i].active = all[i].nv > 1;
         all[i].block_start = 0;
     }
-    // full table on the device for init / norms / permute
+    // full table on the device for init / norms / permute; the copy goes
+    // through the caller's stream and is waited for, so it is ordered with
+    // that stream's earlier kernels and the host vector may change afterwards
     auto upload = [&](const std::vector<syn_bjac_item> &v) -> int {
-        return hip_check(hipMemcpy(items, v.data(), v.size() * sizeof(syn_bjac_item), hipMemcpyHostToDevice),
-                         "bjac table upload");
+        if (int rc = hip_check(hipMemcpyAsync(items, v.data(), v.size() * sizeof(syn_bjac_item),
+                                              hipMemcpyHostToDevice, st),
+                               "bjac table upload"))
+            return rc;
+        return hip_check(hipStreamSynchronize(st), "bjac table upload");
     };
     if (int rc = upload(all)) return rc;
     hipLaunchKernelGGL(bjac_many_init_kernel, dim3(n_items), dim3(256), 0, st, items);
@@ -532,11 +537,14 @@ extern "C" int syn_jacobi_blocked_many_f64(syn_bjac_item *items, const syn_bjac_
     int64_t *dperm = (int64_t *)(work + tot_w);
     int64_t *dpoff = dperm + tot_nv;
     int64_t *dwoff = dpoff + n_items;
-    if (int rc = hip_check(hipMemcpy(dperm, hp.data(), tot_nv * sizeof(int64_t), hipMemcpyHostToDevice), "perm"))
+    if (int rc = hip_check(hipMemcpyAsync(dperm, hp.data(), tot_nv * sizeof(int64_t), hipMemcpyHostToDevice, st),
+                           "perm"))
         return rc;
-    if (int rc = hip_check(hipMemcpy(dpoff, poff.data(), n_items * sizeof(int64_t), hipMemcpyHostToDevice), "poff"))
+    if (int rc = hip_check(hipMemcpyAsync(dpoff, poff.data(), n_items * sizeof(int64_t), hipMemcpyHostToDevice, st),
+                           "poff"))
         return rc;
-    if (int rc = hip_check(hipMemcpy(dwoff, woff.data(), n_items * sizeof(int64_t), hipMemcpyHostToDevice), "woff"))
+    if (int rc = hip_check(hipMemcpyAsync(dwoff, woff.data(), n_items * sizeof(int64_t), hipMemcpyHostToDevice, st),
+                           "woff"))
         return rc;
     hipLaunchKernelGGL(bjac_many_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dwoff, 0);
     hipLaunchKernelGGL(bjac_many_permute_kernel, dim3(n_items), dim3(256), 0, st, items, dperm, dpoff, work, dwoff, 1);

@@ -421,11 +421,25 @@ __global__ __launch_bounds__(256) void diagmax_many_kernel(const syn_diagmax_ite
 
 // Zero an m x n block (leading dimension ld) on the stream.  These routines
 // are not meant for graph capture, so a memset is fine here.
+// Zero fill of a rows x cols block (leading dimension ld): 256 columns per
+// work-group along x, rows strided over y.  A kernel rather than a memset
+// node, so that a stream capture of the CholeskyQR records no memset.
+__global__ __launch_bounds__(256) void zero2d_kernel(double *x, int64_t rows, int64_t cols, int64_t ld) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= cols) return;
+    for (int64_t i = blockIdx.y; i < rows; i += gridDim.y) x[i * ld + j] = 0.0;
+}
+
 static int zero(double *x, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
     if (rows <= 0 || cols <= 0) return SYN_OK;
-    return hip_check(hipMemset2DAsync(x, (size_t)ld * sizeof(double), 0, (size_t)cols * sizeof(double),
-                                      (size_t)rows, s),
-                     "zero fill");
+    const int64_t gx = (cols + 255) / 256, gy = rows < 1024 ? rows : 1024;
+    if (gx > 0x7fffffffLL) {
+        set_error("zero fill: %lld columns", (long long)cols);
+        return SYN_EINVAL;
+    }
+    hipLaunchKernelGGL(zero2d_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, x, rows, cols, ld);
+    SYN_LAUNCH_CHECK("zero fill");
+    return SYN_OK;
 }
 
 // Blocked Cholesky of G (destroyed) into L; Z holds the inverted diagonal
@@ -471,10 +485,7 @@ static int potrf(double *G, int64_t ldg, double *L, int64_t ldl, int64_t n, doub
 // ws (ws_cap doubles), when given, lets the products split along K; the
 // CholeskyQR callers pass none: on the 8192 x 4096 chain the split saved
 // under 1% of a step there.
-// nodes with 64 < n <= 128 in one launch (chol_node_kernel)
-#ifndef SYN_CHOL_NODE
-#define SYN_CHOL_NODE 1
-#endif
+// Nodes with 64 < n <= 128 run in one launch (chol_node_kernel).
 
 static int64_t chol_inv_scratch(int64_t n) {
     if (n <= kNb) return 0;
@@ -492,7 +503,7 @@ static int chol_inv(double *G, int64_t ldg, double *L, int64_t ldl, double *X, i
         SYN_LAUNCH_CHECK("chol_diag");
         return SYN_OK;
     }
-    if (SYN_CHOL_NODE && n <= 2 * kNb) {
+    if (n <= 2 * kNb) {
         ProfScope ps(kTagCholNode, (double)n * n * n * (2.0 / 3.0), s);
         hipLaunchKernelGGL(chol_node_kernel, dim3(1), dim3(kCholThreads), 0, s, G, ldg, L, ldl, X, ldx, n, T, scal,
                            tolrel, dropped);

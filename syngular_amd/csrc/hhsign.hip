@@ -147,8 +147,11 @@ extern "C" int syn_lu_sign_leaf_f64(const syn_lusign_item *items, int n_items, d
     if (n_items == 0) return SYN_OK;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)lu_sign_leaf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kLuLdsBytes);
+        // 66,560 B of dynamic LDS: above the 64 KiB a launch gets without the attribute
+        if (int rc = hip_check(hipFuncSetAttribute((const void *)lu_sign_leaf_kernel,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLuLdsBytes),
+                               "syn_lu_sign_leaf_f64: hipFuncSetAttribute(MaxDynamicSharedMemorySize)"))
+            return rc;
         attr_set = true;
     }
     hipLaunchKernelGGL(lu_sign_leaf_kernel, dim3(n_items), dim3(kLuThreads), kLuLdsBytes, (hipStream_t)stream, items,

@@ -44,7 +44,8 @@ extern "C" int syn_struct_sizes(int64_t *out, int n) {
                              (int64_t)sizeof(syn_bjac_item),    (int64_t)sizeof(syn_eye_item),
                              (int64_t)sizeof(syn_copy_item),    (int64_t)sizeof(syn_diagmax_item),
                              (int64_t)sizeof(syn_chol_item),    (int64_t)sizeof(syn_gbj_item),
-                             (int64_t)sizeof(syn_lusign_item),  (int64_t)sizeof(syn_colscale_item)};
+                             (int64_t)sizeof(syn_lusign_item),  (int64_t)sizeof(syn_colscale_item),
+                             (int64_t)sizeof(syn_vecnorm_item)};
     const int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (!out) return count;
     for (int i = 0; i < n && i < count; ++i) out[i] = sizes[i];

@@ -92,12 +92,6 @@ struct RrPlane {
     static constexpr int kRounds = TP - 1;
 };
 
-// Steps after the first of each round-robin pass run the cross-block order
-// only (0: every step runs the full round-robin sweep of its pair tile)
-#ifndef SYN_TJ_CROSS
-#define SYN_TJ_CROSS 1
-#endif
-
 // Rotation test of plane (p, q): |G_pq| > max(tol sqrt(|G_pp G_qq|), floor),
 // in squares (no square root on the critical path of a round).
 __device__ __forceinline__ bool rotates(double apq, double app, double aqq, double tol, double floor_abs) {
@@ -111,9 +105,6 @@ __device__ __forceinline__ bool rotates(double apq, double app, double aqq, doub
 // c = 1 / sqrt(1 + t^2), s = c t.  The scaled form needs one division and one
 // square root for t; operands far from 1 (|d|, |e| outside 1e-150 .. 1e150)
 // take the tau form, which cannot overflow.
-#ifndef SYN_TJ_FAST_ROT
-#define SYN_TJ_FAST_ROT 1
-#endif
 // 1 / x and 1 / sqrt(x) for normal positive x from the hardware estimates
 // (v_rcp_f64, v_rsq_f64) and two Newton-Raphson steps each: r <- r + r (1 - x r)
 // and y <- y + y (1 - x y^2) / 2, each step squaring the relative error, to
@@ -143,21 +134,13 @@ __device__ __forceinline__ void rotation(double apq, double app, double aqq, dou
     const double ad = fabs(d), ae = fabs(e), m = fmax(ad, ae);
     double t;
     if (m < 1e150 && m > 1e-150) {
-#if SYN_TJ_FAST_ROT
         const double h2 = fma(d, d, e * e);
         t = copysign(ae, d * e) * recip_nr(ad + h2 * rsqrt_nr(h2));
-#else
-        t = copysign(ae, d * e) / (ad + sqrt(fma(d, d, e * e)));
-#endif
     } else {
         const double tau = d / e, at = fabs(tau);
         t = copysign(1.0, tau) / (at < 1e150 ? at + sqrt(fma(at, at, 1.0)) : 2.0 * at);
     }
-#if SYN_TJ_FAST_ROT
     c = rsqrt_nr(fma(t, t, 1.0));
-#else
-    c = 1.0 / sqrt(fma(t, t, 1.0));
-#endif
     s = c * t;
 }
 
@@ -727,7 +710,7 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
         // so a pass treats every plane of G once
         tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, inner_sweeps,
                           tj_wset<TB>(P, D.half, ps) + pw * TP * TP, tj_flags<TB>(P, D.half, ps),
-                          SYN_TJ_CROSS && st != 0);
+                          st != 0);
         return;
     }
     if (!has_update) return;
@@ -980,9 +963,6 @@ static int64_t tj_work_max(int64_t nv) {
 
 static int g_tj_inner = 1;
 static int g_tj_block = 0;   // 0: automatic (16-row blocks: measured at least as fast up to 2048)
-#ifndef SYN_TJ_FUSED
-#define SYN_TJ_FUSED 1
-#endif
 
 template <int TB>
 static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
@@ -1018,9 +998,10 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     // counter (at least tot_nv + 15 doubles are left there)
     int32_t *chk = (int32_t *)(dbytes + 1);
     std::vector<int32_t> hc(n_items), hk(n_items);
-    // fused steps (16-row blocks): buffer holding each problem's Gram matrix
+    // fused steps (16-row blocks; 32-row blocks run solves and updates in
+    // launches of their own): buffer holding each problem's Gram matrix
     // (0: P.g, 1: the workspace copy); all problems of a sweep share it
-    constexpr bool fused = TB == 16 && SYN_TJ_FUSED;
+    constexpr bool fused = TB == 16;
     std::vector<int32_t> gpar(n_items, 0);
     int cur = 0;
     std::vector<int> idx;

@@ -33,13 +33,22 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+# While a HIP graph is captured, the pinned staging buffers of the item
+# tables are appended here and kept for the graph's lifetime: every replay
+# copies from them again, so they must neither be freed nor reused.
+_staging_hold = [None]
+
+
 def _upload(items: np.ndarray) -> torch.Tensor:
     """Copy an item table to the device without blocking the host: the bytes
     go through torch's cached pinned-host allocator, which keeps the staging
-    buffer alive until the asynchronous copy on the current stream is done."""
+    buffer alive until the asynchronous copy on the current stream is done
+    (or, under graph capture, for as long as the graph lives)."""
     raw = np.ascontiguousarray(items).view(np.uint8).reshape(-1)
     host = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True)
     host.numpy()[:] = raw
+    if _staging_hold[0] is not None:
+        _staging_hold[0].append(host)
     return host.to(device(), non_blocking=True)
 
 
@@ -275,6 +284,28 @@ def rowscale_many(problems):
     if start:
         dev = _upload(items)
         call("syn_rowscale_f64", _ptr(dev), len(problems), start, _stream())
+
+
+def vecnorms_many(problems):
+    """problems: list of (X (rows x cols row-major view), axis, out): out
+    (a contiguous float64 vector) gets the column norms of X (axis 0) or its
+    row norms (axis 1)."""
+    items = np.zeros(len(problems), dtype=_lib.VECNORM_ITEM)
+    start = 0
+    for i, (X, axis, out) in enumerate(problems):
+        px, ld, tr = as_matrix(X)
+        if tr:
+            raise ValueError("vecnorms needs a row-major view")
+        rows, cols = X.shape
+        if axis not in (0, 1) or out.numel() < (cols if axis == 0 else rows) or not out.is_contiguous():
+            raise ValueError(f"vecnorms: output of {out.numel()} for axis {axis} of a {rows} x {cols} matrix")
+        it = items[i]
+        it["x"], it["rows"], it["cols"], it["ld"] = px, rows, cols, ld
+        it["axis"], it["out"], it["block_start"] = axis, _ptr(out), start
+        start += (cols + 63) // 64 if axis == 0 else (rows + 3) // 4
+    if start:
+        dev = _upload(items)
+        call("syn_vecnorms_f64", _ptr(dev), len(problems), start, _stream())
 
 
 def copy_many(pairs):

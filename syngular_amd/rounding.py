@@ -14,9 +14,15 @@ this module adds the textbook TT-rounding (Oseledets 2011, Alg. 2):
    cutoff, then <= chi_max, >= 1), and U*S is absorbed into core k-1 with
    the GEMM.
 
-Singular values come from the Gram matrix, so a singular value s_i carries an
-absolute error of about eps * s_max^2 / s_i; all retained values of the
-workloads here are within a few decades of s_max.
+The Gram matrix decides the kept rank and the discarded weight (an
+eigenvalue of G carries an absolute error of about eps * s_max^2, so the
+tail sum is exact to a few eps of the total).  The retained singular values
+themselves are read off the product that carries them in the plain form:
+the column norms of C V = U S (tall C) or the row norms of U^T C = S V^T
+(wide C), which are exact to about eps * s_max / s_i relative, where the
+square root of an eigenvalue of G is only good to eps * (s_max / s_i)^2
+(1e-6 at s_i = 1e-5 s_max, the smallest value a 1e-10 cutoff keeps).  The
+mixed form keeps the FP64 Gram values: its products round to FP32.
 """
 from __future__ import annotations
 
@@ -135,9 +141,13 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None, mixed=False):
             Vt = rt[:cc]                               # (cc x ncol) = V^T
             M = ops.gemm(C, Vt.t(), mixed=mixed)       # C V = U S   (m x cc)
             newc = Vt
+            if not mixed:
+                ops.vecnorms_many([(M, 0, s[:cc])])    # s_i = ||(U S)[:, i]||
         else:
             Ut = rt[:cc].contiguous()                  # (cc x m) = U^T
             newc = ops.gemm(Ut, C, mixed=mixed)        # U^T C = S V^T (cc x ncol)
+            if not mixed:
+                ops.vecnorms_many([(newc, 1, s[:cc])])  # s_i = ||(S V^T)[i, :]||
             ops.rowscale_many([(newc, s, 1)])          # -> V^T
             ops.rowscale_many([(Ut, s, 0)])            # -> (U S)^T
             M = Ut.t()

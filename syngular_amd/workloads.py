@@ -121,6 +121,19 @@ def batching_jobs(count, seed, n_range=(8, 48), chis=(8, 16, 32, 64, 128), d=2):
     return jobs
 
 
+def batching_layout(count, seed, n_range=(8, 48), chis=(8, 16, 32, 64, 128)):
+    """The shapes and gates of the batching workload without its cores (the
+    cores are drawn on the device, batch.ChainBatch.gaussian): N ~ U[n_range],
+    chi ~ U{chis}, and one layer of Haar 2-site gates on the pairs
+    (0,1), (2,3), ... per chain.  Returns (n, chi, gates) with gates stacked
+    in chain order."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    cs = rng.choice(np.asarray(chis), size=count)
+    gates = haar_orthogonal(rng, int(sum(npairs(int(v), 0) for v in ns)))
+    return ns.astype(np.int64), cs.astype(np.int64), gates
+
+
 def serving_jobs(count, rate, seed, n_range=(16, 128), layer_range=(4, 32), chis=(64, 128, 256, 512),
                  cutoff=1e-10):
     """A Poisson stream of brickwork-circuit jobs: arrival times (s) with

@@ -187,6 +187,21 @@ class Emulator:
                 X *= inv[:, None]
         return 0
 
+    def syn_vecnorms_f64(self, items, n, total, stream):
+        from syngular_amd import _lib
+        it = self.table(items, n, _lib.VECNORM_ITEM)
+        blocks = np.where(it["axis"] == 0, (it["cols"] + 63) // 64, (it["rows"] + 3) // 4)
+        st = np.concatenate([[0], np.cumsum(blocks)[:-1]])
+        if not np.array_equal(st, it["block_start"]) or int(blocks.sum()) != int(total):
+            raise BoundsError(f"vecnorms: block table {list(it['block_start'])} / {total} does not fit the items")
+        for r in it:
+            rows, cols = int(r["rows"]), int(r["cols"])
+            X = self.mat(r["x"], rows, cols, int(r["ld"]), 0, "vecnorms x")
+            axis = int(r["axis"])
+            out = self.buf(r["out"], cols if axis == 0 else rows, what="vecnorms out")
+            out[:] = np.sqrt((X * X).sum(axis=axis))
+        return 0
+
     def syn_qr_f64(self, items, n, stream):
         from syngular_amd import _lib
         it = self.table(items, n, _lib.QR_ITEM)

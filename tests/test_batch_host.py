@@ -193,3 +193,52 @@ def test_emulated_mixed_precision_round(emu):
         assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max()
     with pytest.raises(ValueError):
         B.apply_and_round(X, O, 4, precision="bf16")
+
+
+def test_emulated_gaussian_batch_layout_and_round(emu):
+    """ChainBatch.gaussian (cores drawn in place, grouped by chi in the arena)
+    has the tables of the batching workload, cores of variance 1/chi, and
+    rounds like the same cores given from the host; the refined singular
+    values (vector norms of U S / S V^T) match the oracle's value by value."""
+    ns, chis, gates = W.batching_layout(12, seed=5, n_range=(3, 9), chis=(2, 4, 8))
+    X = B.ChainBatch.gaussian(ns, chis, 2, seed=5)
+    host = X.to_numpy()
+    for j, cores in enumerate(host):
+        c = int(chis[j])
+        assert [x.shape for x in cores] == [(1 if k == 0 else c, 2, c if k < ns[j] - 1 else 1)
+                                            for k in range(int(ns[j]))]
+    big = B.ChainBatch.gaussian([40] * 4, [16, 16, 64, 64], 2, seed=1).to_numpy()
+    for cores, c in zip(big, (16, 16, 64, 64)):
+        v = np.concatenate([x.reshape(-1) for x in cores[1:-1]])
+        assert abs(v.var() * c - 1.0) < 0.05
+    Op = B.gate_layer(ns, np.zeros(len(ns), np.int64), gates)
+    recs = []
+    Z = B.apply_and_round(X, Op, chis, 0.0, records=recs).to_numpy()
+    goff = np.concatenate([[0], np.cumsum([W.npairs(int(v), 0) for v in ns])])
+    for j in range(len(ns)):
+        os_ = W.gate_mpo_cores(int(ns[j]), 0, gates[goff[j]:goff[j + 1]])
+        ref = ornd.apply_and_round(os_, host[j], chi_max=int(chis[j]))[0]
+        a, b = oref.dense_mps(Z[j]), oref.dense_mps(ref)
+        assert np.abs(a - b).max() <= TOL * np.abs(b).max()
+    assert "syn_vecnorms_f64" in emu.calls
+
+
+def test_autotune_tiles_follow_the_problem_not_the_launch():
+    from syngular_amd import autotune
+
+    m = np.array([64, 300, 4096, 130, 256])
+    n = np.array([64, 300, 512, 130, 256])
+    k = np.array([10, 20, 30, 40, 50])
+    t = autotune.tiles(m, n, k, False)
+    # 300 and 130: the 128 x 128 padding wastes more than 15% over the 64 x 64 one
+    assert list(t) == [0, 0, 1, 0, 1]
+    # the same problem alone or among others gets the same tile
+    for i in range(len(m)):
+        assert autotune.tiles(m[i:i + 1], n[i:i + 1], k[i:i + 1], False)[0] == t[i]
+    key = int(autotune.classes(m[:1], n[:1], k[:1], False)[0])
+    try:
+        autotune.TABLE[key] = 1
+        assert autotune.tiles(m, n, k, False)[0] == 1
+        assert autotune.decode(key) == {"m": 64, "n": 64, "k": 16, "mixed": False}
+    finally:
+        autotune.TABLE.clear()

@@ -246,3 +246,55 @@ def test_batched_orthogonality_budget():
         _check_against_oracle(A.to_numpy()[q], ref, len(xs))
         n2a, n2b = ornd.norm2(A.to_numpy()[q]), ornd.norm2(Bb.to_numpy()[q])
         assert abs(n2a - n2b) <= 1e-10 * n2b
+
+
+def test_autotuned_tiles_keep_results_and_batch_invariance():
+    """The tile autotuner: record the shape classes of a mixed batch, time
+    both tile configurations per class, then the tuned run still matches the
+    oracle and a chain alone gives the same bits as in the mix (the tile
+    follows the problem's class, not its company)."""
+    from syngular_amd import autotune
+
+    jobs = _mixed_jobs(5)
+    X = B.ChainBatch.from_numpy([j[0] for j in jobs])
+    O = B.OperatorBatch.from_numpy([j[1] for j in jobs])
+    chi = np.array([j[2] for j in jobs])
+    try:
+        autotune.collect()
+        B.apply_and_round(X, O, chi)
+        table = autotune.tune()
+        assert table and all(t in (0, 1) for t in table.values())
+        out = B.apply_and_round(X, O, chi).to_numpy()
+        for q, (xs, os_, c) in enumerate(jobs):
+            ref = ornd.apply_and_round(os_, xs, chi_max=c)[0]
+            _check_against_oracle(out[q], ref, len(xs))
+        q = len(jobs) - 1
+        solo = B.apply_and_round(B.ChainBatch.from_numpy([jobs[q][0]]), B.OperatorBatch.from_numpy([jobs[q][1]]),
+                                 np.array([jobs[q][2]])).to_numpy()[0]
+        assert all(np.array_equal(a, b) for a, b in zip(solo, out[q]))
+    finally:
+        autotune.TABLE.clear()
+
+
+def test_captured_apply_left_sweep_replays_match_eager():
+    """The apply and left sweep of a batch captured in a HIP graph: four
+    replays, with eager work of other forms and synchronisations between
+    them, each giving the eager mixed-form result bit for bit."""
+    jobs = _mixed_jobs(7)
+    X = B.ChainBatch.from_numpy([j[0] for j in jobs])
+    O = B.OperatorBatch.from_numpy([j[1] for j in jobs])
+    chi = np.array([j[2] for j in jobs])
+    eager = B.apply_and_round(X, O, chi, precision="mixed").to_numpy()
+    cap = B.CapturedApplyLeft(X, O, mixed=True)
+    assert len(cap.staging) > 0
+    for rep in range(4):
+        got = cap.round(chi).to_numpy()
+        for q in range(len(jobs)):
+            assert [c.shape for c in got[q]] == [c.shape for c in eager[q]]
+            assert all(np.array_equal(a, b) for a, b in zip(got[q], eager[q])), (rep, q)
+        B.apply_and_round(X, O, chi, precision="fp64")
+        torch.cuda.synchronize()
+    for q, (xs, os_, c) in enumerate(jobs):
+        ref = ornd.apply_and_round(os_, xs, chi_max=c)[0]
+        n2r = ornd.norm2(ref)
+        assert abs(ornd.norm2(got[q]) - n2r) <= 1e-6 * n2r

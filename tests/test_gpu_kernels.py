@@ -595,5 +595,6 @@ def test_mixed_round_vs_oracle():
     ref = ornd.apply_and_round(os_, xs, chi_max=64)[0]
     n2, n2r = float(linalg.overlap(cores, cores).item()), ornd.norm2(ref)
     ov, ovr = float(linalg.overlap(cores, X).item()), oref.overlap(ref, xs)
-    assert abs(n2 - n2r) / n2r < 1e-5
-    assert abs(ov - ovr) / np.sqrt(n2r * ornd.norm2(xs)) < 1e-5
+    # the north star's bar for every non-plain form: 1e-6 against the FP64 oracle
+    assert abs(n2 - n2r) / n2r < 1e-6
+    assert abs(ov - ovr) / abs(ovr) < 1e-6
