@@ -777,17 +777,22 @@ class CapturedApplyLeft:
     def __init__(self, X, Op, mixed=True):
         self.mixed = bool(mixed)
         self.inputs = (X, Op)
-        # one eager run first: kernel attributes and allocator pools settle
-        X.apply(Op).left_orthonormalize(self.mixed, None)
+        # one eager run first: kernel attributes and allocator pools settle,
+        # and the bytes of its item tables size the staging pool
+        ops.count_uploads(True)
+        try:
+            X.apply(Op).left_orthonormalize(self.mixed, None)
+        finally:
+            nbytes = ops.count_uploads(False)
         torch.cuda.synchronize()
+        self.staging = ops.StagingPool(nbytes + 4096)
         self.graph = torch.cuda.CUDAGraph()
-        self.staging = []
-        ops._staging_hold[0] = self.staging
+        ops._staging[0] = self.staging
         try:
             with torch.cuda.graph(self.graph):
                 self.out = X.apply(Op).left_orthonormalize(self.mixed, None)
         finally:
-            ops._staging_hold[0] = None
+            ops._staging[0] = None
 
     def __call__(self):
         """Replay on the current stream; returns the left-orthonormal batch."""

@@ -33,22 +33,61 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-# While a HIP graph is captured, the pinned staging buffers of the item
-# tables are appended here and kept for the graph's lifetime: every replay
-# copies from them again, so they must neither be freed nor reused.
-_staging_hold = [None]
+class StagingPool:
+    """Pinned host memory for the item tables of a HIP graph capture.  No
+    pinned allocation may happen while a stream captures, and every replay
+    copies from the staging bytes again, so the tables are bump-allocated
+    from one buffer allocated beforehand and kept for the graph's lifetime
+    (``count_uploads`` measures the bytes in an eager run of the same work)."""
+
+    ALIGN = 256
+
+    def __init__(self, nbytes):
+        self.buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, pin_memory=True)
+        self.off = 0
+        self.tables = 0
+
+    def take(self, n):
+        o = (self.off + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        if o + n > self.buf.numel():
+            raise RuntimeError(f"staging pool of {self.buf.numel()} bytes exhausted (needs {o + n})")
+        self.off = o + n
+        self.tables += 1
+        return self.buf[o:o + n]
+
+
+# the StagingPool of the capture in progress (None when not capturing)
+_staging = [None]
+# bytes the uploads of an eager run would take from a pool (None: not counting)
+_upload_count = [None]
+
+
+def count_uploads(on=True):
+    """Start (or stop and return) counting the pool bytes of the uploads."""
+    if on:
+        _upload_count[0] = 0
+        return 0
+    n, _upload_count[0] = _upload_count[0], None
+    return n
 
 
 def _upload(items: np.ndarray) -> torch.Tensor:
     """Copy an item table to the device without blocking the host: the bytes
     go through torch's cached pinned-host allocator, which keeps the staging
-    buffer alive until the asynchronous copy on the current stream is done
-    (or, under graph capture, for as long as the graph lives)."""
+    buffer alive until the asynchronous copy on the current stream is done;
+    under graph capture they go through the capture's StagingPool."""
     raw = np.ascontiguousarray(items).view(np.uint8).reshape(-1)
+    if _upload_count[0] is not None:
+        _upload_count[0] += (raw.size + StagingPool.ALIGN - 1) // StagingPool.ALIGN * StagingPool.ALIGN
+    pool = _staging[0]
+    if pool is not None:
+        host = pool.take(raw.size)
+        host.numpy()[:] = raw
+        dev = torch.empty(raw.size, dtype=torch.uint8, device=device())
+        dev.copy_(host, non_blocking=True)
+        return dev
     host = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True)
     host.numpy()[:] = raw
-    if _staging_hold[0] is not None:
-        _staging_hold[0].append(host)
     return host.to(device(), non_blocking=True)
 
 

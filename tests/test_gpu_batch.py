@@ -286,7 +286,7 @@ def test_captured_apply_left_sweep_replays_match_eager():
     chi = np.array([j[2] for j in jobs])
     eager = B.apply_and_round(X, O, chi, precision="mixed").to_numpy()
     cap = B.CapturedApplyLeft(X, O, mixed=True)
-    assert len(cap.staging) > 0
+    assert cap.staging.tables > 0
     for rep in range(4):
         got = cap.round(chi).to_numpy()
         for q in range(len(jobs)):
