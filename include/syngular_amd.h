@@ -417,6 +417,18 @@ int syn_potrf_f64(double *g, int64_t n, int64_t ldg, double *l, int64_t ldl, dou
                   int *dropped, void *stream);
 int syn_cholqr_f64(double *a, int64_t m, int64_t n, int64_t lda, double *q, int64_t ldq, double *l, int64_t ldl,
                    double *work, double tolrel, int *dropped, int mixed, void *stream);
+/* Allocation-free form of syn_cholqr_f64 (no counterpart in the reference:  */
+/* it is for callers that cannot allocate device memory -- code under HIP     */
+/* graph capture, tight serving loops, harnesses that forbid hipMalloc).      */
+/* Same arguments without `work`: the workspace is a 48 MiB device array of   */
+/* the library, one per GPU, held until the process ends, so n is limited to */
+/* syn_cholqr_noalloc_max_n() (1078 columns; m is not limited) and larger n  */
+/* fails with SYN_EINVAL.  Nothing is allocated and no memset is issued.     */
+/* Every call on one GPU shares that array: calls that overlap on one GPU    */
+/* (other streams or host threads) race and must be serialised by the caller.*/
+int syn_cholqr_noalloc_f64(double *a, int64_t m, int64_t n, int64_t lda, double *q, int64_t ldq, double *l,
+                           int64_t ldl, double tolrel, int *dropped, int mixed, void *stream);
+int64_t syn_cholqr_noalloc_max_n(void);
 
 /* ------------------------------------------------------------------ */
 /* Device-side truncation: for sorted singular values s[0..n-1] keep the */

@@ -108,6 +108,8 @@ SIGNATURES = {
     "syn_cholqr_workspace": (_I64, [_I64]),
     "syn_potrf_f64": (_I32, [_VP, _I64, _I64, _VP, _I64, _VP, _D, _VP, _VP]),
     "syn_cholqr_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _VP, _D, _VP, _I32, _VP]),
+    "syn_cholqr_noalloc_f64": (_I32, [_VP, _I64, _I64, _I64, _VP, _I64, _VP, _I64, _D, _VP, _I32, _VP]),
+    "syn_cholqr_noalloc_max_n": (_I64, []),
     "syn_jacobi_blocked_many_f64": (_I32, [_VP, _VP, _I32, _VP, _VP, _D, _I32, ctypes.POINTER(ctypes.c_int), _VP]),
     "syn_jacobi_blocked_many_workspace": (_I64, [_VP, _VP, _I32]),
     "syn_eye_f64": (_I32, [_VP, _I32, _I64, _VP]),
