@@ -77,8 +77,11 @@ def parse():
                     help="time box of the per-chain comparison of the served form")
     ap.add_argument("--chain-steps", type=int, default=5, help="timed steps of the cfg1 chain form")
     ap.add_argument("--only", choices=FORMS, help="run one form only (profiling)")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="run the headline batch eagerly (no HIP graph for its apply and left sweep)")
+    ap.add_argument("--graph-steps", type=int, default=3,
+                    help="steps of the headline batch replayed from a captured HIP graph (apply + left sweep), "
+                         "timed beside the eager steps")
+    ap.add_argument("--autotune-load", help="tile table to load instead of tuning (autotune.save format)")
+    ap.add_argument("--autotune-save", help="write the tuned tile table here")
     for f in FORMS[1:]:
         ap.add_argument(f"--no-{f}", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -122,6 +125,7 @@ def main():
 
     head = batch_headline(np, torch, args) if want("batch") else None
     forms = {}
+    torch.cuda.empty_cache()
     if want("serve"):
         forms["serve_cfg4"] = serve_form(np, torch, args)
         log(f"serve form: {forms['serve_cfg4']}")
@@ -129,6 +133,7 @@ def main():
         forms["chain_cfg1"] = chain_form(np, torch, args)
         log(f"chain form: {forms['chain_cfg1']}")
     if want("precision"):
+        torch.cuda.empty_cache()      # the 8192-bond sweeps need large contiguous blocks
         forms["precision_cfg2"] = precision_form(np, torch)
         log(f"precision form: {forms['precision_cfg2']}")
     if want("api"):
@@ -188,13 +193,20 @@ def batch_headline(np, torch, args):
     # tile autotuning over the GEMM shape classes this (chi, d, D_w) mix
     # produces: one recording step, then both tiles timed per class
     t1 = time.perf_counter()
-    autotune.collect()
-    step()
-    step("fp64")
-    torch.cuda.synchronize()
-    autotune.tune()
+    if args.autotune_load:
+        autotune.load(args.autotune_load)
+        how = f"loaded from {args.autotune_load}"
+    else:
+        autotune.collect()
+        step()
+        step("fp64")
+        torch.cuda.synchronize()
+        autotune.tune()
+        how = "tuned"
+    if args.autotune_save:
+        autotune.save(args.autotune_save)
     tuned = autotune.export()
-    log(f"batch autotune: {len(tuned)} shape classes in {time.perf_counter() - t1:.1f} s, "
+    log(f"batch autotune: {len(tuned)} shape classes {how} in {time.perf_counter() - t1:.1f} s, "
         f"{sum(e['tile'] == '128x128' for e in tuned)} take 128x128")
     for i in range(args.warmup):
         t1 = time.perf_counter()
@@ -211,43 +223,36 @@ def batch_headline(np, torch, args):
     dom = dominant_work_kernel(census)
     log(f"batch census step (all kernels timed): {census_ms:.0f} ms; dominant work kernel {dom}")
     del Z
-    # the apply and the left sweep (no host reads in the mixed form) run as
-    # one captured HIP graph per step, the right sweep eagerly
-    cap = None
-    if not args.no_graph:
-        t1 = time.perf_counter()
-        cap = B.CapturedApplyLeft(X, Op, mixed=True)
-        torch.cuda.synchronize()
-        log(f"batch: apply + left sweep captured in a HIP graph ({cap.staging.tables} table copies) in "
-            f"{time.perf_counter() - t1:.1f} s")
-
-    def timed_step():
-        return cap.round(chis) if cap is not None else step()
-
-    timed_step()
-    torch.cuda.synchronize()
     ops.profile_begin([dom])
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        Z = timed_step()
+        Z = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
     timed = ops.profile_end()
     value = nsites * args.steps / elapsed
     ms = 1e3 * elapsed / args.steps
     log(f"batch mixed: {args.steps} steps in {elapsed:.2f} s, {ms:.0f} ms/step, {value:.0f} sites/s")
-    measured_in = "the timed steps"
-    if dom not in timed:
-        # the dominant kernel runs inside the captured graph, where launches
-        # carry no events: time it over as many eager steps instead
-        ops.profile_begin([dom])
-        for _ in range(args.steps):
-            step()
-        torch.cuda.synchronize()
-        timed = ops.profile_end()
-        measured_in = f"{args.steps} eager steps (inside the timed steps it runs in the captured graph)"
     roof = kernel_roofline(census, timed, dom, args.steps)
-    roof["measured_in"] = measured_in
+    # the same step with its apply and left sweep (no host reads in the
+    # mixed form) replayed from one captured HIP graph, the right sweep eager
+    graph = None
+    if args.graph_steps > 0:
+        t1 = time.perf_counter()
+        cap = B.CapturedApplyLeft(X, Op, mixed=True)
+        cap.round(chis)
+        torch.cuda.synchronize()
+        tc = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        for _ in range(args.graph_steps):
+            cap.round(chis)
+        torch.cuda.synchronize()
+        gms = 1e3 * (time.perf_counter() - t1) / args.graph_steps
+        graph = {"ms_per_step": round(gms, 1), "sites_per_s": round(nsites / (gms * 1e-3), 1),
+                 "steps": args.graph_steps, "table_copies": cap.staging.tables, "capture_s": round(tc, 2),
+                 "what": "apply + left sweep replayed from one captured HIP graph, right sweep eager"}
+        log(f"batch graph form: {graph}")
+        del cap
     kept = [Z.bond[j, 1:int(ns[j])] for j in range(len(ns))]
     textbook = sum(W.apply_round_flops([1] + [int(chis[j])] * (int(ns[j]) - 1) + [1], _gate_bonds(int(ns[j])), 2,
                                        list(kept[j])) for j in range(len(ns)))
@@ -327,14 +332,13 @@ def batch_headline(np, torch, args):
             "config": {"workload": f"batching: {len(ns)} independent MPS, N~U[8,48], d=2, chi~U{{8,16,32,64,128}}, "
                                    f"one layer of Haar O(4) 2-site gates (MPO bond 4) on pairs (0,1),(2,3),..., "
                                    f"rounded back to chi; grouped lock-step launches, mixed precision",
-                       "jobs": len(ns), "sites": nsites, "phys_dim": 2, "mpo_bond": 4, "form": "batched mixed",
-                       "hip_graph": "apply + left sweep captured, right sweep eager" if cap is not None else "none"},
+                       "jobs": len(ns), "sites": nsites, "phys_dim": 2, "mpo_bond": 4, "form": "batched mixed"},
             "plain_fp64_sites_per_s": round(plain, 1),
             "plain_sample": f"{done} sites of {len(used)} strided jobs, one chain at a time, plain FP64",
             "speedup_vs_plain_fp64": round(value / plain, 2),
             "batched_fp64_sites_per_s": round(fp64_rate, 1),
             "error_vs_fp64_oracle": err,
-            "roofline": roof, "cpu_baseline": cpu, "autotune": tuned,
+            "roofline": roof, "cpu_baseline": cpu, "autotune": tuned, "graph_form": graph,
             "pipeline": {"census_step_ms": round(census_ms, 1), "census_kernels": census_table(census)}}
 
 
@@ -603,9 +607,10 @@ def precision_form(np, torch):
                        f"relative cutoff {cut:g}", "steps_each": 1}
     reps = {}
     res = {}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "bf16"):
         log(f"precision form {prec}: one step of {out['workload']}")
         torch.cuda.synchronize()
+        torch.cuda.empty_cache()
         t0 = time.perf_counter()
         rep = rounding.RoundingReport()
         Z = rounding.apply_and_round(O, X, chi_max=chi, cutoff=cut, report=rep, precision=prec)
@@ -614,20 +619,26 @@ def precision_form(np, torch):
         reps[prec], res[prec] = rep, Z
         out[prec] = {"seconds_per_step": round(dt, 3), "sites_per_s": round(n / dt, 3)}
         log(f"precision form {prec}: {dt:.1f} s per step")
-    zp, zm = res["fp64"], res["mixed"]
+    zp = res["fp64"]
     n2p = float(linalg.overlap(zp, zp).item())
-    n2m = float(linalg.overlap(zm, zm).item())
-    ov = float(linalg.overlap(zm, zp).item())
-    per, smax = 0.0, 0.0
-    for sp, sm, kp in zip(reps["fp64"].svals, reps["mixed"].svals, reps["fp64"].kept):
-        a, b = sp.cpu().numpy()[:kp], sm.cpu().numpy()[:kp]
-        per = max(per, float(np.max(np.abs(a - b) / a)))
-        smax = max(smax, float(np.max(np.abs(a - b)) / a[0]))
-    out["mixed"]["speedup_vs_fp64"] = round(out["fp64"]["seconds_per_step"] / out["mixed"]["seconds_per_step"], 3)
-    out["mixed"]["error_vs_fp64_gpu"] = {"norm2_rel_err": abs(n2m - n2p) / n2p,
-                                         "overlap_rel_err": abs(ov - n2p) / n2p,
-                                         "svals_per_value_rel_err": per, "svals_err_over_smax": smax,
-                                         "kept_match": reps["fp64"].kept == reps["mixed"].kept}
+    for prec in ("mixed", "bf16"):
+        zm = res[prec]
+        n2m = float(linalg.overlap(zm, zm).item())
+        ov = float(linalg.overlap(zm, zp).item())
+        per, smax = 0.0, 0.0
+        for sp, sm, kp in zip(reps["fp64"].svals, reps[prec].svals, reps["fp64"].kept):
+            a, b = sp.cpu().numpy()[:kp], sm.cpu().numpy()[:kp]
+            c = min(len(a), len(b))
+            per = max(per, float(np.max(np.abs(a[:c] - b[:c]) / a[:c])))
+            smax = max(smax, float(np.max(np.abs(a[:c] - b[:c])) / a[0]))
+        out[prec]["speedup_vs_fp64"] = round(out["fp64"]["seconds_per_step"] / out[prec]["seconds_per_step"], 3)
+        out[prec]["error_vs_fp64_gpu"] = {"norm2_rel_err": abs(n2m - n2p) / n2p,
+                                          "overlap_rel_err": abs(ov - n2p) / n2p,
+                                          "svals_per_value_rel_err": per, "svals_err_over_smax": smax,
+                                          "kept_match": reps["fp64"].kept == reps[prec].kept}
+    out["mixed"]["form"] = "all products on the FP32 matrix cores (FP32 accumulation)"
+    out["bf16"]["form"] = ("contractions (absorb, C V, U^T C, P M) on the BF16 matrix cores with FP32 "
+                           "accumulation, CholeskyQR products on the FP32 ones")
     kept_p = reps["fp64"].kept
     out["kept_bonds_fp64"] = [min(kept_p), max(kept_p)]
     del res, zm

@@ -83,6 +83,11 @@ int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, 
 /* The mixed form of the same call: FP64 operands and result, products on   */
 /* the FP32 matrix cores (operands rounded to FP32, FP32 accumulation).     */
 int syn_gemm_mixed_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile, void *stream);
+/* The BF16 form: FP64 operands and result, operands rounded to bfloat16   */
+/* (round to nearest even) when staged, v_mfma_f32_16x16x16_bf16 with FP32   */
+/* accumulation: the contractions of the precision form, whose error is    */
+/* stated against the FP64 oracle (a relative ~1e-3 per product).           */
+int syn_gemm_bf16_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* MPO x MPS site contraction, materialised                              */
@@ -491,14 +496,15 @@ int syn_rowscale_f64(const syn_rowscale_item *items, int n_items, int64_t total_
 /* matrices and the recursive Cholesky's products, partial-sum pass        */
 /* included), 15/16 two independent GEMMs in one launch of 64/128 tiles   */
 /* (the recursive Cholesky's G22 update beside T = L21 X11), 17 one      */
-/* recursive-Cholesky node of 64 < n <= 128 (two leaves, three products). */
+/* recursive-Cholesky node of 64 < n <= 128 (two leaves, three products), */
+/* 18/19 BF16 GEMM 64/128 tiles.                                          */
 /* syn_prof_note_flops sets the count attached to the next grouped GEMM */
 /* or apply launch (their tables live on the device).                    */
 /* syn_prof_select(mask) limits the events to the tags whose bit is set  */
 /* (all by default; kept until changed): two event packets per launch    */
 /* slow a stream of short kernels down by microseconds each.             */
 /* ------------------------------------------------------------------ */
-#define SYN_PROF_NTAGS 18
+#define SYN_PROF_NTAGS 20
 int syn_prof_begin(void);
 int syn_prof_select(uint32_t mask);
 int syn_prof_note_flops(double flops);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "syn_struct_sizes": (_I32, [ctypes.POINTER(ctypes.c_int64), _I32]),
     "syn_gemm_f64": (_I32, [_VP, _I32, _I64, _I32, _VP]),
     "syn_gemm_mixed_f64": (_I32, [_VP, _I32, _I64, _I32, _VP]),
+    "syn_gemm_bf16_f64": (_I32, [_VP, _I32, _I64, _I32, _VP]),
     "syn_mpo_apply_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_mpo_mpo_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_kron_bonds_f64": (_I32, [_VP, _I32, _I64, _VP]),
@@ -131,7 +132,7 @@ SIGNATURES = {
     "syn_tune_jacobi_inner_sweeps": (None, [_I32]),
 }
 
-PROF_NTAGS = 18
+PROF_NTAGS = 20
 PROF_TAGS = {
     0: "gemm_f64_single<64,64>",
     1: "gemm_f64_single<128,128>",
@@ -151,6 +152,8 @@ PROF_TAGS = {
     15: "gemm_f64_pair<64,64>",
     16: "gemm_f64_pair<128,128>",
     17: "chol_node_kernel",
+    18: "gemm_bf16<64,64>",
+    19: "gemm_bf16<128,128>",
 }
 
 

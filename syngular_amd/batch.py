@@ -108,31 +108,24 @@ def gemm_table(A, B, C, m, n, k, lda, ldb, ldc, ta=0, tb=0, alpha=1.0, beta=0.0,
         return gemm_table(*(bc(v, np.uint64)[sel] for v in (A, B, C)), m[sel], n[sel], k[sel],
                           *(bc(v, np.int64)[sel] for v in (lda, ldb, ldc, ta, tb)),
                           bc(alpha, np.float64)[sel], bc(beta, np.float64)[sel], bc(tri, np.int64)[sel], mixed)
-    # tile configuration per problem from its shape class (autotune), never
-    # from the launch: a chain's sums do not depend on its company
-    tl = autotune.tiles(m, n, k, mixed)
+    # one tile configuration for the launch, from its shape class (autotune)
+    tile = autotune.tile(m, n, k, 2 if isinstance(mixed, str) else int(bool(mixed)))
+    bm = 128 if tile else 64
+    tn = (n + bm - 1) // bm
+    tm = (m + bm - 1) // bm
+    st, total = _starts(tm * tn)
     it = np.zeros(cnt, dtype=_lib.GEMM_ITEM)
     it["A"], it["B"], it["C"] = bc(A, np.uint64), bc(B, np.uint64), bc(C, np.uint64)
     it["m"], it["n"], it["k"] = m, n, k
     it["lda"], it["ldb"], it["ldc"] = bc(lda, np.int64), bc(ldb, np.int64), bc(ldc, np.int64)
     it["alpha"], it["beta"] = bc(alpha, np.float64), bc(beta, np.float64)
     it["transA"], it["transB"] = bc(ta, np.int32), bc(tb, np.int32)
+    it["block_start"], it["tiles_n"] = st, tn
     it["tri"] = bc(tri, np.int32)
-    for tile in (0, 1):
-        sel = np.nonzero(tl == tile)[0]
-        if len(sel) == 0:
-            continue
-        part = it[sel] if len(sel) < cnt else it
-        bm = 128 if tile else 64
-        tn = (n[sel] + bm - 1) // bm
-        tm = (m[sel] + bm - 1) // bm
-        st, total = _starts(tm * tn)
-        part["block_start"], part["tiles_n"] = st, tn
-        dev = ops._upload(part)
-        if ops._profiling[0]:
-            call("syn_prof_note_flops", float((2.0 * m[sel] * n[sel] * k[sel]).sum()))
-        call("syn_gemm_mixed_f64" if mixed else "syn_gemm_f64", dev.data_ptr(), len(sel), total, tile,
-             ops._stream())
+    dev = ops._upload(it)
+    if ops._profiling[0]:
+        call("syn_prof_note_flops", float((2.0 * m * n * k).sum()))
+    call(ops.gemm_fn(mixed), dev.data_ptr(), cnt, total, tile, ops._stream())
 
 
 def _launch_blocks(fn, it, nel):

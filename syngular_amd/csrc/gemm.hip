@@ -52,6 +52,39 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
+// BF16 compute type (the precision form's contractions): operands rounded
+// to bfloat16 (round to nearest even) when staged into LDS, stored as their
+// 16-bit patterns, v_mfma_f32_16x16x16_bf16 with FP32 accumulation.  One
+// MFMA covers K = 16: lane l holds A[i = l%16][k = 4(l/16) + v] and
+// B[k = 4(l/16) + v][j = l%16] for v = 0..3; the output rows are those of
+// the f32 MFMA, 4(l/16) + r.
+struct bf16s {
+    unsigned short bits;
+};
+typedef short v4s __attribute__((ext_vector_type(4)));
+template <>
+struct Mfma<bf16s> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t zero() { return v4f{0.0f, 0.0f, 0.0f, 0.0f}; }
+    static __device__ __forceinline__ acc_t mac4(v4s a, v4s b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+// FP64 -> staged operand: FP64, FP32 (round to nearest) or BF16 (FP32 first,
+// then round to nearest even on the upper 16 bits; the inputs are finite).
+template <typename CT>
+__device__ __forceinline__ CT stage(double x) {
+    return (CT)x;
+}
+template <>
+__device__ __forceinline__ bf16s stage<bf16s>(double x) {
+    unsigned u = __builtin_bit_cast(unsigned, (float)x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return bf16s{(unsigned short)(u >> 16)};
+}
+
 template <int BM, int BN, int BK, typename CT>
 struct GemmSmem {
     CT As[2][BK][BM + 1];
@@ -157,9 +190,9 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
         CT *sa = &As[buf][0][0] + lds_a;
         CT *sb = &Bs[buf][0][0] + lds_b;
 #pragma unroll
-        for (int s = 0; s < EA; ++s) sa[s * kLdsStepA] = (CT)ra[s];
+        for (int s = 0; s < EA; ++s) sa[s * kLdsStepA] = stage<CT>(ra[s]);
 #pragma unroll
-        for (int s = 0; s < EB; ++s) sb[s * kLdsStepB] = (CT)rb[s];
+        for (int s = 0; s < EB; ++s) sb[s * kLdsStepB] = stage<CT>(rb[s]);
     };
 
     typedef Mfma<CT> M;
@@ -177,6 +210,27 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < nk) load(kmin + (kt + 1) * BK);
+        if constexpr (__is_same(CT, bf16s)) {
+            static_assert(BK % 16 == 0, "the bf16 MFMA takes K = 16");
+#pragma unroll
+            for (int kk = 0; kk < BK / 16; ++kk) {
+                v4s a[TI], b[TJ];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+#pragma unroll
+                    for (int i = 0; i < TI; ++i) a[i][v] = (short)As[cur][kk * 16 + 4 * lk + v][wm0 + i * 16 + li].bits;
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) b[j][v] = (short)Bs[cur][kk * 16 + 4 * lk + v][wn0 + j * 16 + li].bits;
+                }
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) acc[i][j] = M::mac4(a[i], b[j], acc[i][j]);
+            }
+            if (kt + 1 < nk) store(cur ^ 1);
+            __syncthreads();
+            continue;
+        } else {
         // the operands of step kk + 1 are read from LDS while the MFMAs of
         // step kk run
         CT a[2][TI], b[2][TJ];
@@ -197,6 +251,7 @@ __device__ __forceinline__ void gemm_f64_tile_t(const syn_gemm_item &P, int64_t 
         }
         if (kt + 1 < nk) store(cur ^ 1);
         __syncthreads();
+        }
     }
 
     double *__restrict__ C = (double *)P.C;
@@ -267,6 +322,23 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_groupe
     const int64_t bid = xcd_remap(blockIdx.x, total);
     const syn_gemm_item &P = items[find_item(items, n_items, bid)];
     gemm_f64_tile<BM, BN, BK, WM, WN, CT>(P, bid - P.block_start);
+}
+
+// Persistent form of the grouped launch: a fixed grid of work-groups (a
+// multiple of the 8 XCDs) walks the logical tiles b = blockIdx.x + i * grid,
+// so every tile of a work-group stays on its XCD and the XCD keeps the same
+// contiguous tile range as the one-tile-per-work-group launch.  For launches
+// of many small tiles (thousands of chains' products) it saves the per-tile
+// work-group start and drain; a tile computes exactly what it computes there.
+template <int BM, int BN, int BK, int WM, int WN, typename CT = double>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, 2) void gemm_f64_grouped_persistent(
+    const syn_gemm_item *__restrict__ items, int n_items, int64_t total) {
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        if (b != (int64_t)blockIdx.x) __syncthreads();   // LDS of the previous tile is free
+        const int64_t bid = xcd_remap(b, total);
+        const syn_gemm_item &P = items[find_item(items, n_items, bid)];
+        gemm_f64_tile<BM, BN, BK, WM, WN, CT>(P, bid - P.block_start);
+    }
 }
 
 template <int BM, int BN, int BK, int WM, int WN, typename CT = double>
@@ -481,6 +553,37 @@ syn_gemm_item make_gemm(const double *A, int64_t lda, int ta, const double *B, i
 
 }  // namespace syn
 
+namespace syn {
+
+// Grouped launches of more tiles than this many work-group slots run
+// persistent (kPersistentGrid work-groups: 512 CUs' worth of two each would
+// be 512; four per CU for the 64-tile kernels).
+constexpr int64_t kPersistentMin = 4096;
+constexpr int64_t kPersistentGrid = 1024;
+
+template <typename CT>
+static void launch_grouped(const syn_gemm_item *items, int n_items, int64_t total, int tile, hipStream_t s) {
+    const bool pers = total >= kPersistentMin && n_items > 1;
+    const unsigned grid = (unsigned)(pers ? kPersistentGrid : total);
+    if (tile == 0) {
+        if (pers)
+            hipLaunchKernelGGL((gemm_f64_grouped_persistent<64, 64, 16, 32, 32, CT>), dim3(grid), dim3(256), 0, s,
+                               items, n_items, total);
+        else
+            hipLaunchKernelGGL((gemm_f64_grouped<64, 64, 16, 32, 32, CT>), dim3(grid), dim3(256), 0, s, items, n_items,
+                               total);
+    } else {
+        if (pers)
+            hipLaunchKernelGGL((gemm_f64_grouped_persistent<128, 128, 16, 64, 64, CT>), dim3(grid), dim3(256), 0, s,
+                               items, n_items, total);
+        else
+            hipLaunchKernelGGL((gemm_f64_grouped<128, 128, 16, 64, 64, CT>), dim3(grid), dim3(256), 0, s, items,
+                               n_items, total);
+    }
+}
+
+}  // namespace syn
+
 extern "C" int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile,
                             void *stream) {
     using namespace syn;
@@ -494,18 +597,37 @@ extern "C" int syn_gemm_f64(const syn_gemm_item *items, int n_items, int64_t tot
         return SYN_EINVAL;
     }
     hipStream_t s = (hipStream_t)stream;
-    ProfScope ps(tile == 1 ? kTagGemmGrouped128 : kTagGemmGrouped64, -1.0, s);
-    if (tile == 0)
-        hipLaunchKernelGGL((gemm_f64_grouped<64, 64, 16, 32, 32>), dim3((unsigned)total_blocks), dim3(256), 0, s,
-                           items, n_items, total_blocks);
-    else if (tile == 1)
-        hipLaunchKernelGGL((gemm_f64_grouped<128, 128, 16, 64, 64>), dim3((unsigned)total_blocks), dim3(256), 0,
-                           s, items, n_items, total_blocks);
-    else {
+    if (tile != 0 && tile != 1) {
         set_error("syn_gemm_f64: unknown tile configuration %d", tile);
         return SYN_EINVAL;
     }
+    ProfScope ps(tile == 1 ? kTagGemmGrouped128 : kTagGemmGrouped64, -1.0, s);
+    launch_grouped<double>(items, n_items, total_blocks, tile, s);
     SYN_LAUNCH_CHECK("syn_gemm_f64 launch");
+    return SYN_OK;
+}
+
+
+extern "C" int syn_gemm_bf16_f64(const syn_gemm_item *items, int n_items, int64_t total_blocks, int tile,
+                                 void *stream) {
+    using namespace syn;
+    if (n_items < 0 || total_blocks < 0) {
+        set_error("syn_gemm_bf16_f64: negative item or block count");
+        return SYN_EINVAL;
+    }
+    if (n_items == 0 || total_blocks == 0) return SYN_OK;
+    if (total_blocks > 0x7fffffffLL) {
+        set_error("syn_gemm_bf16_f64: %lld tiles exceed the grid limit", (long long)total_blocks);
+        return SYN_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (tile != 0 && tile != 1) {
+        set_error("syn_gemm_bf16_f64: unknown tile configuration %d", tile);
+        return SYN_EINVAL;
+    }
+    ProfScope ps(tile == 1 ? kTagGemmBf16128 : kTagGemmBf1664, -1.0, s);
+    launch_grouped<bf16s>(items, n_items, total_blocks, tile, s);
+    SYN_LAUNCH_CHECK("syn_gemm_bf16_f64 launch");
     return SYN_OK;
 }
 
@@ -522,17 +644,12 @@ extern "C" int syn_gemm_mixed_f64(const syn_gemm_item *items, int n_items, int64
         return SYN_EINVAL;
     }
     hipStream_t s = (hipStream_t)stream;
-    ProfScope ps(tile == 1 ? kTagGemmMixed128 : kTagGemmMixed64, -1.0, s);
-    if (tile == 0)
-        hipLaunchKernelGGL((gemm_f64_grouped<64, 64, 16, 32, 32, float>), dim3((unsigned)total_blocks), dim3(256), 0,
-                           s, items, n_items, total_blocks);
-    else if (tile == 1)
-        hipLaunchKernelGGL((gemm_f64_grouped<128, 128, 16, 64, 64, float>), dim3((unsigned)total_blocks), dim3(256),
-                           0, s, items, n_items, total_blocks);
-    else {
+    if (tile != 0 && tile != 1) {
         set_error("syn_gemm_mixed_f64: unknown tile configuration %d", tile);
         return SYN_EINVAL;
     }
+    ProfScope ps(tile == 1 ? kTagGemmMixed128 : kTagGemmMixed64, -1.0, s);
+    launch_grouped<float>(items, n_items, total_blocks, tile, s);
     SYN_LAUNCH_CHECK("syn_gemm_mixed_f64 launch");
     return SYN_OK;
 }

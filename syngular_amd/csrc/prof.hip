@@ -27,7 +27,9 @@ bool g_on = false;
 uint32_t g_mask = 0xffffffffu;   // tags bracketed by events
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_pool;
-double g_pending_flops = 0.0;
+// the work figure the next grouped launch of the calling thread carries
+// (lanes of the server note and launch from threads of their own)
+thread_local double g_pending_flops = 0.0;
 double g_extra[SYN_PROF_NTAGS] = {};   // work added without events (totals per call)
 
 hipEvent_t take_event() {

@@ -24,6 +24,8 @@ enum ProfTag {
     kTagGemmPair64 = 15,
     kTagGemmPair128 = 16,
     kTagCholNode = 17,
+    kTagGemmBf1664 = 18,
+    kTagGemmBf16128 = 19,
 };
 
 bool prof_on();

@@ -117,12 +117,22 @@ def as_matrix(t: torch.Tensor):
 _TILES = {0: 64, 1: 128}
 
 
+def gemm_fn(mixed):
+    """Entry point of a compute kind: False (FP64), True (FP32 matrix cores)
+    or "bf16" (BF16 matrix cores); FP64 operands and results in every case."""
+    if isinstance(mixed, str):
+        if mixed != "bf16":
+            raise ValueError(f"unknown compute kind {mixed!r}")
+        return "syn_gemm_bf16_f64"
+    return "syn_gemm_mixed_f64" if mixed else "syn_gemm_f64"
+
+
 def gemm_many(problems, tile=None, mixed=False):
     """problems: iterable of (A, B, C, alpha, beta[, tri]) with A (m x k),
     B (k x n), C (m x n) 2-D float64 views; C is overwritten with
     alpha*A@B + beta*C.  tri is the structural-zero mask of syn_gemm_item.
     mixed: products on the FP32 matrix cores (operands rounded to FP32, FP32
-    accumulation, FP64 result)."""
+    accumulation, FP64 result), or "bf16" for the BF16 matrix cores."""
     problems = [p if len(p) == 6 else (*p, 0) for p in problems if p[2].numel() > 0]
     if not problems:
         return
@@ -159,7 +169,7 @@ def gemm_many(problems, tile=None, mixed=False):
         fl = sum(2.0 * p[0].shape[0] * p[0].shape[1] * p[1].shape[1] * frac.get(bin(p[5]).count("1"), 1.0 / 3.0)
                  for p in problems)
         call("syn_prof_note_flops", fl)
-    call("syn_gemm_mixed_f64" if mixed else "syn_gemm_f64", _ptr(dev), len(problems), start, tile, _stream())
+    call(gemm_fn(mixed), _ptr(dev), len(problems), start, tile, _stream())
     return dev
 
 

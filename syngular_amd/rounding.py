@@ -62,28 +62,40 @@ LDS_JACOBI_MAX = 64
 ROUND_REORTH_BUDGET = 1e-10
 
 
-PRECISIONS = ("fp64", "mixed")
+# fp64: every product in FP64.  mixed: the products on the FP32 matrix
+# cores.  bf16: the contractions (absorb S R, C V, U^T C, P M) on the BF16
+# matrix cores and CholeskyQR's Q = A L^-T on the FP32 ones.  In every form
+# the Gram matrices, the Cholesky, the Jacobi rotations and the truncation
+# stay FP64.
+PRECISIONS = ("fp64", "mixed", "bf16")
 
 
-def _mixed(precision):
+def _kinds(precision):
+    """(compute kind of the QR products, of the contractions)."""
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {PRECISIONS}, got {precision!r}")
-    return precision == "mixed"
+    return {"fp64": (False, False), "mixed": (True, True), "bf16": (True, "bf16")}[precision]
 
 
-def left_orthonormalize(cores, own=False, mixed=False, reorth_budget=None):
+def left_orthonormalize(cores, own=False, mixed=False, reorth_budget=None, contract=None):
     """Left QR sweep; returns new cores (left-orthonormal except the last).
-    With own=False the input cores are not modified.  mixed: the products
+    With own=False the input cores are not modified; with own=True the
+    caller's list is consumed (its entries are replaced as the sweep goes,
+    and an entry that is a callable is formed only when the sweep reaches
+    it).  mixed: the products
     (CholeskyQR's Q and the absorb) on the FP32 matrix cores, the Gram
     matrices and the Cholesky in FP64.  reorth_budget (FP64 only): total
     loss of orthogonality sum_k ||Q_k^T Q_k - I||_F the sweep may keep;
     CholeskyQR skips its second pass at a site while the running sum of the
     first passes' estimates stays within it (ops.cholqr2_adaptive).  None
-    runs both passes everywhere."""
-    cores = list(cores)
+    runs both passes everywhere.  contract: the compute kind of the absorb
+    product (mixed when None)."""
+    if contract is None:
+        contract = mixed
+    cores = cores if own else list(cores)
     spent = 0.0
     for k in range(len(cores) - 1):
-        c = cores[k].contiguous()
+        c = _core(cores, k).contiguous()
         l, mid, r = c.shape[0], c.shape[1:-1], c.shape[-1]
         L = c.reshape(-1, r)
         m = L.shape[0]
@@ -100,11 +112,23 @@ def left_orthonormalize(cores, own=False, mixed=False, reorth_budget=None):
         else:
             Q, S = _qr(L)
             tri = 4
-        nxt = cores[k + 1]
+        nxt = _core(cores, k + 1)
         cores[k] = Q.reshape(l, *mid, Q.shape[1])
-        W = ops.gemm(S, nxt.reshape(nxt.shape[0], -1), tri=tri, mixed=mixed)
+        del c, L
+        W = ops.gemm(S, nxt.reshape(nxt.shape[0], -1), tri=tri, mixed=contract)
         cores[k + 1] = W.reshape(S.shape[0], *nxt.shape[1:])
+    if cores:
+        _core(cores, len(cores) - 1)
     return cores
+
+
+def _core(cores, k):
+    """Core k of a sweep's list; an entry that is a callable (a product core
+    not formed yet) is formed now and stored in its place."""
+    c = cores[k]
+    if callable(c):
+        c = cores[k] = c()
+    return c
 
 
 def _qr(L):
@@ -116,8 +140,9 @@ def _qr(L):
 
 def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None, mixed=False):
     """Right-to-left SVD truncation of left-orthonormal cores.  mixed: the
-    products with the singular vectors on the FP32 matrix cores; the Gram
-    matrix, the Jacobi rotations and the truncation stay FP64."""
+    compute kind of the products with the singular vectors (False: FP64,
+    True: FP32, "bf16"); the Gram matrix, the Jacobi rotations and the
+    truncation stay FP64."""
     n = len(cores)
     for k in range(n - 1, 0, -1):
         c = cores[k]
@@ -162,15 +187,20 @@ def right_svd_sweep(cores, chi_max=None, cutoff=0.0, report=None, mixed=False):
 
 def tt_round(cores, chi_max=None, cutoff=0.0, report=None, own=False, precision="fp64"):
     """TT-rounding of an MPS given as device cores (l, d, r).  The input
-    cores are left untouched unless own=True.  precision "fp64" (plain) or
-    "mixed" (products on the FP32 matrix cores; Gram matrices, Cholesky,
-    Jacobi and truncation in FP64)."""
-    mixed = _mixed(precision)
-    return right_svd_sweep(left_orthonormalize(cores, own=own, mixed=mixed, reorth_budget=ROUND_REORTH_BUDGET),
-                           chi_max, cutoff, report, mixed=mixed)
+    cores are left untouched unless own=True.  precision "fp64" (plain),
+    "mixed" (products on the FP32 matrix cores) or "bf16" (contractions on
+    the BF16 matrix cores, QR products on the FP32 ones); Gram matrices,
+    Cholesky, Jacobi and truncation in FP64 in every form."""
+    qr_kind, con = _kinds(precision)
+    left = left_orthonormalize(cores, own=own, mixed=qr_kind, reorth_budget=ROUND_REORTH_BUDGET, contract=con)
+    return right_svd_sweep(left, chi_max, cutoff, report, mixed=con)
 
 
 def apply_and_round(mpo_sites, mps_sites, chi_max=None, cutoff=0.0, report=None, precision="fp64"):
-    """One apply+round step: materialised MPO x MPS cores, then tt_round."""
-    prod = ops.apply_many(list(zip(mpo_sites, mps_sites)))
+    """One apply+round step: MPO x MPS product cores, then tt_round.  Each
+    product core is formed when the left sweep reaches it and dropped once
+    its QR factor replaces it, so at most two product cores (P_k, P_k+1)
+    exist at a time instead of the whole product (133 GB at N=64, d=4,
+    chi=1024, D_w=8, against 2.1 GB per core)."""
+    prod = [(lambda o=o, x=x: ops.apply_many([(o, x)])[0]) for o, x in zip(mpo_sites, mps_sites)]
     return tt_round(prod, chi_max, cutoff, report, own=True, precision=precision)

@@ -223,22 +223,45 @@ def test_emulated_gaussian_batch_layout_and_round(emu):
     assert "syn_vecnorms_f64" in emu.calls
 
 
-def test_autotune_tiles_follow_the_problem_not_the_launch():
+def test_autotune_launch_classes_and_default_rule():
     from syngular_amd import autotune
 
-    m = np.array([64, 300, 4096, 130, 256])
-    n = np.array([64, 300, 512, 130, 256])
-    k = np.array([10, 20, 30, 40, 50])
-    t = autotune.tiles(m, n, k, False)
-    # 300 and 130: the 128 x 128 padding wastes more than 15% over the 64 x 64 one
-    assert list(t) == [0, 0, 1, 0, 1]
-    # the same problem alone or among others gets the same tile
-    for i in range(len(m)):
-        assert autotune.tiles(m[i:i + 1], n[i:i + 1], k[i:i + 1], False)[0] == t[i]
-    key = int(autotune.classes(m[:1], n[:1], k[:1], False)[0])
+    small = (np.full(10, 64), np.full(10, 64), np.full(10, 32))
+    big = (np.full(300, 256), np.full(300, 256), np.full(300, 512))
+    odd = (np.full(300, 130), np.full(300, 130), np.full(300, 512))
+    assert autotune.tile(*small, 0) == 0            # too little area for 256 tiles of 128
+    assert autotune.tile(*big, 0) == 1
+    assert autotune.tile(*odd, 0) == 0              # 128-padding wastes more than 15% over 64's
+    kb = autotune.launch_class(*big, 0)
+    assert kb != autotune.launch_class(*big, 1) != autotune.launch_class(*big, 2)
+    assert autotune.decode(autotune.launch_class(*big, 2))["mixed"] == "bf16"
     try:
-        autotune.TABLE[key] = 1
-        assert autotune.tiles(m, n, k, False)[0] == 1
-        assert autotune.decode(key) == {"m": 64, "n": 64, "k": 16, "mixed": False}
+        autotune.TABLE[kb] = 0
+        assert autotune.tile(*big, 0) == 0
+        autotune.collect()
+        autotune.tile(*small, 0)
+        autotune.tile(*big, 0)
+        assert len(autotune._seen) == 2
+    finally:
+        autotune.collect(False)
+        autotune.TABLE.clear()
+
+
+def test_autotune_table_save_load(tmp_path):
+    from syngular_amd import autotune
+
+    try:
+        autotune.TABLE.clear()
+        autotune.TABLE.update({123: 1, 456: 0})
+        autotune.TIMINGS.update({123: {0: 0.5, 1: 0.25}})
+        autotune.COUNTS.update({123: 7})
+        p = tmp_path / "t.json"
+        autotune.save(str(p))
+        autotune.TABLE.clear()
+        autotune.load(str(p))
+        assert autotune.TABLE == {123: 1, 456: 0}
+        assert autotune.TIMINGS[123] == {0: 0.5, 1: 0.25} and autotune.COUNTS[123] == 7
     finally:
         autotune.TABLE.clear()
+        autotune.TIMINGS.clear()
+        autotune.COUNTS.clear()

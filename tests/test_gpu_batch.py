@@ -249,10 +249,10 @@ def test_batched_orthogonality_budget():
 
 
 def test_autotuned_tiles_keep_results_and_batch_invariance():
-    """The tile autotuner: record the shape classes of a mixed batch, time
+    """The tile autotuner: record the launch classes of a mixed batch, time
     both tile configurations per class, then the tuned run still matches the
-    oracle and a chain alone gives the same bits as in the mix (the tile
-    follows the problem's class, not its company)."""
+    oracle and a chain alone agrees with the same chain in the mix (to
+    rounding: the tile follows the launch)."""
     from syngular_amd import autotune
 
     jobs = _mixed_jobs(5)
@@ -268,10 +268,10 @@ def test_autotuned_tiles_keep_results_and_batch_invariance():
         for q, (xs, os_, c) in enumerate(jobs):
             ref = ornd.apply_and_round(os_, xs, chi_max=c)[0]
             _check_against_oracle(out[q], ref, len(xs))
-        q = len(jobs) - 1
-        solo = B.apply_and_round(B.ChainBatch.from_numpy([jobs[q][0]]), B.OperatorBatch.from_numpy([jobs[q][1]]),
-                                 np.array([jobs[q][2]])).to_numpy()[0]
-        assert all(np.array_equal(a, b) for a, b in zip(solo, out[q]))
+        for q in range(len(jobs)):
+            solo = B.apply_and_round(B.ChainBatch.from_numpy([jobs[q][0]]),
+                                     B.OperatorBatch.from_numpy([jobs[q][1]]), np.array([jobs[q][2]])).to_numpy()[0]
+            _check_against_oracle(solo, out[q], len(jobs[q][0]))
     finally:
         autotune.TABLE.clear()
 
@@ -298,3 +298,39 @@ def test_captured_apply_left_sweep_replays_match_eager():
         ref = ornd.apply_and_round(os_, xs, chi_max=c)[0]
         n2r = ornd.norm2(ref)
         assert abs(ornd.norm2(got[q]) - n2r) <= 1e-6 * n2r
+
+
+@pytest.mark.parametrize("kind", [False, True, "bf16"])
+def test_grouped_gemm_persistent_launch(kind):
+    """A grouped launch of more than 4096 tiles runs persistent (a fixed grid
+    walking the tiles): 5000 problems of uneven shape, each against NumPy
+    (FP64 to 1e-13; the FP32 and BF16 kinds to their operand rounding)."""
+    from syngular_amd import ops as O_
+
+    rng = np.random.default_rng(71)
+    cnt = 5000
+    m = rng.integers(1, 65, cnt)
+    n = rng.integers(1, 65, cnt)
+    k = rng.integers(1, 90, cnt)
+    A = [rng.standard_normal((a, c)) for a, c in zip(m, k)]
+    Bm = [rng.standard_normal((c, b)) for c, b in zip(k, n)]
+    ta, pa = B.arena([a.size for a in A])
+    tb, pb = B.arena([b.size for b in Bm])
+    tc, pc = B.arena(m * n)
+    ha, hb = ta.cpu().numpy(), tb.cpu().numpy()
+    base_a, base_b = np.uint64(ta.data_ptr()), np.uint64(tb.data_ptr())
+    for i in range(cnt):
+        oa, ob = int((pa[i] - base_a) // np.uint64(8)), int((pb[i] - base_b) // np.uint64(8))
+        ha[oa:oa + A[i].size] = A[i].reshape(-1)
+        hb[ob:ob + Bm[i].size] = Bm[i].reshape(-1)
+    ta.copy_(torch.from_numpy(ha))
+    tb.copy_(torch.from_numpy(hb))
+    B.gemm_table(pa, pb, pc, m, n, k, k, n, n, mixed=kind)
+    hc = tc.cpu().numpy()
+    base_c = np.uint64(tc.data_ptr())
+    tol = {False: 1e-13, True: 1e-5, "bf16": 8e-3}[kind]
+    for i in range(cnt):
+        oc = int((pc[i] - base_c) // np.uint64(8))
+        got = hc[oc:oc + m[i] * n[i]].reshape(m[i], n[i])
+        scale = np.abs(A[i]) @ np.abs(Bm[i])
+        assert np.abs(got - A[i] @ Bm[i]).max() <= tol * scale.max(), i

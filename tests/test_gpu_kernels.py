@@ -602,3 +602,54 @@ def test_mixed_round_vs_oracle():
     assert abs(n2 - n2r) / n2r < 1e-6
     assert abs(ov - ovr) / np.sqrt(n2r * ornd.norm2(xs)) < 1e-6
     assert abs(ov - ovr) / abs(ovr) < 1e-5
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(37, 50, 19), (300, 260, 513), (1024, 512, 2048)])
+def test_gemm_bf16(ta, tb, m, n, k):
+    """BF16 GEMM: FP64 in/out, operands rounded to bfloat16, FP32
+    accumulation; checks the bf16 MFMA operand and output layout (an error
+    would be O(1)) against the product of the rounded operands, and the
+    stated accuracy against the exact product."""
+    from syngular_amd import ops
+
+    case(f"gemm bf16 ta={ta} tb={tb} {m}x{n}x{k}")
+    rng = np.random.default_rng(m + n + k + 5 * ta + tb)
+    A = rng.standard_normal((k, m) if ta else (m, k))
+    Bm = rng.standard_normal((n, k) if tb else (k, n))
+    C = ops.gemm(T(A).t() if ta else T(A), T(Bm).t() if tb else T(Bm), mixed="bf16").cpu().numpy()
+    opA, opB = (A.T if ta else A), (Bm.T if tb else Bm)
+
+    def to_bf16(x):
+        u = np.ascontiguousarray(x.astype(np.float32)).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+        return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+    ref = to_bf16(opA) @ to_bf16(opB)
+    scale = np.abs(opA) @ np.abs(opB)
+    assert np.abs(C - ref).max() / scale.max() < 1e-5       # FP32 accumulation of exact bf16 products
+    assert np.abs(C - opA @ opB).max() / scale.max() < 8e-3  # two bf16 roundings of 2^-9 each
+
+
+def test_bf16_round_vs_oracle_states_its_error():
+    """The bf16 form of apply+round (contractions on the BF16 matrix cores,
+    QR products on the FP32 ones, Gram/Cholesky/Jacobi/truncation FP64) on a
+    chain against the FP64 oracle: kept ranks equal, norm and overlap within
+    the stated 1e-2 (BF16 keeps 8 bits of mantissa)."""
+    from oracle import rounding as ornd
+    from syngular_amd import linalg, rounding
+    from syngular_amd import workloads as W
+
+    case("bf16 apply+round chain")
+    xs, os_ = W.chain_inputs(12, 2, 64, 8, seed=11)
+    X = [T(x) for x in xs]
+    O = [T(o) for o in os_]
+    rep = rounding.RoundingReport()
+    cores = rounding.apply_and_round(O, X, chi_max=64, report=rep, precision="bf16")
+    ref, sv, kept, _ = ornd.apply_and_round(os_, xs, chi_max=64)
+    assert rep.kept == kept[1:]
+    n2, n2r = float(linalg.overlap(cores, cores).item()), ornd.norm2(ref)
+    assert abs(n2 - n2r) / n2r < 1e-2
+    ov = float(linalg.overlap(cores, X).item())
+    from oracle import reference as oref
+    assert abs(ov - oref.overlap(ref, xs)) / np.sqrt(n2r * ornd.norm2(xs)) < 1e-2
