@@ -171,6 +171,12 @@ typedef struct {
     double *tau;
 } syn_qr_item;
 int syn_qr_f64(const syn_qr_item *items, int n_items, void *stream);
+/* The same with the launch's dynamic LDS sized by the caller: lds_doubles  */
+/* (at most 12288, the 96 KB of syn_qr_f64); a matrix whose leading block  */
+/* m * min(n, keep) fits works in LDS, a larger one in its workspace.  A   */
+/* batch of small matrices launched with a small size keeps more           */
+/* work-groups on each compute unit.                                        */
+int syn_qr_sized_f64(const syn_qr_item *items, int n_items, int64_t lds_doubles, void *stream);
 
 /* Panel of the blocked Householder QR (one work-group of 1024 threads). */
 /* Factors columns [j0, j0+b) of the m-row matrix w (row-major, ldw;     */

@@ -95,6 +95,7 @@ SIGNATURES = {
     "syn_kron_bonds_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_direct_sum_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_qr_f64": (_I32, [_VP, _I32, _VP]),
+    "syn_qr_sized_f64": (_I32, [_VP, _I32, _I64, _VP]),
     "syn_lu_sign_leaf_f64": (_I32, [_VP, _I32, _D, _VP]),
     "syn_hh_panel_f64": (_I32, [_VP, _I64, _VP, _I64, _VP, _VP, _I64, _I64, _I64, _I64, _VP]),
     "syn_colscale_f64": (_I32, [_VP, _I32, _I64, _VP]),

@@ -185,6 +185,11 @@ def apply_table(x, o, p, chil, din, chir, dl, dout, dr):
     _launch_blocks("syn_mpo_apply_f64", it, nel)
 
 
+# LDS classes (doubles) of the grouped Householder QR: 16, 48 and 96 KB;
+# matrices above the last work from their workspace
+QR_LDS_CLASSES = (2048, 6144, 12288)
+
+
 def qr_table(a, m, n, q, work, tau):
     """Householder QR of every row-major m x n (m > n) matrix: q (m x n) gets
     the thin orthogonal factor."""
@@ -193,9 +198,22 @@ def qr_table(a, m, n, q, work, tau):
     it["m"], it["n"], it["keep"] = m, n, n
     it["q"], it["ldq"], it["q_trans"] = q, n, 0
     it["work"], it["tau"] = work, tau
-    if len(it):
-        dev = ops._upload(it)
-        call("syn_qr_f64", dev.data_ptr(), len(it), ops._stream())
+    # one launch per LDS class of the matrices, each with the dynamic LDS its
+    # largest matrix needs: small matrices do not hold a compute unit's LDS
+    # for the large ones
+    need = np.asarray(m, dtype=np.int64) * np.asarray(n, dtype=np.int64)
+    cls = np.searchsorted(QR_LDS_CLASSES, need, side="left")
+    for c in range(len(QR_LDS_CLASSES) + 1):
+        sel = np.nonzero(cls == c)[0]
+        if len(sel) == 0:
+            continue
+        dev = ops._upload(it[sel])
+        lds = int(need[sel].max()) if c < len(QR_LDS_CLASSES) else 0
+        call("syn_qr_sized_f64", dev.data_ptr(), len(sel), lds, ops._stream())
+
+
+# size classes (matrix order) of the LDS Jacobi launches
+JACOBI_LDS_CLASSES = (16, 32, 64)
 
 
 def jacobi_table(x, g, rt, s):
@@ -205,9 +223,18 @@ def jacobi_table(x, g, rt, s):
     it["x"], it["ldx"], it["nv"], it["len"] = x, g, g, g
     it["rt"], it["ldr"], it["s"], it["sweeps"] = rt, g, s, 0
     it["gram"] = 1
-    if len(it):
-        dev = ops._upload(it)
-        call("syn_jacobi_gram_f64", dev.data_ptr(), len(it), int(np.max(g)), TOL_LDS, ops.JACOBI_SWEEPS,
+    # one launch per size class: the launch's LDS follows its largest matrix,
+    # so small matrices launched apart keep more work-groups on a compute unit
+    g = np.asarray(g, dtype=np.int64)
+    if len(g) and g.max() > JACOBI_LDS_CLASSES[-1]:
+        raise ValueError(f"LDS Jacobi takes order <= {JACOBI_LDS_CLASSES[-1]}, got {g.max()}")
+    cls = np.searchsorted(JACOBI_LDS_CLASSES, g, side="left")
+    for c in range(len(JACOBI_LDS_CLASSES)):
+        sel = np.nonzero(cls == c)[0]
+        if len(sel) == 0:
+            continue
+        dev = ops._upload(it[sel])
+        call("syn_jacobi_gram_f64", dev.data_ptr(), len(sel), int(g[sel].max()), TOL_LDS, ops.JACOBI_SWEEPS,
              ops._stream())
 
 
@@ -370,7 +397,13 @@ def jacobi_gram_blocked_table(x, g, rt, s):
     sweeps = ctypes.c_int(0)
     call("syn_jacobi_gram_blocked_f64", dev_items.data_ptr(), it.ctypes.data_as(ctypes.c_void_p), cnt,
          work.data_ptr(), counters.data_ptr(), TOL_BLOCKED, ops.JACOBI_SWEEPS, ctypes.byref(sweeps), ops._stream())
+    if JACOBI_LOG is not None:
+        JACOBI_LOG.append((g.copy(), sweeps.value))
     return sweeps.value
+
+
+# when a list: every blocked Gram Jacobi call appends (orders, sweeps)
+JACOBI_LOG = None
 
 
 def truncate_table(s, g, chi_max, cutoff, keep, disc):

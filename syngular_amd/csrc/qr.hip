@@ -135,11 +135,14 @@ __device__ void qr_one(const syn_qr_item &P, double *lds_w) {
     }
 }
 
-__global__ __launch_bounds__(kQrThreads) void qr_kernel(const syn_qr_item *__restrict__ items, int n_items) {
+// lds_cap: doubles of dynamic LDS the launch holds; a matrix whose leading
+// block fits works in LDS, a larger one in its workspace.
+__global__ __launch_bounds__(kQrThreads) void qr_kernel(const syn_qr_item *__restrict__ items, int n_items,
+                                                         int64_t lds_cap) {
     extern __shared__ double lds_w[];
     const syn_qr_item &P = items[blockIdx.x];
     const int64_t nb = P.n < P.keep ? P.n : P.keep;
-    if (P.m * nb <= kQrLdsDoubles)
+    if (P.m * nb <= lds_cap)
         qr_one<true>(P, lds_w);
     else
         qr_one<false>(P, lds_w);
@@ -156,13 +159,38 @@ extern "C" int syn_qr_f64(const syn_qr_item *items, int n_items, void *stream) {
     if (n_items == 0) return SYN_OK;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(kQrLdsDoubles * sizeof(double)));
+        if (int rc = hip_check(hipFuncSetAttribute((const void *)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)(kQrLdsDoubles * sizeof(double))),
+                               "syn_qr_f64: hipFuncSetAttribute"))
+            return rc;
         attr_set = true;
     }
     ProfScope ps(kTagQr, 0.0, (hipStream_t)stream);
     hipLaunchKernelGGL(qr_kernel, dim3(n_items), dim3(kQrThreads), kQrLdsDoubles * sizeof(double),
-                       (hipStream_t)stream, items, n_items);
+                       (hipStream_t)stream, items, n_items, kQrLdsDoubles);
     SYN_LAUNCH_CHECK("syn_qr_f64 launch");
+    return SYN_OK;
+}
+
+extern "C" int syn_qr_sized_f64(const syn_qr_item *items, int n_items, int64_t lds_doubles, void *stream) {
+    using namespace syn;
+    if (n_items < 0 || lds_doubles < 0 || lds_doubles > kQrLdsDoubles) {
+        set_error("syn_qr_sized_f64: bad item count %d or LDS size %lld (at most %lld doubles)", n_items,
+                  (long long)lds_doubles, (long long)kQrLdsDoubles);
+        return SYN_EINVAL;
+    }
+    if (n_items == 0) return SYN_OK;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (int rc = hip_check(hipFuncSetAttribute((const void *)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)(kQrLdsDoubles * sizeof(double))),
+                               "syn_qr_sized_f64: hipFuncSetAttribute"))
+            return rc;
+        attr_set = true;
+    }
+    ProfScope ps(kTagQr, 0.0, (hipStream_t)stream);
+    hipLaunchKernelGGL(qr_kernel, dim3(n_items), dim3(kQrThreads), (size_t)lds_doubles * sizeof(double),
+                       (hipStream_t)stream, items, n_items, lds_doubles);
+    SYN_LAUNCH_CHECK("syn_qr_sized_f64 launch");
     return SYN_OK;
 }

@@ -172,6 +172,27 @@ __device__ __forceinline__ int tj_find(const syn_gbj_item *items, int n, int64_t
     return lo;
 }
 
+// Work-group owner table of a sweep's fused steps: entry b < ss names the
+// problem of solve b, entry ss + t the problem of tile t (one work-group per
+// problem fills its runs), so a work-group finds its problem with one load.
+template <int TB>
+__global__ __launch_bounds__(256) void tj_owner_kernel(const syn_gbj_item *__restrict__ items, int64_t ss,
+                                                        int32_t *__restrict__ owner) {
+    const int i = blockIdx.x;
+    const syn_gbj_item &P = items[i];
+    const TjDims D = tj_dims<TB>(P.nv);
+    for (int64_t b = threadIdx.x; b < D.half; b += 256) owner[P.solve_start + b] = i;
+    const int64_t nt = D.half * D.half + D.half * D.ncs;
+    for (int64_t t = threadIdx.x; t < nt; t += 256) owner[ss + P.tile_start + t] = i;
+}
+
+// owner entries (int32) of a problem in the fused steps' table
+template <int TB>
+static int64_t tj_owner_count(int64_t nv) {
+    const TjDims D = tj_dims<TB>(nv);
+    return D.half + D.half * D.half + D.half * D.ncs;
+}
+
 // Per-problem scratch layout (doubles): two sets of W tiles (half * TP^2
 // each) and of rotation flags (half ints each), selected by step parity so a
 // step's pair solves can write theirs while the previous step's update reads
@@ -450,7 +471,7 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
         if (a > b) return;
         const int fa = flags[a], fb = flags[b];
         if (!fa && !fb) return;
-        if (tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);  // read one tile, write one or two
+        if (bytes && tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);  // read one tile, write one or two
         if (tid < TP) {
             ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
             rb[tid] = tj_row<TB>(st, b, D.np, D.nblk, P.nv, tid);
@@ -504,7 +525,7 @@ __global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_i
     const int64_t u = t - D.half * D.half;
     const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
     if (!flags[a]) return;
-    if (tid == 0) atomicAdd(bytes, kSlabBytes);
+    if (bytes && tid == 0) atomicAdd(bytes, kSlabBytes);
     if (tid < TP) ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
     __syncthreads();
     const double *wa = P.w + a * TP * TP;
@@ -565,15 +586,31 @@ __device__ __forceinline__ void tj_owner(int64_t x, int64_t sp, int64_t np, int6
 // pairs A and B at step - 1, S = V^T G_old V (two small MFMA products).
 // Solves come first in the grid (they are the long ones).  W tiles and
 // flags alternate between two sets by step parity.
+// Work-groups are dealt to the eight XCDs round-robin by launch index; the
+// two sections of the grid (solves, then tiles) are each padded to a multiple
+// of 8 and read in XCD-major order, so XCD x takes a contiguous run of each
+// section: the solves and tiles of neighbouring problems, whose W tiles and
+// Gram rows then meet in that XCD's L2 instead of being fetched by all eight.
+__device__ __forceinline__ int64_t tj_xcd_order(int64_t b, int64_t padded) {
+    return (b & 7) * (padded >> 3) + (b >> 3);
+}
+
 template <int TB>
 __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *__restrict__ items, int n,
-                                                          int64_t step, int has_update, int64_t solve_total,
-                                                          int cur, double tol, int inner_sweeps,
-                                                          unsigned long long *bytes) {
+                                                          const int32_t *__restrict__ owner_s,
+                                                          const int32_t *__restrict__ owner_t, int64_t step,
+                                                          int has_update, int64_t solve_pad, int64_t solve_count,
+                                                          int64_t tile_pad, int64_t tile_count, int cur, double tol,
+                                                          int inner_sweeps, unsigned long long *bytes) {
     constexpr int TP = 2 * TB, NT = TB * TB;
     static_assert(TB == 16 && NT == kTUpdThreads, "fused steps: 16-row blocks, one block size for both roles");
-    constexpr int kS = TP * (TP + 1), kV = 2 * TP * (TP + 1), kY = TP * (2 * TP + 1), kX = TP * (kTCol + 1);
-    constexpr int kSolve = 2 * kS + kV + kY, kUpd = 3 * kS + kX;
+    // LDS: the solve role holds S, the two nonzero TP x TB blocks of V
+    // (which W reuses once the tile is built) and Y; a G-tile update holds
+    // Wa, Wb, T and its TP x TP tile, an rt-slab update Wa and the slab.
+    // 4224 doubles (33 KiB) keep four work-groups on a CU.
+    constexpr int kS = TP * (TP + 1), kV = 2 * TP * (TB + 1), kY = TP * (2 * TP + 1), kX = TP * (kTCol + 1);
+    static_assert(kV >= kS, "W reuses the V blocks");
+    constexpr int kSolve = kS + kV + kY, kUpd = (4 * kS > kS + kX) ? 4 * kS : kS + kX;
     __shared__ double smem[kSolve > kUpd ? kSolve : kUpd];
     __shared__ v2d cs[TP / 2];
     __shared__ int rows[TP], rb_[TP];
@@ -588,15 +625,19 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
         ld = P.nv;
         return P.w + tj_g2_off<TB>(half, P.nv);
     };
-    if (bid < solve_total) {
+    if (bid < solve_pad) {
         // ---- pair solve of `step` ------------------------------------------
         double(*S)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem);
+        // Vb[h] (TP x TB): the nonzero block h of V; W takes their place
+        // after the tile build
+        double(*Vb)[TP][TB + 1] = reinterpret_cast<double(*)[TP][TB + 1]>(smem + kS);
         double(*W)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + kS);
-        double(*V)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + 2 * kS);
-        double(*Y)[2 * TP + 1] = reinterpret_cast<double(*)[2 * TP + 1]>(smem + 2 * kS + kV);
-        const syn_gbj_item &P = items[tj_find<true>(items, n, bid)];
+        double(*Y)[2 * TP + 1] = reinterpret_cast<double(*)[2 * TP + 1]>(smem + kS + kV);
+        const int64_t sb = tj_xcd_order(bid, solve_pad);
+        if (sb >= solve_count) return;
+        const syn_gbj_item &P = items[owner_s[sb]];
         const TjDims D = tj_dims<TB>(P.nv);
-        const int64_t pw = bid - P.solve_start;
+        const int64_t pw = sb - P.solve_start;
         const int64_t st = step % (D.np - 1);
         int64_t ldc;
         const double *Gc = gbuf(P, cur, D.half, ldc);
@@ -650,28 +691,22 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
             const double *WB = tj_wset<TB>(P, D.half, pp) + wB * TP * TP;
             // V (2TP x TP): columns c < TB are column hA TB + c of W_A on rows
             // 0 .. TP-1, columns c >= TB column hB TB + c - TB of W_B on rows
-            // TP .. 2TP-1 (identity columns for a pair that did not rotate)
-            // every W load of a thread is issued before the first is used
-            // (from the tile its column block needs; unused rows load too)
-            constexpr int QV = 2 * TP * TP / NT;
+            // TP .. 2TP-1 (identity columns for a pair that did not rotate),
+            // zero elsewhere; only the two nonzero blocks are stored:
+            // Vb[0][j][c] = V[j][c], Vb[1][j][c] = V[TP + j][TB + c].
+            // Every W load of a thread is issued before the first is used.
+            constexpr int QV = 2 * TP * TB / NT;
             double wpre[QV];
 #pragma unroll
             for (int q = 0; q < QV; ++q) {
-                const int e = tid + NT * q, j = e / TP, c = e % TP;
-                wpre[q] = (c < TB) ? WA[(j % TP) * TP + hA * TB + c] : WB[(j % TP) * TP + hB * TB + c - TB];
+                const int e = tid + NT * q, h = e / (TP * TB), j = (e % (TP * TB)) / TB, c = e % TB;
+                wpre[q] = h ? WB[j * TP + hB * TB + c] : WA[j * TP + hA * TB + c];
             }
 #pragma unroll
             for (int q = 0; q < QV; ++q) {
-                const int e = tid + NT * q, j = e / TP, c = e % TP;
-                double v = 0.0;
-                if (c < TB && j < TP) {
-                    const int col = hA * TB + c;
-                    v = fA ? wpre[q] : (j == col ? 1.0 : 0.0);
-                } else if (c >= TB && j >= TP) {
-                    const int col = hB * TB + c - TB, jj = j - TP;
-                    v = fB ? wpre[q] : (jj == col ? 1.0 : 0.0);
-                }
-                V[j][c] = v;
+                const int e = tid + NT * q, h = e / (TP * TB), j = (e % (TP * TB)) / TB, c = e % TB;
+                const int col = (h ? hB : hA) * TB + c;
+                Vb[h][j][c] = (h ? fB : fA) ? wpre[q] : (j == col ? 1.0 : 0.0);
             }
             __syncthreads();
             // Y = V^T G_old (TP x 2TP): 2 x 4 tiles of 16, two per wave; the
@@ -682,10 +717,8 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
                 const int q = wv + 4 * h, tr = q >> 2, tc = q & 3;
                 v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4) {
-                    const int k = tr * TP + k0 + lk;
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(V[k][16 * tr + li], gpre[h][k0 / 4], acc, 0, 0, 0);
-                }
+                for (int k0 = 0; k0 < TP; k0 += 4)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vb[tr][k0 + lk][li], gpre[h][k0 / 4], acc, 0, 0, 0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) Y[16 * tr + lk + 4 * r][16 * tc + li] = acc[r];
             }
@@ -696,13 +729,13 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
                 const int ti = wv >> 1, tj = wv & 1;
                 v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4) {
-                    const int k = tj * TP + k0 + lk;
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Y[16 * ti + li][k], V[k][16 * tj + li], acc, 0, 0, 0);
-                }
+                for (int k0 = 0; k0 < TP; k0 += 4)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Y[16 * ti + li][tj * TP + k0 + lk], Vb[tj][k0 + lk][li],
+                                                                acc, 0, 0, 0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) S[16 * ti + lk + 4 * r][16 * tj + li] = acc[r];
             }
+            __syncthreads();   // every wave is done with Vb before W takes its place
         }
         const int ps = (int)(step & 1);
         // the first step of a pass meets every block and treats the planes
@@ -720,10 +753,12 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
     double(*Wa)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem);
     double(*Wb)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + kS);
     double(*Tt)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + 2 * kS);
-    double(*X)[kTCol + 1] = reinterpret_cast<double(*)[kTCol + 1]>(smem + 3 * kS);
+    double(*Xg)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + 3 * kS);     // G tile
+    double(*X)[kTCol + 1] = reinterpret_cast<double(*)[kTCol + 1]>(smem + kS);    // rt slab
     int *ra = rows, *rb = rb_;
-    const int64_t ub = bid - solve_total;
-    const syn_gbj_item &P = items[tj_find<false>(items, n, ub)];
+    const int64_t ub = tj_xcd_order(bid - solve_pad, tile_pad);
+    if (ub >= tile_count) return;
+    const syn_gbj_item &P = items[owner_t[ub]];
     const TjDims D = tj_dims<TB>(P.nv);
     const int64_t su = step - 1, st = su % (D.np - 1);
     const int pu = (int)(su & 1);
@@ -758,10 +793,10 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
 #pragma unroll
         for (int q = 0; q < QX; ++q) {
             const int e = tid + NT * q, i = e / TP, k = e % TP;
-            X[i][k] = (ra[i] >= 0 && rb[k] >= 0) ? xv[q] : 0.0;
+            Xg[i][k] = (ra[i] >= 0 && rb[k] >= 0) ? xv[q] : 0.0;
         }
         if (fa || fb) {
-            if (tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);
+            if (bytes && tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);
 #pragma unroll
             for (int q = 0; q < QX; ++q) {
                 const int e = tid + NT * q, i = e / TP, k = e % TP;
@@ -774,7 +809,7 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
                 v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int k0 = 0; k0 < TP; k0 += 4)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li],
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], Xg[k0 + lk][tj * 16 + li],
                                                                 acc, 0, 0, 0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) Tt[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
@@ -788,17 +823,17 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
                     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tt[ti * 16 + li][k0 + lk], Wb[k0 + lk][tj * 16 + li],
                                                                 acc, 0, 0, 0);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) X[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
+                for (int r = 0; r < 4; ++r) Xg[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
             }
         }
         __syncthreads();
         for (int e = tid; e < TP * TP; e += NT) {
             const int i = e / TP, k = e % TP;
             const int gr = ra[i], gc = rb[k];
-            if (gr >= 0 && gc >= 0) Gn[(int64_t)gr * ldn + gc] = X[i][k];
+            if (gr >= 0 && gc >= 0) Gn[(int64_t)gr * ldn + gc] = Xg[i][k];
             if (a != b) {
                 const int tr = rb[i], tc = ra[k];
-                if (tr >= 0 && tc >= 0) Gn[(int64_t)tr * ldn + tc] = X[k][i];
+                if (tr >= 0 && tc >= 0) Gn[(int64_t)tr * ldn + tc] = Xg[k][i];
             }
         }
         return;
@@ -807,7 +842,7 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
     const int64_t u = t - D.half * D.half;
     const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
     if (!flags[a]) return;
-    if (tid == 0) atomicAdd(bytes, kSlabBytes);
+    if (bytes && tid == 0) atomicAdd(bytes, kSlabBytes);
     if (tid < TP) ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
     __syncthreads();
     // W and the rt slab are loaded before the first value is used (padding
@@ -993,10 +1028,15 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     unsigned long long *dbytes = (unsigned long long *)(work + tot_w + tot_nv + 2 * (int64_t)n_items);
     hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, (int32_t *)dbytes, 2);
     SYN_LAUNCH_CHECK("gram byte counter");
+    // the update launches count their bytes only when a profile asks for them
+    // (one atomic per work-group on a single address)
+    unsigned long long *kbytes = prof_wants(kTagGramUpdate) ? dbytes : nullptr;
 
     // end-of-sweep test flags, in the workspace's spare tail after the byte
     // counter (at least tot_nv + 15 doubles are left there)
     int32_t *chk = (int32_t *)(dbytes + 1);
+    // owner table of the fused steps, after n_items doubles for the flags
+    int32_t *owner = (int32_t *)(dbytes + 1 + n_items);
     std::vector<int32_t> hc(n_items), hk(n_items);
     // fused steps (16-row blocks; 32-row blocks run solves and updates in
     // launches of their own): buffer holding each problem's Gram matrix
@@ -1023,14 +1063,19 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
         }
         if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, counters, n_items);
+        if (fused) {
+            hipLaunchKernelGGL(tj_owner_kernel<TB>, dim3((unsigned)act.size()), dim3(256), 0, st, items, ss, owner);
+            SYN_LAUNCH_CHECK("gram owner table");
+        }
         for (int64_t step = 0; fused && step < max_np; ++step) {
             // solves of step (all but the last launch) beside the updates of
             // step - 1 (all but the first)
             const bool hs = step < max_np - 1, hu = step > 0;
-            const int64_t grid = (hs ? ss : 0) + (hu ? ts : 0);
+            const int64_t sp = hs ? (ss + 7) / 8 * 8 : 0, tp = hu ? (ts + 7) / 8 * 8 : 0;
             ProfScope ps(kTagGramSolve, 0.0, st);
-            hipLaunchKernelGGL(tjf_step_kernel<16>, dim3((unsigned)grid), dim3(256), 0, st, items, (int)act.size(),
-                               step, hu ? 1 : 0, hs ? ss : (int64_t)0, cur, tol, g_tj_inner, dbytes);
+            hipLaunchKernelGGL(tjf_step_kernel<16>, dim3((unsigned)(sp + tp)), dim3(256), 0, st, items,
+                               (int)act.size(), owner, owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp,
+                               ts, cur, tol, g_tj_inner, kbytes);
             if (hu) cur ^= 1;
         }
         if (fused)
@@ -1044,7 +1089,7 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
             {
                 ProfScope ps(kTagGramUpdate, 0.0, st);
                 hipLaunchKernelGGL(tj_update_kernel<TB>, dim3((unsigned)ts), dim3(kTUpdThreads), 0, st, items,
-                                   (int)act.size(), step, dbytes);
+                                   (int)act.size(), step, kbytes);
             }
         }
         SYN_LAUNCH_CHECK("gram step");
@@ -1135,8 +1180,9 @@ extern "C" void syn_tune_gram_block_rows(int rows) { syn::g_tj_block = (rows == 
 
 extern "C" int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items) {
     int64_t tot = 0;
-    for (int i = 0; i < n_items; ++i) tot += syn::tj_work_max(nv[i]) + 2 * nv[i];
-    return tot + 2 * (int64_t)n_items + 16;
+    for (int i = 0; i < n_items; ++i)
+        tot += syn::tj_work_max(nv[i]) + 2 * nv[i] + (syn::tj_owner_count<16>(nv[i]) + 1) / 2;
+    return tot + 3 * (int64_t)n_items + 16;
 }
 
 extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_host, int n_items,

@@ -220,6 +220,11 @@ class Emulator:
                 self.mat(r["q"], m, kq, int(r["ldq"]), 0, "qr q")[...] = Q
         return 0
 
+    def syn_qr_sized_f64(self, items, n, lds_doubles, stream):
+        if not 0 <= lds_doubles <= 12288:
+            raise BoundsError("qr LDS size beyond 96 KB")
+        return self.syn_qr_f64(items, n, stream)
+
     def _jacobi_one(self, x, ldx, nv, ln, rt, ldr, s, gram):
         X = self.mat(x, nv, ln, ldx, 0, "jacobi x")
         R = self.mat(rt, nv, nv, ldr, 0, "jacobi rt")
@@ -493,8 +498,11 @@ def _gram_workspace(nv):
             nblk = (int(v) + tb - 1) // tb
             half = (nblk + (nblk & 1)) // 2
             work.append(half * 4 * tb * tb + (half + 1) // 2 + 1 + int(v) * int(v) + int(v) + 4)
-        tot += max(work) + 2 * int(v)
-    return tot + 2 * len(nv) + 16
+        nblk = (int(v) + 15) // 16
+        half = (nblk + (nblk & 1)) // 2
+        owners = half + half * half + half * ((int(v) + 63) // 64)
+        tot += max(work) + 2 * int(v) + (owners + 1) // 2
+    return tot + 3 * len(nv) + 16
 
 
 class _FakeLib:

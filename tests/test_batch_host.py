@@ -42,7 +42,7 @@ def test_emulated_batched_round_matches_oracle(emu):
     recs = []
     Z = B.apply_and_round(X, O, np.array([j[2] for j in jobs]), 0.0, records=recs)
     out = Z.to_numpy()
-    for fn in ("syn_jacobi_gram_blocked_f64", "syn_qr_f64", "syn_eye_f64", "syn_chol_leaf_f64", "syn_jacobi_gram_f64"):
+    for fn in ("syn_jacobi_gram_blocked_f64", "syn_qr_sized_f64", "syn_eye_f64", "syn_chol_leaf_f64", "syn_jacobi_gram_f64"):
         assert fn in emu.calls
     for q, (xs, os_, c) in enumerate(jobs):
         ref, sv, kept, disc = ornd.apply_and_round(os_, xs, chi_max=c)
