@@ -345,11 +345,6 @@ typedef struct {
 int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_host, int n_items, double *work,
                                 int32_t *counters, double tol, int max_sweeps, int *sweeps, void *stream);
 int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items);
-/* inner Jacobi sweeps per pair solve (1..8, default 1) */
-void syn_tune_gram_inner_sweeps(int sweeps);
-/* rows per block (16 or 32; 0 = automatic, currently 16: 32-row blocks halve */
-/* the steps but double the rounds of each pair solve, and measured no faster) */
-void syn_tune_gram_block_rows(int rows);
 
 /* Batched copy: dst[0..n) = src[0..n) for every item (compaction). */
 typedef struct {

@@ -121,8 +121,6 @@ SIGNATURES = {
                                      ctypes.POINTER(ctypes.c_int), _I32, _VP]),
     "syn_jacobi_gram_blocked_f64": (_I32, [_VP, _VP, _I32, _VP, _VP, _D, _I32, ctypes.POINTER(ctypes.c_int), _VP]),
     "syn_jacobi_gram_blocked_workspace": (_I64, [_VP, _I32]),
-    "syn_tune_gram_inner_sweeps": (None, [_I32]),
-    "syn_tune_gram_block_rows": (None, [_I32]),
     "syn_chol_leaf_f64": (_I32, [_VP, _I32, _VP]),
     "syn_prof_begin": (_I32, []),
     "syn_prof_select": (_I32, [ctypes.c_uint32]),

@@ -158,18 +158,12 @@ __host__ __device__ __forceinline__ TjDims tj_dims(int64_t nv) {
     return d;
 }
 
-template <bool SOLVE>
-__device__ __forceinline__ int tj_find(const syn_gbj_item *items, int n, int64_t b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        const int64_t st = SOLVE ? items[mid].solve_start : items[mid].tile_start;
-        if (st <= b)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
+// Update work of a problem at a step: the upper-triangle G tiles (a, b),
+// a <= b, two per work-group (each by two waves, one per 16-column half),
+// then one work-group per rt slab of kTCol columns (one wave per 16 columns).
+__host__ __device__ __forceinline__ int64_t tj_gtiles(int64_t half) { return half * (half + 1) / 2; }
+__host__ __device__ __forceinline__ int64_t tj_ugroups(const TjDims &D) {
+    return (tj_gtiles(D.half) + 1) / 2 + D.half * D.ncs;
 }
 
 // Work-group owner table of a sweep's fused steps: entry b < ss names the
@@ -182,7 +176,7 @@ __global__ __launch_bounds__(256) void tj_owner_kernel(const syn_gbj_item *__res
     const syn_gbj_item &P = items[i];
     const TjDims D = tj_dims<TB>(P.nv);
     for (int64_t b = threadIdx.x; b < D.half; b += 256) owner[P.solve_start + b] = i;
-    const int64_t nt = D.half * D.half + D.half * D.ncs;
+    const int64_t nt = tj_ugroups(D);
     for (int64_t t = threadIdx.x; t < nt; t += 256) owner[ss + P.tile_start + t] = i;
 }
 
@@ -190,7 +184,7 @@ __global__ __launch_bounds__(256) void tj_owner_kernel(const syn_gbj_item *__res
 template <int TB>
 static int64_t tj_owner_count(int64_t nv) {
     const TjDims D = tj_dims<TB>(nv);
-    return D.half + D.half * D.half + D.half * D.ncs;
+    return D.half + tj_ugroups(D);
 }
 
 // Per-problem scratch layout (doubles): two sets of W tiles (half * TP^2
@@ -417,141 +411,6 @@ __device__ __forceinline__ void tj_pair_solve(const syn_gbj_item &P, const TjDim
 }
 
 
-template <int TB>
-__global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *__restrict__ items, int n,
-                                                           int64_t step, double tol, int inner_sweeps) {
-    constexpr int TP = 2 * TB, NT = TB * TB;
-    __shared__ double S[TP][TP + 1];
-    __shared__ double W[TP][TP + 1];
-    __shared__ v2d cs[TP / 2];   // (c, s) of each plane of the round
-    __shared__ int rows[TP];
-    __shared__ int do_rot, any_rot;
-    const int64_t bid = blockIdx.x;
-    const syn_gbj_item &P = items[tj_find<true>(items, n, bid)];
-    const TjDims D = tj_dims<TB>(P.nv);
-    const int64_t pw = bid - P.solve_start;
-    const int64_t st = step % (D.np - 1);
-    const int tid = threadIdx.x;
-    if (tid < TP) rows[tid] = tj_row<TB>(st, pw, D.np, D.nblk, P.nv, tid);
-    __syncthreads();
-    for (int e = tid; e < TP * TP; e += NT) {
-        const int i = e / TP, k = e % TP, ri = rows[i], rk = rows[k];
-        S[i][k] = (ri >= 0 && rk >= 0) ? P.g[(int64_t)ri * P.ldg + rk] : 0.0;
-    }
-    tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, inner_sweeps, tj_wset<TB>(P, D.half, 0) + pw * TP * TP,
-                      tj_flags<TB>(P, D.half, 0));
-}
-
-// MFMA f64 16x16x4 layout: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
-// D reg r of lane l = D[l/16 + 4r][l%16].
-template <int TB>
-__global__ __launch_bounds__(kTUpdThreads) void tj_update_kernel(const syn_gbj_item *__restrict__ items, int n,
-                                                                  int64_t step, unsigned long long *bytes) {
-    constexpr int TP = 2 * TB, NS = TP / 16;            // 16 x 16 sub-tiles per edge
-    constexpr unsigned long long kTileBytes = (unsigned long long)TP * TP * sizeof(double);
-    constexpr unsigned long long kSlabBytes = 2ull * TP * kTCol * sizeof(double);
-    __shared__ double Wa[TP][TP + 1];
-    __shared__ double Wb[TP][TP + 1];
-    __shared__ double X[TP][kTCol + 1];   // G tile (TP columns used) or rt slab
-    __shared__ double Tt[TP][TP + 1];
-    __shared__ int ra[TP], rb[TP];
-    static_assert(TP <= kTCol, "a G tile must fit the slab buffer");
-    const int64_t bid = blockIdx.x;
-    const syn_gbj_item &P = items[tj_find<false>(items, n, bid)];
-    const TjDims D = tj_dims<TB>(P.nv);
-    const int64_t st = step % (D.np - 1);
-    const int64_t t = bid - P.tile_start;
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
-    const int *flags = tj_flags<TB>(P, D.half);
-    if (t < D.half * D.half) {
-        // ---- G tile (a, b) -------------------------------------------------
-        // G stays symmetric: the work-group of (a, b), a <= b, also writes
-        // (b, a) as the transpose; those of a > b have nothing to do
-        const int64_t a = t / D.half, b = t % D.half;
-        if (a > b) return;
-        const int fa = flags[a], fb = flags[b];
-        if (!fa && !fb) return;
-        if (bytes && tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);  // read one tile, write one or two
-        if (tid < TP) {
-            ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
-            rb[tid] = tj_row<TB>(st, b, D.np, D.nblk, P.nv, tid);
-        }
-        __syncthreads();
-        const double *wa = P.w + a * TP * TP, *wb = P.w + b * TP * TP;
-        for (int e = tid; e < TP * TP; e += kTUpdThreads) {
-            const int i = e / TP, k = e % TP;
-            Wa[i][k] = fa ? wa[e] : (i == k ? 1.0 : 0.0);
-            Wb[i][k] = fb ? wb[e] : (i == k ? 1.0 : 0.0);
-            const int r = ra[i], c = rb[k];
-            X[i][k] = (r >= 0 && c >= 0) ? P.g[(int64_t)r * P.ldg + c] : 0.0;
-        }
-        __syncthreads();
-        // T = Wa^T X, sub-tiles dealt to the waves
-        for (int q = w; q < NS * NS; q += kTUpdThreads / 64) {
-            const int ti = q / NS, tj = q % NS;
-            v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int k0 = 0; k0 < TP; k0 += 4)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc,
-                                                            0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Tt[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
-        }
-        __syncthreads();
-        // G_ab = T Wb, into X (the input tile is no longer needed)
-        for (int q = w; q < NS * NS; q += kTUpdThreads / 64) {
-            const int ti = q / NS, tj = q % NS;
-            v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int k0 = 0; k0 < TP; k0 += 4)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tt[ti * 16 + li][k0 + lk], Wb[k0 + lk][tj * 16 + li], acc,
-                                                            0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) X[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
-        }
-        __syncthreads();
-        for (int e = tid; e < TP * TP; e += kTUpdThreads) {
-            const int i = e / TP, k = e % TP;
-            const int gr = ra[i], gc = rb[k];
-            if (gr >= 0 && gc >= 0) P.g[(int64_t)gr * P.ldg + gc] = X[i][k];
-            if (a != b) {
-                const int tr = rb[i], tc = ra[k];
-                if (tr >= 0 && tc >= 0) P.g[(int64_t)tr * P.ldg + tc] = X[k][i];
-            }
-        }
-        return;
-    }
-    // ---- rt slab (a, columns [c0, c0 + kTCol)) ------------------------------
-    const int64_t u = t - D.half * D.half;
-    const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
-    if (!flags[a]) return;
-    if (bytes && tid == 0) atomicAdd(bytes, kSlabBytes);
-    if (tid < TP) ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
-    __syncthreads();
-    const double *wa = P.w + a * TP * TP;
-    for (int e = tid; e < TP * TP; e += kTUpdThreads) Wa[e / TP][e % TP] = wa[e];
-    for (int e = tid; e < TP * kTCol; e += kTUpdThreads) {
-        const int i = e / kTCol, j = e % kTCol, r = ra[i];
-        X[i][j] = (r >= 0 && c0 + j < P.nv) ? P.rt[(int64_t)r * P.ldr + c0 + j] : 0.0;
-    }
-    __syncthreads();
-    // rt_a = Wa^T X: NS x 4 sub-tiles of 16 x 16 dealt to the waves
-    for (int q = w; q < NS * (kTCol / 16); q += kTUpdThreads / 64) {
-        const int ti = q / (kTCol / 16), tj = q % (kTCol / 16);
-        v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int k0 = 0; k0 < TP; k0 += 4)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc, 0, 0,
-                                                        0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = ti * 16 + lk + 4 * r, j = tj * 16 + li;
-            const int gr = ra[i];
-            if (gr >= 0 && c0 + j < P.nv) P.rt[(int64_t)gr * P.ldr + c0 + j] = acc[r];
-        }
-    }
-}
-
 // Pair of step `step` - 1 holding block x, and the half (0: first 16 rows,
 // 1: second) it sits in: the inverse of tj_pair's round-robin map.
 // 32-bit arithmetic as in tj_pair (x, sp < np, np far below 2^31).
@@ -576,44 +435,49 @@ __device__ __forceinline__ void tj_owner(int64_t x, int64_t sp, int64_t np, int6
 }
 
 // One fused step: the tile updates of step - 1 (has_update) and the pair
-// solves of step (has_solve) in one launch, so the updates' work-groups run
-// beside the latency-bound solves.  The Gram matrix is read from buffer `cur`
-// (0: P.g, 1: the workspace copy) and the updates write the other buffer,
-// every tile (unrotated ones are copied), so no buffer is read and written in
-// the same launch.  A solve builds its tile of G_step = W^T G_(step-1) W from
-// buffer `cur` and the previous step's rotations: for its blocks a and b,
-// with V = diag(W_A(:, half of a), W_B(:, half of b)) over the rows of their
-// pairs A and B at step - 1, S = V^T G_old V (two small MFMA products).
-// Solves come first in the grid (they are the long ones).  W tiles and
-// flags alternate between two sets by step parity.
+// solves of step in one launch, so the updates' work-groups run beside the
+// latency-bound solves.  The Gram matrix is read from buffer `cur` (0: P.g,
+// 1: the workspace copy) and the updates write the other buffer, every tile
+// (unrotated ones are copied), so no buffer is read and written in the same
+// launch.  A solve builds its tile of G_step = W^T G_(step-1) W from buffer
+// `cur` and the previous step's rotations: for its blocks a and b, with
+// V = diag(V_A, V_B), V_A = W_A(:, half of a) over the rows of a's pair A at
+// step - 1 (and V_B likewise), S = V^T G_old V.  Solves come first in the
+// grid (they are the long ones).  W tiles and flags alternate between two
+// sets by step parity.
+//
 // Work-groups are dealt to the eight XCDs round-robin by launch index; the
-// two sections of the grid (solves, then tiles) are each padded to a multiple
-// of 8 and read in XCD-major order, so XCD x takes a contiguous run of each
-// section: the solves and tiles of neighbouring problems, whose W tiles and
-// Gram rows then meet in that XCD's L2 instead of being fetched by all eight.
+// two sections of the grid (solves, then updates) are each padded to a
+// multiple of 8 and read in XCD-major order, so XCD x takes a contiguous run
+// of each section: the solves and tiles of neighbouring problems, whose W
+// tiles and Gram rows then meet in that XCD's L2 instead of being fetched by
+// all eight.
 __device__ __forceinline__ int64_t tj_xcd_order(int64_t b, int64_t padded) {
     return (b & 7) * (padded >> 3) + (b >> 3);
 }
 
+// The register products below use the f64 MFMA 16x16x4 layout: lane
+// l = 16 lk + li supplies A[li][k] and B[k][li] of each k-step and holds
+// D[lk + 4r][li] in register r.  A contraction may visit its index in any
+// order as long as both operands follow it: k-step j of lane lk visits
+// index 8 lk + j where the operand comes from a Gram row (eight consecutive
+// doubles of one block per lane), and 4 j + lk where it comes from an MFMA
+// result held in registers (whose rows are lk + 4r).
 template <int TB>
-__global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *__restrict__ items, int n,
+__global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item *__restrict__ items,
                                                           const int32_t *__restrict__ owner_s,
                                                           const int32_t *__restrict__ owner_t, int64_t step,
                                                           int has_update, int64_t solve_pad, int64_t solve_count,
                                                           int64_t tile_pad, int64_t tile_count, int cur, double tol,
-                                                          int inner_sweeps, unsigned long long *bytes) {
+                                                          unsigned long long *bytes) {
     constexpr int TP = 2 * TB, NT = TB * TB;
     static_assert(TB == 16 && NT == kTUpdThreads, "fused steps: 16-row blocks, one block size for both roles");
-    // LDS: the solve role holds S, the two nonzero TP x TB blocks of V
-    // (which W reuses once the tile is built) and Y; a G-tile update holds
-    // Wa, Wb, T and its TP x TP tile, an rt-slab update Wa and the slab.
-    // 4224 doubles (33 KiB) keep four work-groups on a CU.
-    constexpr int kS = TP * (TP + 1), kV = 2 * TP * (TB + 1), kY = TP * (2 * TP + 1), kX = TP * (kTCol + 1);
-    static_assert(kV >= kS, "W reuses the V blocks");
-    constexpr int kSolve = kS + kV + kY, kUpd = (4 * kS > kS + kX) ? 4 * kS : kS + kX;
-    __shared__ double smem[kSolve > kUpd ? kSolve : kUpd];
+    // LDS: the solve role's tile S and rotation W (17 KiB); the update role
+    // works in registers
+    __shared__ double S[TP][TP + 1];
+    __shared__ double W[TP][TP + 1];
     __shared__ v2d cs[TP / 2];
-    __shared__ int rows[TP], rb_[TP];
+    __shared__ int rows[TP];
     __shared__ int do_rot, any_rot;
     const int64_t bid = blockIdx.x;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
@@ -627,12 +491,6 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
     };
     if (bid < solve_pad) {
         // ---- pair solve of `step` ------------------------------------------
-        double(*S)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem);
-        // Vb[h] (TP x TB): the nonzero block h of V; W takes their place
-        // after the tile build
-        double(*Vb)[TP][TB + 1] = reinterpret_cast<double(*)[TP][TB + 1]>(smem + kS);
-        double(*W)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + kS);
-        double(*Y)[2 * TP + 1] = reinterpret_cast<double(*)[2 * TP + 1]>(smem + kS + kV);
         const int64_t sb = tj_xcd_order(bid, solve_pad);
         if (sb >= solve_count) return;
         const syn_gbj_item &P = items[owner_s[sb]];
@@ -649,113 +507,76 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
                 S[i][k] = (ri >= 0 && rk >= 0) ? Gc[(int64_t)ri * ldc + rk] : 0.0;
             }
         } else {
+            // wave (si, sj) builds tile (si, sj) of S: with the old pair wr
+            // of new block si and wc of new block sj,
+            //   Z = G_old(rows of wr, rows of wc) V_sj   (32 x 16)
+            //   S(si, sj) = V_si^T Z                     (16 x 16)
+            // so the four waves read the four quadrants of G_old once.
             const int64_t sp = (step - 1) % (D.np - 1);
+            const int si = wv >> 1, sj = wv & 1;
             int64_t ba, bb, wA, wB;
             int hA, hB;
             tj_pair(st, pw, D.np, ba, bb);
             tj_owner(ba, sp, D.np, D.half, wA, hA);
             tj_owner(bb, sp, D.np, D.half, wB, hB);
-            // the old rows are computed by every thread for its own loads (no
-            // barrier before the first global reads)
-            int64_t oa0, ob0, oa1, ob1;   // blocks of the old pairs wA and wB
-            tj_pair(sp, wA, D.np, oa0, ob0);
-            tj_pair(sp, wB, D.np, oa1, ob1);
-            // G_old operands of this wave's two Y tiles (tile row h: the rows of
-            // old pair h, wA or wB; tile column wv: entries 16 wv .. 16 wv + 15
-            // of the two old pairs' 2 TP), loaded first so their latency
-            // overlaps the V build below; padding loads row or column 0 and is
-            // selected to zero afterwards
-            const int rc = tj_row<TB>(sp, wv >= 2 ? wB : wA, D.np, D.nblk, P.nv, 16 * (wv & 1) + li);
-            int rkv[2][TP / 4];
-            double gpre[2][TP / 4];
+            const int64_t wr = si ? wB : wA, wc = sj ? wB : wA;
+            const int hr = si ? hB : hA, hc = sj ? hB : hA;
+            const int pp = (int)((step - 1) & 1);
+            const int *fl = tj_flags<TB>(P, D.half, pp);
+            const int fr = fl[wr], fc = fl[wc];
+            const double *Wr = tj_wset<TB>(P, D.half, pp) + wr * TP * TP;
+            const double *Wc = tj_wset<TB>(P, D.half, pp) + wc * TP * TP;
+            // G_old rows 16 rho + li of pair wr, columns 8 lk .. 8 lk + 7 of
+            // pair wc (one block, consecutive); padding reads as zero
+            const int cb = tj_row<TB>(sp, wc, D.np, D.nblk, P.nv, 8 * lk);
+            double g[2][8], v[8];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int rho = 0; rho < 2; ++rho) {
+                const int r = tj_row<TB>(sp, wr, D.np, D.nblk, P.nv, 16 * rho + li);
 #pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4) {
-                    const int64_t blk = k0 < TB ? (h ? oa1 : oa0) : (h ? ob1 : ob0);
-                    const int64_t r = blk * TB + ((k0 + lk) % TB);
-                    const int rk = (blk < D.nblk && r < P.nv) ? (int)r : -1;
-                    rkv[h][k0 / 4] = rk;
-                    gpre[h][k0 / 4] = Gc[(int64_t)max(rk, 0) * ldc + max(rc, 0)];
+                for (int j = 0; j < 8; ++j) {
+                    const bool ok = r >= 0 && cb >= 0 && cb + j < P.nv;
+                    const double val = Gc[ok ? (int64_t)r * ldc + cb + j : 0];
+                    g[rho][j] = ok ? val : 0.0;
                 }
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4)
-                    if (rkv[h][k0 / 4] < 0 || rc < 0) gpre[h][k0 / 4] = 0.0;
-            const int pp = (int)((step - 1) & 1);
-            const int *fl = tj_flags<TB>(P, D.half, pp);
-            const int fA = fl[wA], fB = fl[wB];
-            const double *WA = tj_wset<TB>(P, D.half, pp) + wA * TP * TP;
-            const double *WB = tj_wset<TB>(P, D.half, pp) + wB * TP * TP;
-            // V (2TP x TP): columns c < TB are column hA TB + c of W_A on rows
-            // 0 .. TP-1, columns c >= TB column hB TB + c - TB of W_B on rows
-            // TP .. 2TP-1 (identity columns for a pair that did not rotate),
-            // zero elsewhere; only the two nonzero blocks are stored:
-            // Vb[0][j][c] = V[j][c], Vb[1][j][c] = V[TP + j][TB + c].
-            // Every W load of a thread is issued before the first is used.
-            constexpr int QV = 2 * TP * TB / NT;
-            double wpre[QV];
-#pragma unroll
-            for (int q = 0; q < QV; ++q) {
-                const int e = tid + NT * q, h = e / (TP * TB), j = (e % (TP * TB)) / TB, c = e % TB;
-                wpre[q] = h ? WB[j * TP + hB * TB + c] : WA[j * TP + hA * TB + c];
+            for (int j = 0; j < 8; ++j) {
+                const int e = 8 * lk + j;
+                v[j] = fc ? Wc[e * TP + hc * TB + li] : (e == hc * TB + li ? 1.0 : 0.0);
             }
+            v4d z[2];
 #pragma unroll
-            for (int q = 0; q < QV; ++q) {
-                const int e = tid + NT * q, h = e / (TP * TB), j = (e % (TP * TB)) / TB, c = e % TB;
-                const int col = (h ? hB : hA) * TB + c;
-                Vb[h][j][c] = (h ? fB : fA) ? wpre[q] : (j == col ? 1.0 : 0.0);
+            for (int rho = 0; rho < 2; ++rho) {
+                z[rho] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) z[rho] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[rho][j], v[j], z[rho], 0, 0, 0);
             }
-            __syncthreads();
-            // Y = V^T G_old (TP x 2TP): 2 x 4 tiles of 16, two per wave; the
-            // rows of tile row tr come from V's column block tr, nonzero on
-            // rows tr TP .. tr TP + TP - 1 only
+            v4d sacc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int q = wv + 4 * h, tr = q >> 2, tc = q & 3;
-                v4d acc = {0.0, 0.0, 0.0, 0.0};
+            for (int rho = 0; rho < 2; ++rho)
 #pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vb[tr][k0 + lk][li], gpre[h][k0 / 4], acc, 0, 0, 0);
+                for (int r = 0; r < 4; ++r) {
+                    const int e = 16 * rho + 4 * r + lk;
+                    const double a = fr ? Wr[e * TP + hr * TB + li] : (e == hr * TB + li ? 1.0 : 0.0);
+                    sacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, z[rho][r], sacc, 0, 0, 0);
+                }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) Y[16 * tr + lk + 4 * r][16 * tc + li] = acc[r];
-            }
-            __syncthreads();
-            // S = Y V (TP x TP): one 16 x 16 tile per wave; column block tj of
-            // V is nonzero on rows tj TP .. tj TP + TP - 1 only
-            {
-                const int ti = wv >> 1, tj = wv & 1;
-                v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Y[16 * ti + li][tj * TP + k0 + lk], Vb[tj][k0 + lk][li],
-                                                                acc, 0, 0, 0);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) S[16 * ti + lk + 4 * r][16 * tj + li] = acc[r];
-            }
-            __syncthreads();   // every wave is done with Vb before W takes its place
+            for (int r = 0; r < 4; ++r) S[TB * si + lk + 4 * r][TB * sj + li] = sacc[r];
         }
         const int ps = (int)(step & 1);
         // the first step of a pass meets every block and treats the planes
         // within blocks; the others only the planes between their two blocks,
         // so a pass treats every plane of G once
-        tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, inner_sweeps,
-                          tj_wset<TB>(P, D.half, ps) + pw * TP * TP, tj_flags<TB>(P, D.half, ps),
-                          st != 0);
+        tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, 1, tj_wset<TB>(P, D.half, ps) + pw * TP * TP,
+                          tj_flags<TB>(P, D.half, ps), st != 0);
         return;
     }
     if (!has_update) return;
-    // ---- tile update of step - 1: buffer cur -> buffer cur ^ 1 ----------------
+    // ---- updates of step - 1: buffer cur -> buffer cur ^ 1, in registers ----
+    // (no barrier from here on: waves return independently)
     constexpr unsigned long long kTileBytes = (unsigned long long)TP * TP * sizeof(double);
     constexpr unsigned long long kSlabBytes = 2ull * TP * kTCol * sizeof(double);
-    double(*Wa)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem);
-    double(*Wb)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + kS);
-    double(*Tt)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + 2 * kS);
-    double(*Xg)[TP + 1] = reinterpret_cast<double(*)[TP + 1]>(smem + 3 * kS);     // G tile
-    double(*X)[kTCol + 1] = reinterpret_cast<double(*)[kTCol + 1]>(smem + kS);    // rt slab
-    int *ra = rows, *rb = rb_;
     const int64_t ub = tj_xcd_order(bid - solve_pad, tile_pad);
     if (ub >= tile_count) return;
     const syn_gbj_item &P = items[owner_t[ub]];
@@ -763,123 +584,120 @@ __global__ __launch_bounds__(TB * TB) void tjf_step_kernel(const syn_gbj_item *_
     const int64_t su = step - 1, st = su % (D.np - 1);
     const int pu = (int)(su & 1);
     const int64_t t = ub - P.tile_start;
+    const int64_t ng = tj_gtiles(D.half), gw = (ng + 1) / 2;
     const int *flags = tj_flags<TB>(P, D.half, pu);
     const double *wset = tj_wset<TB>(P, D.half, pu);
-    int64_t ldc, ldn;
-    const double *Gc = gbuf(P, cur, D.half, ldc);
-    double *Gn = gbuf(P, cur ^ 1, D.half, ldn);
-    if (t < D.half * D.half) {
-        // G tile (a, b), a <= b, and its transpose (b, a)
-        const int64_t a = t / D.half, b = t % D.half;
-        if (a > b) return;
+    if (t < gw) {
+        // G tile (a, b), a <= b, and its transpose (b, a): waves 2h, 2h + 1
+        // take tile 2t + h, wave parity the 16 output columns tj
+        const int64_t gt = 2 * t + (wv >> 1);
+        if (gt >= ng) return;
+        int64_t a = 0, rem = gt;
+        while (rem >= D.half - a) {
+            rem -= D.half - a;
+            ++a;
+        }
+        const int64_t b = a + rem;
+        const int tj = wv & 1;
         const int fa = flags[a], fb = flags[b];
-        if (tid < TP) {
-            ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
-            rb[tid] = tj_row<TB>(st, b, D.np, D.nblk, P.nv, tid);
+        int64_t ldc, ldn;
+        const double *Gc = gbuf(P, cur, D.half, ldc);
+        double *Gn = gbuf(P, cur ^ 1, D.half, ldn);
+        // X rows 16 ti + li (rows of pair a), columns 8 lk .. 8 lk + 7 (pair b)
+        const int cb = tj_row<TB>(st, b, D.np, D.nblk, P.nv, 8 * lk);
+        int ra2[2];
+        double x[2][8];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) {
+            ra2[ti] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, TB * ti + li);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const bool ok = ra2[ti] >= 0 && cb >= 0 && cb + j < P.nv;
+                const double val = Gc[ok ? (int64_t)ra2[ti] * ldc + cb + j : 0];
+                x[ti][j] = ok ? val : 0.0;
+            }
         }
-        __syncthreads();
-        // the G tile and the two W tiles are loaded before the first value
-        // is used (padding rows from row 0, selected to zero afterwards)
-        constexpr int QX = TP * TP / NT;
+        if (!fa && !fb) {
+            // nothing rotated: copy this wave's columns (lanes lk = 2 tj, 2 tj + 1)
+            if ((lk >> 1) == tj && cb >= 0) {
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti) {
+                    if (ra2[ti] < 0) continue;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if (cb + j >= P.nv) break;
+                        Gn[(int64_t)ra2[ti] * ldn + cb + j] = x[ti][j];
+                        if (a != b) Gn[(int64_t)(cb + j) * ldn + ra2[ti]] = x[ti][j];
+                    }
+                }
+            }
+            return;
+        }
+        if (bytes && tj == 0 && lane == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);
         const double *wa = wset + a * TP * TP, *wb = wset + b * TP * TP;
-        double xv[QX], wva[QX], wvb[QX];
+        // U = X Wb(:, tj): two 16 x 16 tiles (rows 0-15, 16-31)
+        double vb[8];
 #pragma unroll
-        for (int q = 0; q < QX; ++q) {
-            const int e = tid + NT * q, i = e / TP, k = e % TP;
-            xv[q] = Gc[(int64_t)max(ra[i], 0) * ldc + max(rb[k], 0)];
-            wva[q] = wa[e];
-            wvb[q] = wb[e];
+        for (int j = 0; j < 8; ++j) {
+            const int e = 8 * lk + j, c = TB * tj + li;
+            vb[j] = fb ? wb[e * TP + c] : (e == c ? 1.0 : 0.0);
         }
+        v4d u[2];
 #pragma unroll
-        for (int q = 0; q < QX; ++q) {
-            const int e = tid + NT * q, i = e / TP, k = e % TP;
-            Xg[i][k] = (ra[i] >= 0 && rb[k] >= 0) ? xv[q] : 0.0;
+        for (int ti = 0; ti < 2; ++ti) {
+            u[ti] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) u[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[ti][j], vb[j], u[ti], 0, 0, 0);
         }
-        if (fa || fb) {
-            if (bytes && tid == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);
+        // X_new(ti, tj) = Wa(:, 16 ti ..)^T U, contraction rows 4 j + lk
+        const int gc = tj_row<TB>(st, b, D.np, D.nblk, P.nv, TB * tj + li);
 #pragma unroll
-            for (int q = 0; q < QX; ++q) {
-                const int e = tid + NT * q, i = e / TP, k = e % TP;
-                Wa[i][k] = fa ? wva[q] : (i == k ? 1.0 : 0.0);
-                Wb[i][k] = fb ? wvb[q] : (i == k ? 1.0 : 0.0);
+        for (int ti = 0; ti < 2; ++ti) {
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int e = 4 * j + lk, c = TB * ti + li;
+                const double av = fa ? wa[e * TP + c] : (e == c ? 1.0 : 0.0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, u[j >> 2][j & 3], acc, 0, 0, 0);
             }
-            __syncthreads();
-            {   // T = Wa^T X, one 16 x 16 tile per wave
-                const int ti = wv >> 1, tj = wv & 1;
-                v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], Xg[k0 + lk][tj * 16 + li],
-                                                                acc, 0, 0, 0);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Tt[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
-            }
-            __syncthreads();
-            {   // G_ab = T Wb, into X
-                const int ti = wv >> 1, tj = wv & 1;
-                v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int k0 = 0; k0 < TP; k0 += 4)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Tt[ti * 16 + li][k0 + lk], Wb[k0 + lk][tj * 16 + li],
-                                                                acc, 0, 0, 0);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Xg[ti * 16 + lk + 4 * r][tj * 16 + li] = acc[r];
-            }
-        }
-        __syncthreads();
-        for (int e = tid; e < TP * TP; e += NT) {
-            const int i = e / TP, k = e % TP;
-            const int gr = ra[i], gc = rb[k];
-            if (gr >= 0 && gc >= 0) Gn[(int64_t)gr * ldn + gc] = Xg[i][k];
-            if (a != b) {
-                const int tr = rb[i], tc = ra[k];
-                if (tr >= 0 && tc >= 0) Gn[(int64_t)tr * ldn + tc] = Xg[k][i];
+            for (int r = 0; r < 4; ++r) {
+                const int gr = tj_row<TB>(st, a, D.np, D.nblk, P.nv, TB * ti + lk + 4 * r);
+                if (gr >= 0 && gc >= 0) {
+                    Gn[(int64_t)gr * ldn + gc] = acc[r];
+                    if (a != b) Gn[(int64_t)gc * ldn + gr] = acc[r];
+                }
             }
         }
         return;
     }
-    // rt slab (a, columns [c0, c0 + kTCol)), in place
-    const int64_t u = t - D.half * D.half;
+    // rt slab (a, columns [c0, c0 + kTCol)), in place: wave wv takes columns
+    // c0 + 16 wv .. c0 + 16 wv + 15 of every row of pair a
+    const int64_t u = t - gw;
     const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
     if (!flags[a]) return;
     if (bytes && tid == 0) atomicAdd(bytes, kSlabBytes);
-    if (tid < TP) ra[tid] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, tid);
-    __syncthreads();
-    // W and the rt slab are loaded before the first value is used (padding
-    // rows and columns from row 0 / column nv - 1, selected to zero)
     const double *wa = wset + a * TP * TP;
-    constexpr int QW = TP * TP / NT, QS = TP * kTCol / NT;
-    double wq[QW], sv[QS];
+    const int64_t col = c0 + TB * wv + li;
+    const bool cok = col < P.nv;
+    double xs[8];
 #pragma unroll
-    for (int q = 0; q < QW; ++q) wq[q] = wa[tid + NT * q];
-#pragma unroll
-    for (int q = 0; q < QS; ++q) {
-        const int e = tid + NT * q, i = e / kTCol, j = e % kTCol;
-        sv[q] = P.rt[(int64_t)max(ra[i], 0) * P.ldr + min(c0 + j, P.nv - 1)];
+    for (int j = 0; j < 8; ++j) {
+        const int r = tj_row<TB>(st, a, D.np, D.nblk, P.nv, 4 * j + lk);
+        const bool ok = r >= 0 && cok;
+        const double val = P.rt[ok ? (int64_t)r * P.ldr + col : 0];
+        xs[j] = ok ? val : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < QW; ++q) {
-        const int e = tid + NT * q;
-        Wa[e / TP][e % TP] = wq[q];
-    }
-#pragma unroll
-    for (int q = 0; q < QS; ++q) {
-        const int e = tid + NT * q, i = e / kTCol, j = e % kTCol;
-        X[i][j] = (ra[i] >= 0 && c0 + j < P.nv) ? sv[q] : 0.0;
-    }
-    __syncthreads();
-    for (int q = wv; q < (TP / 16) * (kTCol / 16); q += NT / 64) {
-        const int ti = q / (kTCol / 16), tj = q % (kTCol / 16);
+    for (int ti = 0; ti < 2; ++ti) {
         v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k0 = 0; k0 < TP; k0 += 4)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Wa[k0 + lk][ti * 16 + li], X[k0 + lk][tj * 16 + li], acc, 0, 0,
-                                                        0);
+        for (int j = 0; j < 8; ++j)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[(4 * j + lk) * TP + TB * ti + li], xs[j], acc, 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = ti * 16 + lk + 4 * r, j = tj * 16 + li;
-            const int gr = ra[i];
-            if (gr >= 0 && c0 + j < P.nv) P.rt[(int64_t)gr * P.ldr + c0 + j] = acc[r];
+            const int gr = tj_row<TB>(st, a, D.np, D.nblk, P.nv, TB * ti + lk + 4 * r);
+            if (gr >= 0 && cok) P.rt[(int64_t)gr * P.ldr + col] = acc[r];
         }
     }
 }
@@ -991,14 +809,6 @@ static int64_t tj_work(int64_t nv) {
     return tj_g2_off<TB>(half, nv) + nv * nv;
 }
 
-static int64_t tj_work_max(int64_t nv) {
-    const int64_t a = tj_work<16>(nv), b = tj_work<32>(nv);
-    return a > b ? a : b;
-}
-
-static int g_tj_inner = 1;
-static int g_tj_block = 0;   // 0: automatic (16-row blocks: measured at least as fast up to 2048)
-
 template <int TB>
 static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
                         int32_t *counters, double tol, int max_sweeps, int *sweeps, hipStream_t st) {
@@ -1038,10 +848,8 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     // owner table of the fused steps, after n_items doubles for the flags
     int32_t *owner = (int32_t *)(dbytes + 1 + n_items);
     std::vector<int32_t> hc(n_items), hk(n_items);
-    // fused steps (16-row blocks; 32-row blocks run solves and updates in
-    // launches of their own): buffer holding each problem's Gram matrix
-    // (0: P.g, 1: the workspace copy); all problems of a sweep share it
-    constexpr bool fused = TB == 16;
+    // buffer holding each problem's Gram matrix (0: P.g, 1: the workspace
+    // copy); all problems of a sweep share it
     std::vector<int32_t> gpar(n_items, 0);
     int cur = 0;
     std::vector<int> idx;
@@ -1057,48 +865,32 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
             it.solve_start = ss;
             it.tile_start = ts;
             ss += D.half;
-            ts += D.half * D.half + D.half * D.ncs;
+            ts += tj_ugroups(D);
             if (D.np > max_np) max_np = D.np;
             act.push_back(it);
         }
         if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, counters, n_items);
-        if (fused) {
-            hipLaunchKernelGGL(tj_owner_kernel<TB>, dim3((unsigned)act.size()), dim3(256), 0, st, items, ss, owner);
-            SYN_LAUNCH_CHECK("gram owner table");
-        }
-        for (int64_t step = 0; fused && step < max_np; ++step) {
+        hipLaunchKernelGGL(tj_owner_kernel<TB>, dim3((unsigned)act.size()), dim3(256), 0, st, items, ss, owner);
+        SYN_LAUNCH_CHECK("gram owner table");
+        for (int64_t step = 0; step < max_np; ++step) {
             // solves of step (all but the last launch) beside the updates of
             // step - 1 (all but the first)
             const bool hs = step < max_np - 1, hu = step > 0;
             const int64_t sp = hs ? (ss + 7) / 8 * 8 : 0, tp = hu ? (ts + 7) / 8 * 8 : 0;
             ProfScope ps(kTagGramSolve, 0.0, st);
-            hipLaunchKernelGGL(tjf_step_kernel<16>, dim3((unsigned)(sp + tp)), dim3(256), 0, st, items,
-                               (int)act.size(), owner, owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp,
-                               ts, cur, tol, g_tj_inner, kbytes);
+            hipLaunchKernelGGL(tjf_step_kernel<TB>, dim3((unsigned)(sp + tp)), dim3(256), 0, st, items, owner,
+                               owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp, ts, cur, tol, kbytes);
             if (hu) cur ^= 1;
         }
-        if (fused)
-            for (int i : idx) gpar[i] = cur;
-        for (int64_t step = 0; !fused && step < max_np - 1; ++step) {
-            {
-                ProfScope ps(kTagGramSolve, 0.0, st);
-                hipLaunchKernelGGL(tj_solve_kernel<TB>, dim3((unsigned)ss), dim3(TB * TB), 0, st, items,
-                                   (int)act.size(), step, tol, g_tj_inner);
-            }
-            {
-                ProfScope ps(kTagGramUpdate, 0.0, st);
-                hipLaunchKernelGGL(tj_update_kernel<TB>, dim3((unsigned)ts), dim3(kTUpdThreads), 0, st, items,
-                                   (int)act.size(), step, kbytes);
-            }
-        }
+        for (int i : idx) gpar[i] = cur;
         SYN_LAUNCH_CHECK("gram step");
         // a problem goes on when its sweep rotated and the end-of-sweep test
         // finds an entry that would still rotate: this saves the last sweep,
         // which only confirmed convergence, with the same result
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, chk, n_items);
         hipLaunchKernelGGL(tj_check_kernel<TB>, dim3((unsigned)act.size(), 64), dim3(256), 0, st, items,
-                           fused ? cur : 0, tol, counters, chk);
+                           cur, tol, counters, chk);
         SYN_LAUNCH_CHECK("gram sweep test");
         if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,
                                               st),
@@ -1121,13 +913,11 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
 
     // eigenvalues, host argsort per problem, permutation of rt rows and s
     if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
-    if (fused) {
-        // the Gram matrix is diagonalised in place: bring back the ones the
-        // last fused step left in the workspace copy
-        if (int rc = upload(gpar.data(), counters, n_items * sizeof(int32_t), "gram parity")) return rc;
-        hipLaunchKernelGGL(tjf_copyback_kernel<TB>, dim3(n_items, 64), dim3(256), 0, st, items, counters);
-        SYN_LAUNCH_CHECK("gram copy back");
-    }
+    // the Gram matrix is diagonalised in place: bring back the ones the last
+    // fused step left in the workspace copy
+    if (int rc = upload(gpar.data(), counters, n_items * sizeof(int32_t), "gram parity")) return rc;
+    hipLaunchKernelGGL(tjf_copyback_kernel<TB>, dim3(n_items, 64), dim3(256), 0, st, items, counters);
+    SYN_LAUNCH_CHECK("gram copy back");
     hipLaunchKernelGGL(tj_diag_kernel, dim3(n_items), dim3(256), 0, st, items);
     SYN_LAUNCH_CHECK("gram diag");
     std::vector<int64_t> poff(n_items), toff(n_items);
@@ -1174,14 +964,10 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
 
 }  // namespace syn
 
-extern "C" void syn_tune_gram_inner_sweeps(int sweeps) { syn::g_tj_inner = (sweeps >= 1 && sweeps <= 8) ? sweeps : 1; }
-
-extern "C" void syn_tune_gram_block_rows(int rows) { syn::g_tj_block = (rows == 16 || rows == 32) ? rows : 0; }
-
 extern "C" int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items) {
     int64_t tot = 0;
     for (int i = 0; i < n_items; ++i)
-        tot += syn::tj_work_max(nv[i]) + 2 * nv[i] + (syn::tj_owner_count<16>(nv[i]) + 1) / 2;
+        tot += syn::tj_work<16>(nv[i]) + 2 * nv[i] + (syn::tj_owner_count<16>(nv[i]) + 1) / 2;
     return tot + 3 * (int64_t)n_items + 16;
 }
 
@@ -1196,18 +982,12 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
     }
     if (n_items == 0) return SYN_OK;
     std::vector<syn_gbj_item> all(items_host, items_host + n_items);
-    int64_t max_nv = 0;
     for (int i = 0; i < n_items; ++i) {
         if (all[i].nv <= 0 || all[i].ldg < all[i].nv || all[i].ldr < all[i].nv || !all[i].g || !all[i].rt ||
             !all[i].s) {
             set_error("syn_jacobi_gram_blocked_f64: bad problem %d (nv=%lld)", i, (long long)all[i].nv);
             return SYN_EINVAL;
         }
-        if (all[i].nv > max_nv) max_nv = all[i].nv;
     }
-    const int tb = g_tj_block ? g_tj_block : 16;
-    (void)max_nv;
-    hipStream_t st = (hipStream_t)stream;
-    if (tb == 32) return gram_blocked<32>(items, all, n_items, work, counters, tol, max_sweeps, sweeps, st);
-    return gram_blocked<16>(items, all, n_items, work, counters, tol, max_sweeps, sweeps, st);
+    return gram_blocked<16>(items, all, n_items, work, counters, tol, max_sweeps, sweeps, (hipStream_t)stream);
 }
