@@ -2,28 +2,26 @@
 //
 // For the right sweep of the SVD rounding the matrix to decompose is a Gram
 // matrix G = C^T C (or C C^T) of size g = 65 .. a few thousand.  The rows and
-// columns of G are split into blocks of kTB; a sweep pairs every two blocks
+// columns of G are split into blocks of 16; a sweep pairs every two blocks
 // once in the round-robin order, and in one step the pairs are disjoint:
 //
-//   solve   one small work-group per pair: the kTP x kTP diagonal tile of the
-//           pair (rows and columns of both blocks) is loaded into LDS and
-//           rotated by two-sided Jacobi, accumulating the orthogonal W of
-//           the pair: a full round-robin sweep of the tile at the first step
-//           of a pass (which meets every block), only the planes between
-//           the pair's two blocks at the others, so a pass treats every
-//           plane of G once;
-//   update  one work-group per tile: G_ab <- W_a^T G_ab W_b for every pair of
-//           pairs (a, b) and rt_a <- W_a^T rt_a for every 64-column slab of
-//           the accumulated eigenvector rows, on the f64 MFMA.
+//   solve   one work-group per pair: the 32 x 32 diagonal tile of the pair
+//           (rows and columns of both blocks) is built in LDS and rotated by
+//           two-sided Jacobi, accumulating the orthogonal W of the pair: a
+//           full round-robin sweep of the tile at the first step of a pass
+//           (which meets every block), only the planes between the pair's
+//           two blocks at the others, so a pass treats every plane of G once;
+//   update  G_ab <- W_a^T G_ab W_b for every pair of pairs a <= b (written
+//           with its transpose) and rt_a <- W_a^T rt_a for every 64-column
+//           slab of the accumulated eigenvector rows, on the f64 MFMA with
+//           the operands in registers.
 //
-// Both launches of a step spread over many work-groups (the update over
-// (g/kTP)^2 + (g/kTP)(g/64) tiles), and several problems share every launch.
-// With 16-row blocks a step is one launch: the solves of step s beside the
-// updates of step s - 1.  A pair whose tile has no |G_ik| > max(tol
-// sqrt(|G_ii G_kk|), eps max_j G_jj) among the planes it treats is not
-// rotated; a problem stops after a pass that rotated nothing or when no
-// off-diagonal entry of G passes that test at the end of a pass, so on exit
-// every off-diagonal entry of G meets it, G is diagonal to it and
+// A step is one launch, the solves of step s beside the updates of step
+// s - 1, and several problems share every launch.  A pair whose tile has no
+// |G_ik| > max(tol sqrt(|G_ii G_kk|), eps max_j G_jj) among the planes it
+// treats is not rotated; a problem stops after a pass that rotated nothing
+// or when no off-diagonal entry of G passes that test at the end of a pass,
+// so on exit every off-diagonal entry of G meets it, G is diagonal to it and
 // G_in = rt^T diag(G_out) rt.  This is the two-sided block Jacobi method
 // (Hansen; Van Loan; Becka, Oksa & Vajtersic) in its parallel round-robin
 // form, with the planes in the cyclic-by-blocks order.
@@ -36,10 +34,10 @@
 
 namespace syn {
 
-// Blocks of TB rows (16 or 32, chosen per call); a pair tile has TP = 2 TB
-// rows.  The pair solve uses one thread per 2 x 2 block of a round, (TP/2)^2
-// threads; the tile update four waves.
-constexpr int kTUpdThreads = 256;    // four waves per tile update
+// Blocks of TB = 16 rows (the helpers keep TB as a parameter); a pair tile
+// has TP = 2 TB rows.  The pair solve uses one thread per 2 x 2 block of a
+// round, (TP/2)^2 threads; an update work-group four waves.
+constexpr int kTUpdThreads = 256;    // four waves per update work-group
 constexpr int kTCol = 64;            // columns of an rt slab
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -457,12 +455,11 @@ __device__ __forceinline__ int64_t tj_xcd_order(int64_t b, int64_t padded) {
 }
 
 // The register products below use the f64 MFMA 16x16x4 layout: lane
-// l = 16 lk + li supplies A[li][k] and B[k][li] of each k-step and holds
-// D[lk + 4r][li] in register r.  A contraction may visit its index in any
-// order as long as both operands follow it: k-step j of lane lk visits
-// index 8 lk + j where the operand comes from a Gram row (eight consecutive
-// doubles of one block per lane), and 4 j + lk where it comes from an MFMA
-// result held in registers (whose rows are lk + 4r).
+// l = 16 lk + li supplies A[li][k] and B[k][li] of k-step j (k = 4 j + lk)
+// and holds D[lk + 4r][li] in register r, so a product's result is the B
+// operand of a next product that contracts over its rows.  Gram tiles enter
+// as A operands through their transposes: A[li][k] = G(row of k, row of li),
+// sixteen consecutive doubles of one Gram row per load.
 template <int TB>
 __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item *__restrict__ items,
                                                           const int32_t *__restrict__ owner_s,
@@ -526,23 +523,23 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
             const int fr = fl[wr], fc = fl[wc];
             const double *Wr = tj_wset<TB>(P, D.half, pp) + wr * TP * TP;
             const double *Wc = tj_wset<TB>(P, D.half, pp) + wc * TP * TP;
-            // G_old rows 16 rho + li of pair wr, columns 8 lk .. 8 lk + 7 of
-            // pair wc (one block, consecutive); padding reads as zero
-            const int cb = tj_row<TB>(sp, wc, D.np, D.nblk, P.nv, 8 * lk);
+            // G_old(rows of wr, columns of wc), read through its transpose
+            // (G is kept symmetric) so that lanes li take 16 consecutive
+            // doubles of one row; padding reads as zero
+            int rr[2];
+#pragma unroll
+            for (int rho = 0; rho < 2; ++rho) rr[rho] = tj_row<TB>(sp, wr, D.np, D.nblk, P.nv, 16 * rho + li);
             double g[2][8], v[8];
 #pragma unroll
-            for (int rho = 0; rho < 2; ++rho) {
-                const int r = tj_row<TB>(sp, wr, D.np, D.nblk, P.nv, 16 * rho + li);
+            for (int j = 0; j < 8; ++j) {
+                const int cr = tj_row<TB>(sp, wc, D.np, D.nblk, P.nv, 4 * j + lk);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const bool ok = r >= 0 && cb >= 0 && cb + j < P.nv;
-                    const double val = Gc[ok ? (int64_t)r * ldc + cb + j : 0];
+                for (int rho = 0; rho < 2; ++rho) {
+                    const bool ok = rr[rho] >= 0 && cr >= 0;
+                    const double val = Gc[ok ? (int64_t)cr * ldc + rr[rho] : 0];
                     g[rho][j] = ok ? val : 0.0;
                 }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int e = 8 * lk + j;
+                const int e = 4 * j + lk;
                 v[j] = fc ? Wc[e * TP + hc * TB + li] : (e == hc * TB + li ? 1.0 : 0.0);
             }
             v4d z[2];
@@ -603,31 +600,35 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
         int64_t ldc, ldn;
         const double *Gc = gbuf(P, cur, D.half, ldc);
         double *Gn = gbuf(P, cur ^ 1, D.half, ldn);
-        // X rows 16 ti + li (rows of pair a), columns 8 lk .. 8 lk + 7 (pair b)
-        const int cb = tj_row<TB>(st, b, D.np, D.nblk, P.nv, 8 * lk);
-        int ra2[2];
+        // x[ti][j] = X(16 ti + li, 4 j + lk), X = G(rows of pair a, rows of
+        // pair b), read through its transpose (G is kept symmetric) so that
+        // lanes li take 16 consecutive doubles of one row
+        int ra2[2], cr[8];
         double x[2][8];
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti) {
-            ra2[ti] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, TB * ti + li);
+        for (int ti = 0; ti < 2; ++ti) ra2[ti] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, TB * ti + li);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const bool ok = ra2[ti] >= 0 && cb >= 0 && cb + j < P.nv;
-                const double val = Gc[ok ? (int64_t)ra2[ti] * ldc + cb + j : 0];
+        for (int j = 0; j < 8; ++j) {
+            cr[j] = tj_row<TB>(st, b, D.np, D.nblk, P.nv, 4 * j + lk);
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti) {
+                const bool ok = ra2[ti] >= 0 && cr[j] >= 0;
+                const double val = Gc[ok ? (int64_t)cr[j] * ldc + ra2[ti] : 0];
                 x[ti][j] = ok ? val : 0.0;
             }
         }
         if (!fa && !fb) {
-            // nothing rotated: copy this wave's columns (lanes lk = 2 tj, 2 tj + 1)
-            if ((lk >> 1) == tj && cb >= 0) {
+            // nothing rotated: wave tj copies rows 16 tj .. 16 tj + 15 of X
+            // (and their transpose)
 #pragma unroll
-                for (int ti = 0; ti < 2; ++ti) {
-                    if (ra2[ti] < 0) continue;
+            for (int ti = 0; ti < 2; ++ti) {   // static register indices only
+                const int gr = ra2[ti];
+                if (ti != tj || gr < 0) continue;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        if (cb + j >= P.nv) break;
-                        Gn[(int64_t)ra2[ti] * ldn + cb + j] = x[ti][j];
-                        if (a != b) Gn[(int64_t)(cb + j) * ldn + ra2[ti]] = x[ti][j];
+                for (int j = 0; j < 8; ++j) {
+                    if (cr[j] >= 0) {
+                        Gn[(int64_t)gr * ldn + cr[j]] = x[ti][j];
+                        if (a != b) Gn[(int64_t)cr[j] * ldn + gr] = x[ti][j];
                     }
                 }
             }
@@ -639,7 +640,7 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
         double vb[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int e = 8 * lk + j, c = TB * tj + li;
+            const int e = 4 * j + lk, c = TB * tj + li;
             vb[j] = fb ? wb[e * TP + c] : (e == c ? 1.0 : 0.0);
         }
         v4d u[2];
