@@ -73,8 +73,9 @@ def parse():
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="fp64",
                     help="arithmetic of the served form (the per-chain comparison is always plain FP64); "
                          "FP32 products add ~1e-7 per layer, which long circuits accumulate past 1e-6")
-    ap.add_argument("--serve-plain-seconds", type=float, default=30.0,
-                    help="time box of the per-chain comparison of the served form")
+    ap.add_argument("--serve-plain-seconds", type=float, default=150.0,
+                    help="time box of the per-chain comparison of the served form (its complete circuits "
+                         "normally end well inside it)")
     ap.add_argument("--chain-steps", type=int, default=5, help="timed steps of the cfg1 chain form")
     ap.add_argument("--only", choices=FORMS, help="run one form only (profiling)")
     ap.add_argument("--graph-steps", type=int, default=3,
@@ -406,13 +407,17 @@ def chain_form(np, torch, args):
 # describes them, so they get no roofline; bench.py names the largest of
 # them beside the roofline when it takes more GPU time than the dominant
 # work kernel.
-LATENCY_KERNELS = ("gram_solve_kernel", "chol_diag_kernel", "jacobi_kernel", "qr_kernel", "bjacobi_step_kernel")
+LATENCY_KERNELS = ("chol_diag_kernel", "jacobi_kernel", "qr_kernel", "bjacobi_step_kernel")
+# work figures recorded without launches of their own: the bytes the fused
+# Jacobi step's update work-groups move (counted on the device in the census
+# step only, where every kernel is timed)
+WORK_ONLY = {"tjf_step_bytes": "tjf_step_kernel"}
 
 
 def dominant_work_kernel(prof):
     """The kernel with an algorithmic work count (not in LATENCY_KERNELS)
     that took the most event time."""
-    work = {k: v for k, v in prof.items() if k not in LATENCY_KERNELS}
+    work = {k: v for k, v in prof.items() if k not in LATENCY_KERNELS and k not in WORK_ONLY}
     return max(work.items(), key=lambda kv: kv[1][1])[0]
 
 
@@ -429,6 +434,10 @@ def kernel_roofline(census, timed, dom, steps):
     the census step's GPU time and the largest latency-bound kernel."""
     total_ms = sum(v[1] for v in census.values())
     dl, dms, dwork = timed[dom]
+    for wk, kern in WORK_ONLY.items():
+        if kern == dom and wk in census:
+            # counted in the census step (one step); the same per step
+            dwork = census[wk][2] * steps
     if dom in BYTE_KERNELS:
         achieved = dwork / (dms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
@@ -453,10 +462,16 @@ def kernel_roofline(census, timed, dom, steps):
 
 
 def census_table(census):
+    def work(k, v):
+        for wk, kern in WORK_ONLY.items():
+            if kern == k and wk in census:
+                return census[wk][2]
+        return v[2]
     return {k: {"launches": v[0], "ms": round(v[1], 2),
-                "rate": round(v[2] / (v[1] * 1e-3) / (1e9 if k in BYTE_KERNELS else 1e12), 3),
+                "rate": round(work(k, v) / (v[1] * 1e-3) / (1e9 if k in BYTE_KERNELS else 1e12), 3)
+                if v[1] else 0.0,
                 "rate_unit": "GB/s" if k in BYTE_KERNELS else "TFLOP/s"}
-            for k, v in sorted(census.items(), key=lambda kv: -kv[1][1])}
+            for k, v in sorted(census.items(), key=lambda kv: -kv[1][1]) if k not in WORK_ONLY}
 
 
 def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d, rep):
@@ -487,7 +502,7 @@ def measured_traffic(kernel):
 
 
 # kernels whose profiler work figure is bytes moved (the rest: flops)
-BYTE_KERNELS = ("apply_kernel", "gram_update_kernel")
+BYTE_KERNELS = ("apply_kernel", "tjf_step_kernel")
 
 
 def sval_errors(np, got, sv, kept):
@@ -704,12 +719,13 @@ def serve_form(np, torch, args):
     log(f"serve: {rep.site_layers} site-layers served in {rep.makespan:.1f} s; per-chain sample next")
     lat = rep.latency_percentiles()
     pyth = max(abs(r.norm2 + r.discarded - 1.0) for r in rep.results)
-    # per-chain form: complete circuits of every third job of the same stream,
-    # one chain and one layer at a time, time-boxed (a box that closes early
-    # leaves the later, heavier layers out, which favours this form)
+    # per-chain form: complete circuits of every fourth job of the same
+    # stream, one chain and one layer at a time, inside a time box (a box that
+    # closed early would leave the later, heavier layers out and favour this
+    # form; the result says when it did)
     dev = ops.device()
     done, t0, boxed = 0, time.perf_counter(), False
-    subset = jobs[::3]
+    subset = jobs[::4]
     last_log = time.perf_counter()
     for j in subset:
         cores = [torch.from_numpy(np.eye(2)[0].reshape(1, 2, 1).copy()).to(dev) for _ in range(j["n"])]

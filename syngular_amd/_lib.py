@@ -142,8 +142,8 @@ PROF_TAGS = {
     6: "qr_kernel",
     7: "jacobi_kernel",
     8: "apply_kernel",
-    9: "gram_solve_kernel",
-    10: "gram_update_kernel",
+    9: "tjf_step_kernel",
+    10: "tjf_step_bytes",
     11: "gemm_mixed<64,64>",
     12: "gemm_mixed<128,128>",
     13: "gemm_f64_splitk<128,128>",
@@ -175,7 +175,7 @@ def prof_begin(tags=None):
 
 def prof_end():
     """Stop profiling; returns {tag_name: (launches, ms, flops)} for the tags
-    that recorded launches."""
+    that recorded launches or work (a work-only tag has no launches)."""
     n = PROF_NTAGS
     la = (ctypes.c_int64 * n)()
     ms = (ctypes.c_double * n)()
@@ -183,7 +183,7 @@ def prof_end():
     rc = load().syn_prof_end(la, ms, fl, n)
     if rc < 0:
         raise SyngularError(f"syn_prof_end failed ({rc}): {load().syn_last_error().decode()}")
-    return {PROF_TAGS.get(i, f"tag{i}"): (la[i], ms[i], fl[i]) for i in range(n) if la[i]}
+    return {PROF_TAGS.get(i, f"tag{i}"): (la[i], ms[i], fl[i]) for i in range(n) if la[i] or fl[i]}
 
 
 class SyngularError(RuntimeError):

@@ -35,7 +35,7 @@ from ._lib import call
 F64 = torch.float64
 ALIGN = 32                      # doubles (256 B) between sub-allocations
 LDS_JACOBI_MAX = 64             # Gram matrices up to this size: one work-group in LDS
-CHOLQR_MIN_COLS = 96            # tall unfoldings with this many columns: CholeskyQR2
+CHOLQR_MIN_COLS = 96            # tall unfoldings with this many columns (or too big for LDS): CholeskyQR2
 # Jacobi orthogonality thresholds, fixed per path (not per launch) so that a
 # chain's rotations do not depend on the other chains of its batch: the
 # rounding bound length * eps of the largest problem each path takes.
@@ -624,9 +624,11 @@ class ChainBatch:
             wide = m <= r
             tall = ~wide
             new_r = np.where(wide, m, r)
-            # Q factors: identity for wide unfoldings, Householder or
-            # CholeskyQR2 (which brings its own Q buffer) for tall ones
-            cq = tall & (r >= CHOLQR_MIN_COLS)
+            # Q factors: identity for wide unfoldings, Householder (narrow
+            # matrices that fit one work-group's LDS) or CholeskyQR2 (which
+            # brings its own Q buffer) for tall ones
+            hh = tall & (r < CHOLQR_MIN_COLS) & (m * r <= QR_LDS_CLASSES[-1])
+            cq = tall & ~hh
             qt, q = arena(np.where(cq, 0, m * new_r))
             if wide.any():
                 wi = wide.nonzero()[0]
@@ -634,7 +636,6 @@ class ChainBatch:
             S = ptr[a, k].copy()          # wide: S = L itself (m x r, ld r)
             lds = r.copy()
             sta = np.zeros(len(a), np.int64)   # 1: S holds L1 and the absorb takes L1^T
-            hh = tall & (r < CHOLQR_MIN_COLS)
             if hh.any():
                 # Householder, one work-group per matrix; S = Q^T L  (r x r)
                 ti = hh.nonzero()[0]

@@ -15,8 +15,8 @@ enum ProfTag {
     kTagQr = 6,
     kTagJacobi = 7,
     kTagApply = 8,
-    kTagGramSolve = 9,
-    kTagGramUpdate = 10,
+    kTagGramStep = 9,      // the fused Jacobi step (tjf_step_kernel)
+    kTagGramBytes = 10,    // bytes its update work-groups move (work only, no launches)
     kTagGemmMixed64 = 11,
     kTagGemmMixed128 = 12,
     kTagGemmSplitK128 = 13,

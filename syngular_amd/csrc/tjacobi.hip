@@ -28,8 +28,6 @@
 #include "common.hpp"
 #include "prof.hpp"
 
-#include <algorithm>
-#include <numeric>
 #include <vector>
 
 namespace syn {
@@ -634,7 +632,7 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
             }
             return;
         }
-        if (bytes && tj == 0 && lane == 0) atomicAdd(bytes, (a == b ? 2ull : 3ull) * kTileBytes);
+        if (bytes && tj == 0 && lane == 0) atomicAdd(bytes + owner_t[ub], (a == b ? 2ull : 3ull) * kTileBytes);
         const double *wa = wset + a * TP * TP, *wb = wset + b * TP * TP;
         // U = X Wb(:, tj): two 16 x 16 tiles (rows 0-15, 16-31)
         double vb[8];
@@ -677,7 +675,7 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
     const int64_t u = t - gw;
     const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
     if (!flags[a]) return;
-    if (bytes && tid == 0) atomicAdd(bytes, kSlabBytes);
+    if (bytes && tid == 0) atomicAdd(bytes + owner_t[ub], kSlabBytes);
     const double *wa = wset + a * TP * TP;
     const int64_t col = c0 + TB * wv + li;
     const bool cok = col < P.nv;
@@ -798,6 +796,39 @@ __global__ __launch_bounds__(256) void tj_permute_kernel(const syn_gbj_item *__r
     }
 }
 
+// Decreasing order of each problem's s (ties by index, as a stable sort):
+// rank[i] = #{j : s_j > s_i or (s_j == s_i and j < i)}, perm[rank[i]] = i.
+// s is staged through LDS in chunks of kRankChunk values.
+constexpr int kRankChunk = 2048;
+__global__ __launch_bounds__(256) void tj_rank_kernel(const syn_gbj_item *__restrict__ items,
+                                                      const int64_t *__restrict__ poff, int64_t *__restrict__ perm) {
+    __shared__ double sv[kRankChunk];
+    const syn_gbj_item &P = items[blockIdx.x];
+    const int64_t nv = P.nv;
+    int64_t *pm = perm + poff[blockIdx.x];
+    for (int64_t i0 = 0; i0 < nv; i0 += 256) {
+        const int64_t i = i0 + threadIdx.x;
+        // a NaN orders last (as -inf), so the ranks always form a permutation
+        const double s0 = i < nv ? P.s[i] : 0.0, si = s0 == s0 ? s0 : -__builtin_huge_val();
+        int64_t rk = 0;
+        for (int64_t c0 = 0; c0 < nv; c0 += kRankChunk) {
+            const int64_t cn = nv - c0 < kRankChunk ? nv - c0 : kRankChunk;
+            __syncthreads();
+            for (int64_t e = threadIdx.x; e < cn; e += 256) {
+                const double x = P.s[c0 + e];
+                sv[e] = x == x ? x : -__builtin_huge_val();
+            }
+            __syncthreads();
+            if (i < nv)
+                for (int64_t e = 0; e < cn; ++e) {
+                    const double sj = sv[e];
+                    rk += (sj > si) || (sj == si && c0 + e < i);
+                }
+        }
+        if (i < nv) pm[rk] = i;
+    }
+}
+
 __global__ __launch_bounds__(256) void tj_zero_kernel(int32_t *c, int n) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) c[i] = 0;
 }
@@ -837,17 +868,33 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     int64_t tot_nv = 0;
     for (int i = 0; i < n_items; ++i) tot_nv += all[i].nv;
     unsigned long long *dbytes = (unsigned long long *)(work + tot_w + tot_nv + 2 * (int64_t)n_items);
-    hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, (int32_t *)dbytes, 2);
+    hipLaunchKernelGGL(tj_zero_kernel, dim3((n_items + 127) / 128), dim3(256), 0, st, (int32_t *)dbytes,
+                       2 * n_items);
     SYN_LAUNCH_CHECK("gram byte counter");
-    // the update launches count their bytes only when a profile asks for them
-    // (one atomic per work-group on a single address)
-    unsigned long long *kbytes = prof_wants(kTagGramUpdate) ? dbytes : nullptr;
+    // the update work-groups count the Gram and rt bytes they move, one
+    // counter per problem of the sweep, only when a profile asks for them
+    unsigned long long *kbytes = prof_wants(kTagGramBytes) ? dbytes : nullptr;
 
     // end-of-sweep test flags, in the workspace's spare tail after the byte
     // counter (at least tot_nv + 15 doubles are left there)
-    int32_t *chk = (int32_t *)(dbytes + 1);
+    int32_t *chk = (int32_t *)(dbytes + n_items);
     // owner table of the fused steps, after n_items doubles for the flags
-    int32_t *owner = (int32_t *)(dbytes + 1 + n_items);
+    int32_t *owner = (int32_t *)(dbytes + 2 * (int64_t)n_items);
+    // order and permutation scratch of the end: each problem's offset in the
+    // permutation array and its rt-copy scratch
+    int64_t *dperm = (int64_t *)(work + tot_w);
+    int64_t *dpoff = dperm + tot_nv;
+    int64_t *dtoff = dpoff + n_items;
+    {
+        std::vector<int64_t> pt(2 * (size_t)n_items);
+        int64_t acc_nv = 0;
+        for (int i = 0; i < n_items; ++i) {
+            pt[i] = acc_nv;
+            acc_nv += all[i].nv;
+            pt[n_items + i] = woff[i] + tj_floor_off<TB>(tj_dims<TB>(all[i].nv).half) + 1;
+        }
+        if (int rc = upload(pt.data(), dpoff, pt.size() * sizeof(int64_t), "gram offsets")) return rc;
+    }
     std::vector<int32_t> hc(n_items), hk(n_items);
     // buffer holding each problem's Gram matrix (0: P.g, 1: the workspace
     // copy); all problems of a sweep share it
@@ -879,7 +926,7 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
             // step - 1 (all but the first)
             const bool hs = step < max_np - 1, hu = step > 0;
             const int64_t sp = hs ? (ss + 7) / 8 * 8 : 0, tp = hu ? (ts + 7) / 8 * 8 : 0;
-            ProfScope ps(kTagGramSolve, 0.0, st);
+            ProfScope ps(kTagGramStep, 0.0, st);
             hipLaunchKernelGGL(tjf_step_kernel<TB>, dim3((unsigned)(sp + tp)), dim3(256), 0, st, items, owner,
                                owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp, ts, cur, tol, kbytes);
             if (hu) cur ^= 1;
@@ -912,7 +959,8 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     }
     if (sweeps) *sweeps = used;
 
-    // eigenvalues, host argsort per problem, permutation of rt rows and s
+    // eigenvalues, their decreasing order (ties by index) on the device, the
+    // permutation of rt rows and s
     if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
     // the Gram matrix is diagonalised in place: bring back the ones the last
     // fused step left in the workspace copy
@@ -921,45 +969,23 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     SYN_LAUNCH_CHECK("gram copy back");
     hipLaunchKernelGGL(tj_diag_kernel, dim3(n_items), dim3(256), 0, st, items);
     SYN_LAUNCH_CHECK("gram diag");
-    std::vector<int64_t> poff(n_items), toff(n_items);
-    int64_t acc_nv = 0;
-    for (int i = 0; i < n_items; ++i) {
-        poff[i] = acc_nv;
-        acc_nv += all[i].nv;
-        toff[i] = woff[i] + tj_floor_off<TB>(tj_dims<TB>(all[i].nv).half) + 1;
-    }
-    std::vector<double> hs(tot_nv);
-    for (int i = 0; i < n_items; ++i)
-        if (int rc = hip_check(hipMemcpyAsync(hs.data() + poff[i], all[i].s, all[i].nv * sizeof(double),
-                                              hipMemcpyDeviceToHost, st),
-                               "gram s copy"))
-            return rc;
-    if (int rc = hip_check(hipStreamSynchronize(st), "gram s sync")) return rc;
-    std::vector<int64_t> hp(tot_nv);
-    for (int i = 0; i < n_items; ++i) {
-        int64_t *p = hp.data() + poff[i];
-        const double *sv = hs.data() + poff[i];
-        std::iota(p, p + all[i].nv, 0);
-        std::stable_sort(p, p + all[i].nv, [&](int64_t a, int64_t b) { return sv[a] > sv[b]; });
-    }
-    int64_t *dperm = (int64_t *)(work + tot_w);
-    int64_t *dpoff = dperm + tot_nv;
-    int64_t *dtoff = dpoff + n_items;
-    if (int rc = upload(hp.data(), dperm, tot_nv * sizeof(int64_t), "gram perm")) return rc;
-    if (int rc = upload(poff.data(), dpoff, n_items * sizeof(int64_t), "gram poff")) return rc;
-    if (int rc = upload(toff.data(), dtoff, n_items * sizeof(int64_t), "gram toff")) return rc;
+    hipLaunchKernelGGL(tj_rank_kernel, dim3(n_items), dim3(256), 0, st, items, dpoff, dperm);
+    SYN_LAUNCH_CHECK("gram order");
     // grid.y spreads each problem's rows over 64 work-groups
     hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items, 64), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 0);
     hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items, 64), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 1);
     SYN_LAUNCH_CHECK("gram permute");
-    if (prof_wants(kTagGramUpdate)) {
-        unsigned long long hb = 0;
-        if (int rc = hip_check(hipMemcpyAsync(&hb, dbytes, sizeof(hb), hipMemcpyDeviceToHost, st), "gram bytes"))
+    if (kbytes) {
+        std::vector<unsigned long long> hb(n_items);
+        if (int rc = hip_check(hipMemcpyAsync(hb.data(), dbytes, n_items * sizeof(unsigned long long),
+                                              hipMemcpyDeviceToHost, st),
+                               "gram bytes"))
             return rc;
         if (int rc = hip_check(hipStreamSynchronize(st), "gram bytes sync")) return rc;
-        prof_add_work(kTagGramUpdate, (double)hb);
+        double tot_b = 0.0;
+        for (unsigned long long b : hb) tot_b += (double)b;
+        prof_add_work(kTagGramBytes, tot_b);
     }
-    (void)TP;
     return hip_check(hipStreamSynchronize(st), "gram final sync");
 }
 
@@ -969,7 +995,7 @@ extern "C" int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_it
     int64_t tot = 0;
     for (int i = 0; i < n_items; ++i)
         tot += syn::tj_work<16>(nv[i]) + 2 * nv[i] + (syn::tj_owner_count<16>(nv[i]) + 1) / 2;
-    return tot + 3 * (int64_t)n_items + 16;
+    return tot + 4 * (int64_t)n_items + 16;
 }
 
 extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_item *items_host, int n_items,

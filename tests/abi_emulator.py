@@ -502,7 +502,7 @@ def _gram_workspace(nv):
         half = (nblk + (nblk & 1)) // 2
         owners = half + half * half + half * ((int(v) + 63) // 64)
         tot += max(work) + 2 * int(v) + (owners + 1) // 2
-    return tot + 3 * len(nv) + 16
+    return tot + 4 * len(nv) + 16
 
 
 class _FakeLib:
