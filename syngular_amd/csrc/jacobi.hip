@@ -200,10 +200,14 @@ __global__ __launch_bounds__(kJacThreads) void jacobi_kernel(const syn_jacobi_it
 // sqrt(max(lambda_i, 0)) are the singular values of C when G = C^T C, and the
 // rows of rt are the matching eigenvectors.
 // ---------------------------------------------------------------------------
+// threads per problem by the largest order of the launch: one wave up to
+// 16 (at most 8 planes a round), two up to 32, four up to 64, so small
+// problems do not wait at four-wave barriers and more of them share a CU
 constexpr int kGramThreads = 256;
 constexpr int kGramMax = 64;
 
-__global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jacobi_item *__restrict__ items,
+template <int NT>
+__global__ __launch_bounds__(NT) void jacobi_gram_kernel(const syn_jacobi_item *__restrict__ items,
                                                                    int nv_max, double tol, int max_sweeps) {
     extern __shared__ double lds[];
     __shared__ double cs[kGramMax / 2][2];
@@ -214,13 +218,13 @@ __global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jac
     const int tid = threadIdx.x;
     if (P.nv < 1 || P.nv > nv_max || P.len != P.nv) {
         // not a problem this kernel takes: flag it in s and leave the rest
-        for (int64_t i = tid; i < P.nv; i += kGramThreads) P.s[i] = __builtin_nan("");
+        for (int64_t i = tid; i < P.nv; i += NT) P.s[i] = __builtin_nan("");
         return;
     }
     const int g = (int)P.nv, ld = g + 1;
     double *A = lds;
     double *R = lds + g * ld;
-    for (int e = tid; e < g * g; e += kGramThreads) {
+    for (int e = tid; e < g * g; e += NT) {
         const int i = e / g, j = e - i * g;
         A[i * ld + j] = P.x[i * P.ldx + j];
         R[i * ld + j] = (i == j) ? 1.0 : 0.0;
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jac
                 qq[tid] = q;
             }
             __syncthreads();
-            for (int e = tid; e < half * half; e += kGramThreads) {
+            for (int e = tid; e < half * half; e += NT) {
                 const int ka = e / half, kb = e - ka * half;
                 const double ca = cs[ka][0], sa = cs[ka][1], cb = cs[kb][0], sb = cs[kb][1];
                 if (sa == 0.0 && sb == 0.0) continue;
@@ -292,7 +296,7 @@ __global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jac
                     A[q * ld + p] = 0.0;
                 }
             }
-            for (int e = tid; e < half * g; e += kGramThreads) {
+            for (int e = tid; e < half * g; e += NT) {
                 const int k = e / g, i = e - k * g;
                 const double sn = cs[k][1];
                 if (sn != 0.0) {
@@ -311,7 +315,7 @@ __global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jac
     }
     __syncthreads();
     // decreasing order by eigenvalue (ties by index): rank of each row
-    for (int i = tid; i < g; i += kGramThreads) {
+    for (int i = tid; i < g; i += NT) {
         const double li = A[i * ld + i];
         int rk = 0;
         for (int j = 0; j < g; ++j) {
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(kGramThreads) void jacobi_gram_kernel(const syn_jac
         P.s[rk] = sqrt(fmax(li, 0.0));
     }
     __syncthreads();
-    for (int e = tid; e < g * g; e += kGramThreads) {
+    for (int e = tid; e < g * g; e += NT) {
         const int i = e / g, j = e - i * g;
         P.rt[(int64_t)rank[i] * P.ldr + j] = R[i * ld + j];
     }
@@ -340,14 +344,20 @@ extern "C" int syn_jacobi_gram_f64(const syn_jacobi_item *items, int n_items, in
     }
     if (n_items == 0) return SYN_OK;
     static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)jacobi_gram_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(2 * kGramMax * (kGramMax + 1) * sizeof(double)));
+    if (!attr_set) {   // orders above 32 take more than 64 KiB of LDS
+        if (int rc = hip_check(hipFuncSetAttribute((const void *)jacobi_gram_kernel<kGramThreads>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)(2 * kGramMax * (kGramMax + 1) * sizeof(double))),
+                               "syn_jacobi_gram_f64: hipFuncSetAttribute"))
+            return rc;
         attr_set = true;
     }
     const size_t shm = 2 * (size_t)nv_max * (size_t)(nv_max + 1) * sizeof(double);
     ProfScope ps(kTagJacobi, 0.0, (hipStream_t)stream);
-    hipLaunchKernelGGL(jacobi_gram_kernel, dim3(n_items), dim3(kGramThreads), shm, (hipStream_t)stream, items,
+    auto kern = nv_max <= 16 ? jacobi_gram_kernel<64>
+                             : (nv_max <= 32 ? jacobi_gram_kernel<128> : jacobi_gram_kernel<kGramThreads>);
+    const int nt = nv_max <= 16 ? 64 : (nv_max <= 32 ? 128 : kGramThreads);
+    hipLaunchKernelGGL(kern, dim3(n_items), dim3(nt), shm, (hipStream_t)stream, items,
                        (int)nv_max, tol, max_sweeps);
     SYN_LAUNCH_CHECK("syn_jacobi_gram_f64 launch");
     return SYN_OK;
