@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="time box of each host oracle baseline")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-census", action="store_true",
+                    help="time every kernel of the served stream with the library's events (slows it; for the "
+                         "served roofline) and report the census in the served form")
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="fp64",
                     help="arithmetic of the served form (the per-chain comparison is always plain FP64); "
                          "FP32 products add ~1e-7 per layer, which long circuits accumulate past 1e-6")
@@ -485,20 +488,25 @@ def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d
     return roof, pipeline
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cholqr_8192x4096_gemm_traffic.json")
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
+                 ("cholqr_8192x4096_gemm_traffic.json", "batch4096_jacobi_step_traffic.json")]
 
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a known-size copy), or None
-    when no counter run covers it.  The counters cannot be read from inside
-    this process; profiles/cholqr_8192x4096_gemm_traffic.md has the commands."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            rec = json.load(f)
-    except (OSError, ValueError):
-        return None
-    return rec.get("bytes_per_launch", {}).get(kernel)
+    summaries (FETCH_SIZE and WRITE_SIZE scaled by a known-size copy), or
+    None when no counter run covers it.  The counters cannot be read from
+    inside this process; the .md beside each summary has its commands."""
+    for path in TRAFFIC_FILES:
+        try:
+            with open(path) as f:
+                rec = json.load(f)
+        except (OSError, ValueError):
+            continue
+        got = rec.get("bytes_per_launch", {}).get(kernel)
+        if got is not None:
+            return got
+    return None
 
 
 # kernels whose profiler work figure is bytes moved (the rest: flops)
@@ -714,8 +722,11 @@ def serve_form(np, torch, args):
         W.serving_jobs(2, 1e6, seed=99, n_range=(16, 20), layer_range=(2, 2), chis=(64,)))
     torch.cuda.synchronize()
     log(f"serve: {len(jobs)} jobs at {args.serve_rate}/s into the server")
+    if args.serve_census:
+        ops.profile_begin()
     rep = srv.run(jobs)
     torch.cuda.synchronize()
+    census = ops.profile_end() if args.serve_census else None
     log(f"serve: {rep.site_layers} site-layers served in {rep.makespan:.1f} s; per-chain sample next")
     lat = rep.latency_percentiles()
     pyth = max(abs(r.norm2 + r.discarded - 1.0) for r in rep.results)
@@ -804,6 +815,7 @@ def serve_form(np, torch, args):
             "norm2_rel_err_max": max(e[0] for e in errs),
             "overlap_rel_err_max": max(e[1] for e in errs),
             "roofline": served_roofline(),
+            "census": None if census is None else {k: list(v) for k, v in census.items()},
             "cpu_baseline": cpu}
 
 

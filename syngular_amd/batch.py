@@ -34,12 +34,19 @@ from ._lib import call
 
 F64 = torch.float64
 ALIGN = 32                      # doubles (256 B) between sub-allocations
-LDS_JACOBI_MAX = 64             # Gram matrices up to this size: one work-group in LDS
+# Gram matrices up to this size: one work-group (one wave) in LDS; larger
+# ones take the block Jacobi.  Past 16 the LDS kernel's rounds are bound by
+# LDS bandwidth (every round rewrites the whole matrix and its rotations),
+# while the block method confines the rounds to 32 x 32 pair tiles and
+# updates the rest on the MFMA: orders 17..64 there took the batch of 4096
+# chains from 1255 to 1135 ms per step.
+LDS_JACOBI_MAX = 16
 CHOLQR_MIN_COLS = 96            # tall unfoldings with this many columns (or too big for LDS): CholeskyQR2
 # Jacobi orthogonality thresholds, fixed per path (not per launch) so that a
 # chain's rotations do not depend on the other chains of its batch: the
-# rounding bound length * eps of the largest problem each path takes.
-TOL_LDS = ops.jacobi_tol(LDS_JACOBI_MAX)
+# rounding bound length * eps of the largest problem a path may take (the
+# LDS kernel takes up to 64).
+TOL_LDS = ops.jacobi_tol(64)
 TOL_BLOCKED = ops.jacobi_tol(2048)
 
 

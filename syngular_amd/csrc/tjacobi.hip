@@ -615,6 +615,8 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
                 x[ti][j] = ok ? val : 0.0;
             }
         }
+        // bytes moved for this tile: one tile read, one or two written
+        if (bytes && tj == 0 && lane == 0) atomicAdd(bytes + owner_t[ub], (a == b ? 2ull : 3ull) * kTileBytes);
         if (!fa && !fb) {
             // nothing rotated: wave tj copies rows 16 tj .. 16 tj + 15 of X
             // (and their transpose)
@@ -632,7 +634,6 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
             }
             return;
         }
-        if (bytes && tj == 0 && lane == 0) atomicAdd(bytes + owner_t[ub], (a == b ? 2ull : 3ull) * kTileBytes);
         const double *wa = wset + a * TP * TP, *wb = wset + b * TP * TP;
         // U = X Wb(:, tj): two 16 x 16 tiles (rows 0-15, 16-31)
         double vb[8];
@@ -904,23 +905,34 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     for (int i = 0; i < n_items; ++i)
         if (all[i].active) idx.push_back(i);
     int used = 0;
+    // the table of a sweep's active problems and its owner table are built
+    // again only when the active set changed (a sweep that ends with every
+    // problem still going leaves both as they are)
+    std::vector<int> built;
+    int64_t ss = 0, ts = 0, max_np = 2;
+    int n_act = 0;
     for (; used < max_sweeps && !idx.empty(); ++used) {
-        std::vector<syn_gbj_item> act;
-        int64_t ss = 0, ts = 0, max_np = 2;
-        for (int i : idx) {
-            syn_gbj_item it = all[i];
-            const TjDims D = tj_dims<TB>(it.nv);
-            it.solve_start = ss;
-            it.tile_start = ts;
-            ss += D.half;
-            ts += tj_ugroups(D);
-            if (D.np > max_np) max_np = D.np;
-            act.push_back(it);
+        if (idx != built) {
+            std::vector<syn_gbj_item> act;
+            ss = ts = 0;
+            max_np = 2;
+            for (int i : idx) {
+                syn_gbj_item it = all[i];
+                const TjDims D = tj_dims<TB>(it.nv);
+                it.solve_start = ss;
+                it.tile_start = ts;
+                ss += D.half;
+                ts += tj_ugroups(D);
+                if (D.np > max_np) max_np = D.np;
+                act.push_back(it);
+            }
+            if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
+            n_act = (int)act.size();
+            hipLaunchKernelGGL(tj_owner_kernel<TB>, dim3((unsigned)n_act), dim3(256), 0, st, items, ss, owner);
+            SYN_LAUNCH_CHECK("gram owner table");
+            built = idx;
         }
-        if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, counters, n_items);
-        hipLaunchKernelGGL(tj_owner_kernel<TB>, dim3((unsigned)act.size()), dim3(256), 0, st, items, ss, owner);
-        SYN_LAUNCH_CHECK("gram owner table");
         for (int64_t step = 0; step < max_np; ++step) {
             // solves of step (all but the last launch) beside the updates of
             // step - 1 (all but the first)
@@ -937,7 +949,7 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
         // finds an entry that would still rotate: this saves the last sweep,
         // which only confirmed convergence, with the same result
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, chk, n_items);
-        hipLaunchKernelGGL(tj_check_kernel<TB>, dim3((unsigned)act.size(), 64), dim3(256), 0, st, items,
+        hipLaunchKernelGGL(tj_check_kernel<TB>, dim3((unsigned)n_act, 64), dim3(256), 0, st, items,
                            cur, tol, counters, chk);
         SYN_LAUNCH_CHECK("gram sweep test");
         if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,

@@ -49,9 +49,10 @@ class RoundingReport:
 # MFMA GEMM work); smaller ones the one-work-group Householder kernel.
 CHOLQR_MIN_COLS = 96
 # Right-sweep Gram matrices up to this size are diagonalised in LDS by one
-# work-group (two-sided Jacobi); larger ones by the block one-sided Jacobi
-# spread over many work-groups.
-LDS_JACOBI_MAX = 64
+# work-group (two-sided Jacobi); larger ones by the two-sided block Jacobi
+# spread over many work-groups (past 16 the LDS kernel's rounds are bound by
+# LDS bandwidth and the block method is faster).
+LDS_JACOBI_MAX = 16
 # Orthogonality budget of the left sweep inside TT-rounding (plain FP64).  A
 # left factor with Q^T Q = I + E scales the bond's singular values by
 # (I + E)^(1/2): relative change at most ||E||_2 / 2 <= ||E||_F / 2, and the
