@@ -510,11 +510,6 @@ int syn_prof_begin(void);
 int syn_prof_select(uint32_t mask);
 int syn_prof_note_flops(double flops);
 int syn_prof_end(int64_t *launches, double *ms, double *flops, int ntags);
-/* Tuning knob of syn_jacobi_blocked_f64: rows per Jacobi block (8 or 16;  */
-/* anything else restores the size-based default).                       */
-void syn_tune_jacobi_block_rows(int rows);
-/* Inner Jacobi sweeps per block pair (1..8; anything else restores 1).    */
-void syn_tune_jacobi_inner_sweeps(int sweeps);
 
 #ifdef __cplusplus
 }

@@ -127,8 +127,6 @@ SIGNATURES = {
     "syn_prof_note_flops": (_I32, [_D]),
     "syn_prof_end": (_I32, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_double), _I32]),
-    "syn_tune_jacobi_block_rows": (None, [_I32]),
-    "syn_tune_jacobi_inner_sweeps": (None, [_I32]),
 }
 
 PROF_NTAGS = 20

@@ -346,17 +346,13 @@ extern "C" int64_t syn_jacobi_blocked_workspace(int64_t nv, int64_t len) {
 }
 
 // Rows per block: 16 (four waves per pair) from 384 rows up, else 8 (one wave).
-static int g_block_rows_override = 0;
-// Inner Jacobi sweeps on the 2KB x 2KB pair Gram (the outer sweep repeats).
-static int g_inner_sweeps = 1;
-
-extern "C" void syn_tune_jacobi_block_rows(int rows) { g_block_rows_override = (rows == 8 || rows == 16) ? rows : 0; }
-extern "C" void syn_tune_jacobi_inner_sweeps(int sweeps) { g_inner_sweeps = (sweeps >= 1 && sweeps <= 8) ? sweeps : 1; }
+// One inner Jacobi sweep on the 2KB x 2KB pair Gram (the outer sweep repeats).
+static constexpr int kInnerSweeps = 1;
 
 extern "C" int syn_jacobi_blocked_f64(double *X, int64_t nv, int64_t len, int64_t ldx, double *rt, int64_t ldr,
                                       double *s, double *work, double tol, int max_sweeps, int gram, int *sweeps,
                                       void *stream) {
-    const int block_rows = g_block_rows_override ? g_block_rows_override : (nv >= 384 ? 16 : 8);
+    const int block_rows = nv >= 384 ? 16 : 8;
     using namespace syn;
     if (nv <= 0 || len <= 0 || ldx < len || ldr < nv || !X || !rt || !s || !work || max_sweeps < 0 ||
         !(tol >= 0.0)) {
@@ -384,10 +380,10 @@ extern "C" int syn_jacobi_blocked_f64(double *X, int64_t nv, int64_t len, int64_
                          (double)(np / 2) * 2.0 * (2.0 * kb) * (2.0 * kb) * (double)(2 * len + nv), st);
             if (kb == 16)
                 hipLaunchKernelGGL((bjacobi_step_kernel<16, 256>), dim3((unsigned)(np / 2)), dim3(256), 0, st, X, nv,
-                                   len, ldx, rt, ldr, nblk, step, tol, counter, g_inner_sweeps);
+                                   len, ldx, rt, ldr, nblk, step, tol, counter, kInnerSweeps);
             else
                 hipLaunchKernelGGL((bjacobi_step_kernel<8, 64>), dim3((unsigned)(np / 2)), dim3(64), 0, st, X, nv,
-                                   len, ldx, rt, ldr, nblk, step, tol, counter, g_inner_sweeps);
+                                   len, ldx, rt, ldr, nblk, step, tol, counter, kInnerSweeps);
         }
         SYN_LAUNCH_CHECK("bjacobi_step");
         int host_count = 0;
@@ -488,7 +484,7 @@ extern "C" int syn_jacobi_blocked_many_f64(syn_bjac_item *items, const syn_bjac_
         for (int64_t step = 0; step < max_np - 1; ++step) {
             ProfScope ps(kTagBJacobiStep, 0.0, st);
             hipLaunchKernelGGL(bjacobi_many_step_kernel, dim3((unsigned)start), dim3(256), 0, st, items,
-                               (int)act.size(), step, tol, g_inner_sweeps);
+                               (int)act.size(), step, tol, kInnerSweeps);
         }
         SYN_LAUNCH_CHECK("bjac many step");
         if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,

@@ -845,7 +845,6 @@ static int64_t tj_work(int64_t nv) {
 template <int TB>
 static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
                         int32_t *counters, double tol, int max_sweeps, int *sweeps, hipStream_t st) {
-    constexpr int TP = 2 * TB;
     std::vector<int64_t> woff(n_items);
     int64_t tot_w = 0;
     for (int i = 0; i < n_items; ++i) {
