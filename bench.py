@@ -237,7 +237,7 @@ def batch_headline(np, torch, args):
     value = nsites * args.steps / elapsed
     ms = 1e3 * elapsed / args.steps
     log(f"batch mixed: {args.steps} steps in {elapsed:.2f} s, {ms:.0f} ms/step, {value:.0f} sites/s")
-    roof = kernel_roofline(census, timed, dom, args.steps)
+    roof = kernel_roofline(census, timed, dom, args.steps, BATCH_TRAFFIC)
     # the same step with its apply and left sweep (no host reads in the
     # mixed form) replayed from one captured HIP graph, the right sweep eager
     graph = None
@@ -430,7 +430,7 @@ def _peak(kernel):
     return PEAKS["fp32_mfma_tflops"] if "mixed" in kernel else PEAKS["fp64_mfma_tflops"]
 
 
-def kernel_roofline(census, timed, dom, steps):
+def kernel_roofline(census, timed, dom, steps, traffic_file):
     """Roofline entry of the dominant work kernel: algorithmic work (flops
     for the GEMMs, bytes for the memory-bound kernels) over its event time in
     the timed steps, where only its launches carry events; plus its share of
@@ -448,7 +448,7 @@ def kernel_roofline(census, timed, dom, steps):
         achieved = dwork / (dms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": _peak(dom), "unit": "TFLOP/s"}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
-    roof["traffic"] = measured_traffic(dom)
+    roof["traffic"] = measured_traffic(dom, traffic_file)
     roof["kernel"] = dom
     roof["launches_per_step"] = dl / steps
     roof["avg_launch_us"] = round(1e3 * dms / dl, 2)
@@ -480,7 +480,7 @@ def census_table(census):
 def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d, rep):
     """The chain's roofline entry (kernel_roofline) and its pipeline: the
     textbook flops of the step and the census step's per-kernel table."""
-    roof = kernel_roofline(census, timed, dom, steps)
+    roof = kernel_roofline(census, timed, dom, steps, CHAIN_TRAFFIC)
     textbook = W.apply_round_flops(*W.chain_shapes(n, chi, dw), d, list(rep.kept))
     pipeline = {"textbook_flops_per_step": textbook,
                 "textbook_tflops": round(textbook / (ms_per_step * 1e-3) / 1e12, 3),
@@ -488,25 +488,24 @@ def roofline(census, timed, dom, steps, ms_per_step, census_ms, W, n, chi, dw, d
     return roof, pipeline
 
 
-TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
-                 ("cholqr_8192x4096_gemm_traffic.json", "batch4096_jacobi_step_traffic.json")]
+# committed rocprofv3 --pmc summaries, per workload (a kernel's bytes per
+# launch depend on the problems its launches carry)
+BATCH_TRAFFIC = os.path.join(ROOT, "profiles", "batch4096_jacobi_step_traffic.json")
+CHAIN_TRAFFIC = os.path.join(ROOT, "profiles", "cholqr_8192x4096_gemm_traffic.json")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summaries (FETCH_SIZE and WRITE_SIZE scaled by a known-size copy), or
-    None when no counter run covers it.  The counters cannot be read from
-    inside this process; the .md beside each summary has its commands."""
-    for path in TRAFFIC_FILES:
-        try:
-            with open(path) as f:
-                rec = json.load(f)
-        except (OSError, ValueError):
-            continue
-        got = rec.get("bytes_per_launch", {}).get(kernel)
-        if got is not None:
-            return got
-    return None
+def measured_traffic(kernel, path):
+    """HBM bytes per launch of `kernel` in this workload from its committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a known-size
+    copy), or None when no counter run covers it.  The counters cannot be
+    read from inside this process; the .md beside the summary has the
+    commands."""
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return rec.get("bytes_per_launch", {}).get(kernel)
 
 
 # kernels whose profiler work figure is bytes moved (the rest: flops)

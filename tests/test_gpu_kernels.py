@@ -119,6 +119,34 @@ def test_householder_qr_matches_lapack(m, n, keep):
     np.testing.assert_allclose(Qt.cpu().numpy(), Qn[:, :kq].T, atol=1e-12)
 
 
+def test_householder_qr_sized_launch_classes():
+    """The grouped QR launches each LDS class (16, 48, 96 KB, and the
+    workspace class beyond) with the LDS its largest matrix needs: a batch
+    with matrices in every class gives LAPACK's Q for each."""
+    from syngular_amd import batch as B
+
+    shapes = [(40, 8), (64, 32), (120, 40), (200, 60), (150, 80), (400, 48), (1100, 64), (30, 29)]
+    case(f"qr sized classes {shapes}")
+    need = [m * n for m, n in shapes]
+    classes = set(int(np.searchsorted(B.QR_LDS_CLASSES, v)) for v in need)
+    assert classes == {0, 1, 2, 3}
+    rng = np.random.default_rng(5)
+    dev = torch.device("cuda")
+    As = [rng.standard_normal(sh) for sh in shapes]
+    A = [torch.tensor(x, device=dev) for x in As]
+    Q = [torch.empty(sh, dtype=torch.float64, device=dev) for sh in shapes]
+    work = [torch.empty(m * n, dtype=torch.float64, device=dev) for m, n in shapes]
+    tau = [torch.empty(n, dtype=torch.float64, device=dev) for _, n in shapes]
+    ptr = lambda ts: np.array([t.data_ptr() for t in ts], dtype=np.uint64)
+    m = np.array([sh[0] for sh in shapes], np.int64)
+    n = np.array([sh[1] for sh in shapes], np.int64)
+    B.qr_table(ptr(A), m, n, ptr(Q), ptr(work), ptr(tau))
+    torch.cuda.synchronize()
+    for q, x in enumerate(As):
+        Qn = np.linalg.qr(x, mode="reduced")[0]
+        np.testing.assert_allclose(Q[q].cpu().numpy(), Qn, atol=1e-12, err_msg=str(shapes[q]))
+
+
 def test_householder_qr_strided_and_rank_deficient():
     from syngular_amd import ops
 
