@@ -841,6 +841,15 @@ class CapturedApplyLeft:
             out.compact()
         return out
 
+    def close(self):
+        """Release the graph, its memory pool and the staging pool now
+        (an instantiated graph keeps device resources until it is reset)."""
+        torch.cuda.synchronize()
+        self.graph.reset()
+        self.out = None
+        self.staging = None
+        self.inputs = None
+
 
 class OperatorBatch:
     """J MPO chains (cores (D_k, din_k, dout_k, D_k+1)) on the device."""
