@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "prof.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace syn {
@@ -158,8 +159,11 @@ __host__ __device__ __forceinline__ TjDims tj_dims(int64_t nv) {
 // a <= b, two per work-group (each by two waves, one per 16-column half),
 // then one work-group per rt slab of kTCol columns (one wave per 16 columns).
 __host__ __device__ __forceinline__ int64_t tj_gtiles(int64_t half) { return half * (half + 1) / 2; }
+// update work-groups of a problem at a step: 16-row blocks put two 32 x 32
+// G tiles in a work-group, 32-row blocks one 64 x 64 tile; one per rt slab
+template <int TB>
 __host__ __device__ __forceinline__ int64_t tj_ugroups(const TjDims &D) {
-    return (tj_gtiles(D.half) + 1) / 2 + D.half * D.ncs;
+    return (TB == 16 ? (tj_gtiles(D.half) + 1) / 2 : tj_gtiles(D.half)) + D.half * D.ncs;
 }
 
 // Work-group owner table of a sweep's fused steps: entry b < ss names the
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(256) void tj_owner_kernel(const syn_gbj_item *__res
     const syn_gbj_item &P = items[i];
     const TjDims D = tj_dims<TB>(P.nv);
     for (int64_t b = threadIdx.x; b < D.half; b += 256) owner[P.solve_start + b] = i;
-    const int64_t nt = tj_ugroups(D);
+    const int64_t nt = tj_ugroups<TB>(D);
     for (int64_t t = threadIdx.x; t < nt; t += 256) owner[ss + P.tile_start + t] = i;
 }
 
@@ -180,7 +184,7 @@ __global__ __launch_bounds__(256) void tj_owner_kernel(const syn_gbj_item *__res
 template <int TB>
 static int64_t tj_owner_count(int64_t nv) {
     const TjDims D = tj_dims<TB>(nv);
-    return D.half + tj_ugroups(D);
+    return D.half + tj_ugroups<TB>(D);
 }
 
 // Per-problem scratch layout (doubles): two sets of W tiles (half * TP^2
@@ -702,6 +706,152 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
     }
 }
 
+// Wide blocks (32 rows, 64 x 64 pair tiles), for launches of many problems:
+// a sweep takes half the steps of 16-row blocks, so the Gram matrix and rt
+// stream through HBM half as often for the same MFMA work.  Solves and
+// updates are launches of their own, one after the other: the solve reads
+// its tile straight from G (as the previous step's updates left it) and
+// rotates it in LDS with 1024 threads; the update applies the step's
+// rotations to G and rt in place, in registers.  W tiles and flags use the
+// first parity set only.
+template <int TB>
+__global__ __launch_bounds__(TB * TB) void tj_solve_kernel(const syn_gbj_item *__restrict__ items,
+                                                           const int32_t *__restrict__ owner_s, int64_t step,
+                                                           int64_t solve_pad, int64_t solve_count, double tol) {
+    constexpr int TP = 2 * TB, NT = TB * TB;
+    __shared__ double S[TP][TP + 1];
+    __shared__ double W[TP][TP + 1];
+    __shared__ v2d cs[TP / 2];
+    __shared__ int rows[TP];
+    __shared__ int do_rot, any_rot;
+    const int64_t sb = tj_xcd_order(blockIdx.x, solve_pad);
+    if (sb >= solve_count) return;
+    const syn_gbj_item &P = items[owner_s[sb]];
+    const TjDims D = tj_dims<TB>(P.nv);
+    const int64_t pw = sb - P.solve_start;
+    const int64_t st = step % (D.np - 1);
+    const int tid = threadIdx.x;
+    if (tid < TP) rows[tid] = tj_row<TB>(st, pw, D.np, D.nblk, P.nv, tid);
+    __syncthreads();
+    for (int e = tid; e < TP * TP; e += NT) {
+        const int i = e / TP, k = e % TP, ri = rows[i], rk = rows[k];
+        S[i][k] = (ri >= 0 && rk >= 0) ? P.g[(int64_t)ri * P.ldg + rk] : 0.0;
+    }
+    // the first step of a pass sweeps the whole tile, the others the planes
+    // between its two blocks
+    tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, 1, tj_wset<TB>(P, D.half, 0) + pw * TP * TP,
+                      tj_flags<TB>(P, D.half, 0), st != 0);
+}
+
+// Updates of a step with 32-row blocks, in place: a work-group takes one
+// 64 x 64 G tile (a, b), a <= b (wave w: output columns 16 w .. 16 w + 15;
+// every wave reads the whole tile, so the writes wait for a barrier), or one
+// rt slab of 64 rows x 64 columns (wave w: columns 16 w .. 16 w + 15, which
+// it alone reads and writes).  Tiles of unrotated pairs stay as they are.
+__global__ __launch_bounds__(256) void tjw_update_kernel(const syn_gbj_item *__restrict__ items,
+                                                         const int32_t *__restrict__ owner_t, int64_t step,
+                                                         int64_t tile_pad, int64_t tile_count,
+                                                         unsigned long long *bytes) {
+    constexpr int TB = 32, TP = 64;
+    constexpr unsigned long long kTileBytes = (unsigned long long)TP * TP * sizeof(double);
+    constexpr unsigned long long kSlabBytes = 2ull * TP * kTCol * sizeof(double);
+    const int64_t ub = tj_xcd_order(blockIdx.x, tile_pad);
+    if (ub >= tile_count) return;
+    const syn_gbj_item &P = items[owner_t[ub]];
+    const TjDims D = tj_dims<TB>(P.nv);
+    const int64_t st = step % (D.np - 1);
+    const int64_t t = ub - P.tile_start, ng = tj_gtiles(D.half);
+    const int *flags = tj_flags<TB>(P, D.half, 0);
+    const double *wset = tj_wset<TB>(P, D.half, 0);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    if (t < ng) {
+        int64_t a = 0, rem = t;
+        while (rem >= D.half - a) {
+            rem -= D.half - a;
+            ++a;
+        }
+        const int64_t b = a + rem;
+        const int fa = flags[a], fb = flags[b];
+        if (!fa && !fb) return;
+        if (bytes && tid == 0) atomicAdd(bytes + owner_t[ub], (a == b ? 2ull : 3ull) * kTileBytes);
+        double *G = P.g;
+        const int64_t ld = P.ldg;
+        const double *wa = wset + a * TP * TP, *wb = wset + b * TP * TP;
+        const int tj = wv;
+        int ra[4];
+#pragma unroll
+        for (int rho = 0; rho < 4; ++rho) ra[rho] = tj_row<TB>(st, a, D.np, D.nblk, P.nv, 16 * rho + li);
+        // U = X Wb(:, 16 tj ..): X(i, k) read as G(row of k, row of i)
+        v4d u[4];
+#pragma unroll
+        for (int rho = 0; rho < 4; ++rho) u[rho] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int j = 0; j < 16; ++j) {
+            const int e = 4 * j + lk, c = 16 * tj + li;
+            const int cr = tj_row<TB>(st, b, D.np, D.nblk, P.nv, e);
+            const double vb = fb ? wb[e * TP + c] : (e == c ? 1.0 : 0.0);
+#pragma unroll
+            for (int rho = 0; rho < 4; ++rho) {
+                const bool ok = ra[rho] >= 0 && cr >= 0;
+                const double xv = G[ok ? (int64_t)cr * ld + ra[rho] : 0];
+                u[rho] = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? xv : 0.0, vb, u[rho], 0, 0, 0);
+            }
+        }
+        // X_new(16 ti .., 16 tj ..) = Wa(:, 16 ti ..)^T U
+        v4d acc[4];
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) {
+            acc[ti] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int e = 4 * j + lk, c = 16 * ti + li;
+                const double av = fa ? wa[e * TP + c] : (e == c ? 1.0 : 0.0);
+                acc[ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, u[j >> 2][j & 3], acc[ti], 0, 0, 0);
+            }
+        }
+        __syncthreads();   // every wave has read the tile
+        const int gc = tj_row<TB>(st, b, D.np, D.nblk, P.nv, 16 * tj + li);
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = tj_row<TB>(st, a, D.np, D.nblk, P.nv, 16 * ti + lk + 4 * r);
+                if (gr >= 0 && gc >= 0) {
+                    G[(int64_t)gr * ld + gc] = acc[ti][r];
+                    if (a != b) G[(int64_t)gc * ld + gr] = acc[ti][r];
+                }
+            }
+        return;
+    }
+    const int64_t u2 = t - ng;
+    const int64_t a = u2 / D.ncs, c0 = (u2 % D.ncs) * kTCol;
+    if (!flags[a]) return;
+    if (bytes && tid == 0) atomicAdd(bytes + owner_t[ub], kSlabBytes);
+    const double *wa = wset + a * TP * TP;
+    const int64_t col = c0 + 16 * wv + li;
+    const bool cok = col < P.nv;
+    double xs[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int r = tj_row<TB>(st, a, D.np, D.nblk, P.nv, 4 * j + lk);
+        const bool ok = r >= 0 && cok;
+        const double val = P.rt[ok ? (int64_t)r * P.ldr + col : 0];
+        xs[j] = ok ? val : 0.0;
+    }
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) {
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[(4 * j + lk) * TP + 16 * ti + li], xs[j], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gr = tj_row<TB>(st, a, D.np, D.nblk, P.nv, 16 * ti + lk + 4 * r);
+            if (gr >= 0 && cok) P.rt[(int64_t)gr * P.ldr + col] = acc[r];
+        }
+    }
+}
+
 // Copy the Gram matrix back from the workspace buffer (ld nv) into P.g for
 // the problems whose last fused step left it there (which[i] != 0).
 template <int TB>
@@ -842,6 +992,12 @@ static int64_t tj_work(int64_t nv) {
     return tj_g2_off<TB>(half, nv) + nv * nv;
 }
 
+// the wide (32-row) blocks take the problems of order kWideMinOrder and
+// above when together they hold at least kWideMinEntries Gram entries (below
+// that a call is latency-bound and the fused 16-row steps finish first)
+constexpr int64_t kWideMinOrder = 192;
+constexpr int64_t kWideMinEntries = int64_t(1) << 23;
+
 template <int TB>
 static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
                         int32_t *counters, double tol, int max_sweeps, int *sweeps, hipStream_t st) {
@@ -921,7 +1077,7 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
                 it.solve_start = ss;
                 it.tile_start = ts;
                 ss += D.half;
-                ts += tj_ugroups(D);
+                ts += tj_ugroups<TB>(D);
                 if (D.np > max_np) max_np = D.np;
                 act.push_back(it);
             }
@@ -932,15 +1088,30 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
             built = idx;
         }
         hipLaunchKernelGGL(tj_zero_kernel, dim3(1), dim3(256), 0, st, counters, n_items);
-        for (int64_t step = 0; step < max_np; ++step) {
-            // solves of step (all but the last launch) beside the updates of
-            // step - 1 (all but the first)
-            const bool hs = step < max_np - 1, hu = step > 0;
-            const int64_t sp = hs ? (ss + 7) / 8 * 8 : 0, tp = hu ? (ts + 7) / 8 * 8 : 0;
-            ProfScope ps(kTagGramStep, 0.0, st);
-            hipLaunchKernelGGL(tjf_step_kernel<TB>, dim3((unsigned)(sp + tp)), dim3(256), 0, st, items, owner,
-                               owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp, ts, cur, tol, kbytes);
-            if (hu) cur ^= 1;
+        if constexpr (TB == 16) {
+            for (int64_t step = 0; step < max_np; ++step) {
+                // solves of step (all but the last launch) beside the
+                // updates of step - 1 (all but the first)
+                const bool hs = step < max_np - 1, hu = step > 0;
+                const int64_t sp = hs ? (ss + 7) / 8 * 8 : 0, tp = hu ? (ts + 7) / 8 * 8 : 0;
+                ProfScope ps(kTagGramStep, 0.0, st);
+                hipLaunchKernelGGL(tjf_step_kernel<TB>, dim3((unsigned)(sp + tp)), dim3(256), 0, st, items, owner,
+                                   owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp, ts, cur, tol, kbytes);
+                if (hu) cur ^= 1;
+            }
+        } else {
+            // wide blocks: the solves of a step, then its updates, in place
+            const int64_t sp = (ss + 7) / 8 * 8, tp = (ts + 7) / 8 * 8;
+            for (int64_t step = 0; step < max_np - 1; ++step) {
+                {
+                    ProfScope ps(kTagGramStep, 0.0, st);
+                    hipLaunchKernelGGL(tj_solve_kernel<TB>, dim3((unsigned)sp), dim3(TB * TB), 0, st, items, owner,
+                                       step, sp, ss, tol);
+                }
+                ProfScope ps(kTagGramStep, 0.0, st);
+                hipLaunchKernelGGL(tjw_update_kernel, dim3((unsigned)tp), dim3(256), 0, st, items, owner + ss, step,
+                                   tp, ts, kbytes);
+            }
         }
         for (int i : idx) gpar[i] = cur;
         SYN_LAUNCH_CHECK("gram step");
@@ -1005,7 +1176,8 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
 extern "C" int64_t syn_jacobi_gram_blocked_workspace(const int64_t *nv, int n_items) {
     int64_t tot = 0;
     for (int i = 0; i < n_items; ++i)
-        tot += syn::tj_work<16>(nv[i]) + 2 * nv[i] + (syn::tj_owner_count<16>(nv[i]) + 1) / 2;
+        tot += std::max(syn::tj_work<16>(nv[i]), syn::tj_work<32>(nv[i])) + 2 * nv[i] +
+               (std::max(syn::tj_owner_count<16>(nv[i]), syn::tj_owner_count<32>(nv[i])) + 1) / 2;
     return tot + 4 * (int64_t)n_items + 16;
 }
 
@@ -1027,5 +1199,31 @@ extern "C" int syn_jacobi_gram_blocked_f64(syn_gbj_item *items, const syn_gbj_it
             return SYN_EINVAL;
         }
     }
-    return gram_blocked<16>(items, all, n_items, work, counters, tol, max_sweeps, sweeps, (hipStream_t)stream);
+    // many large problems: throughput, where the wide blocks halve the bytes
+    // a sweep moves; small or few problems: latency, where the fused 16-row
+    // steps overlap each step's solves with the previous step's updates.  The
+    // two groups run one after the other in the same workspace (each pass
+    // waits for its stream before it returns).
+    std::vector<syn_gbj_item> wide, narrow;
+    int64_t wide_entries = 0;
+    for (const syn_gbj_item &it : all) {
+        const bool w = it.nv >= kWideMinOrder;
+        (w ? wide : narrow).push_back(it);
+        if (w) wide_entries += it.nv * it.nv;
+    }
+    if (wide_entries < kWideMinEntries) {
+        narrow.swap(all);
+        wide.clear();
+    }
+    int sw_w = 0, sw_n = 0;
+    if (!wide.empty())
+        if (int rc = gram_blocked<32>(items, wide, (int)wide.size(), work, counters, tol, max_sweeps, &sw_w,
+                                      (hipStream_t)stream))
+            return rc;
+    if (!narrow.empty())
+        if (int rc = gram_blocked<16>(items, narrow, (int)narrow.size(), work, counters, tol, max_sweeps, &sw_n,
+                                      (hipStream_t)stream))
+            return rc;
+    if (sweeps) *sweeps = std::max(sw_w, sw_n);
+    return SYN_OK;
 }

@@ -493,15 +493,17 @@ def _workspace(nv, ln):
 def _gram_workspace(nv):
     tot = 0
     for v in np.asarray(nv, np.int64):
-        work = []
+        v = int(v)
+        work, owners = [], []
         for tb in (16, 32):
-            nblk = (int(v) + tb - 1) // tb
+            nblk = (v + tb - 1) // tb
             half = (nblk + (nblk & 1)) // 2
-            work.append(half * 4 * tb * tb + (half + 1) // 2 + 1 + int(v) * int(v) + int(v) + 4)
-        nblk = (int(v) + 15) // 16
-        half = (nblk + (nblk & 1)) // 2
-        owners = half + half * half + half * ((int(v) + 63) // 64)
-        tot += max(work) + 2 * int(v) + (owners + 1) // 2
+            # two W-tile sets, rotation flags, floor, sort scratch, second Gram buffer
+            work.append(2 * half * 4 * tb * tb + half + 1 + v * v + v + 4 + v * v)
+            gtiles = half * (half + 1) // 2
+            ugroups = ((gtiles + 1) // 2 if tb == 16 else gtiles) + half * ((v + 63) // 64)
+            owners.append(half + ugroups)
+        tot += max(work) + 2 * v + (max(owners) + 1) // 2
     return tot + 4 * len(nv) + 16
 
 

@@ -119,6 +119,40 @@ def test_householder_qr_matches_lapack(m, n, keep):
     np.testing.assert_allclose(Qt.cpu().numpy(), Qn[:, :kq].T, atol=1e-12)
 
 
+def test_jacobi_gram_blocked_wide_batch():
+    """A call with hundreds of Gram matrices: those of order 192 and above
+    hold more than 2^23 entries together and take the 32-row blocks, the
+    rest the fused 16-row steps after them.  Singular values against numpy,
+    orthonormal eigenvector rows and the reconstruction of G, over orders
+    with partial and odd blocks on both sides of the switch."""
+    from syngular_amd import batch as B
+
+    rng = np.random.default_rng(21)
+    orders = [int(v) for v in rng.integers(17, 520, size=300)]
+    orders[:8] = [17, 32, 33, 64, 191, 192, 193, 519]
+    assert sum(g * g for g in orders if g >= 192) >= 1 << 23
+    case(f"jacobi gram blocked wide, {len(orders)} problems, orders {min(orders)}..{max(orders)}")
+    Cs = [rng.standard_normal((g + 3, g)) for g in orders]
+    Cs[5] = Cs[5] * np.logspace(0, -3, orders[5])
+    Gs = [c.T @ c for c in Cs]
+    dev = torch.device("cuda")
+    G = [torch.tensor(x, device=dev) for x in Gs]
+    R = [torch.empty_like(x) for x in G]
+    S = [torch.empty(x.shape[0], dtype=torch.float64, device=dev) for x in G]
+    ptr = lambda ts: np.array([t.data_ptr() for t in ts], dtype=np.uint64)
+    sweeps = B.jacobi_gram_blocked_table(ptr(G), np.array(orders, np.int64), ptr(R), ptr(S))
+    torch.cuda.synchronize()
+    assert 0 < sweeps < 60
+    for q in range(0, len(orders), 7):
+        g = orders[q]
+        s, r = S[q].cpu().numpy(), R[q].cpu().numpy()
+        ref = np.linalg.svd(Cs[q], compute_uv=False)
+        assert np.all(np.diff(s) <= 0), q
+        np.testing.assert_allclose(s, ref, rtol=1e-9, atol=1e-12 * ref[0], err_msg=str(q))
+        np.testing.assert_allclose(r @ r.T, np.eye(g), atol=1e-12, err_msg=str(q))
+        np.testing.assert_allclose(r.T @ np.diag(s ** 2) @ r, Gs[q], atol=1e-11 * ref[0] ** 2, err_msg=str(q))
+
+
 def test_householder_qr_sized_launch_classes():
     """The grouped QR launches each LDS class (16, 48, 96 KB, and the
     workspace class beyond) with the LDS its largest matrix needs: a batch

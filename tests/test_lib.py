@@ -50,3 +50,20 @@ def test_product_path_does_not_import_the_oracle():
             if f.endswith(".py"):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in re.findall(r"^\s*(?:from|import)\s+(\S+)", src, flags=re.M), f
+
+
+def test_gram_workspace_matches_the_host_emulator():
+    # the blocked Jacobi's workspace size is host arithmetic; the emulator the
+    # host tests run on must ask for the same size at orders on both sides of
+    # the wide-block switch and at its edges
+    import ctypes
+
+    import numpy as np
+
+    from abi_emulator import _gram_workspace
+
+    lib = _lib.load()
+    for nv in ([1], [16], [17], [191, 192, 193], list(range(1, 700, 13)), [4096]):
+        a = np.asarray(nv, np.int64)
+        got = int(lib.syn_jacobi_gram_blocked_workspace(a.ctypes.data_as(ctypes.c_void_p), len(a)))
+        assert got == _gram_workspace(a), nv
