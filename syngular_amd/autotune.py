@@ -25,6 +25,8 @@ shares (its tile), which moves results at the rounding level only.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 TABLE: dict = {}
@@ -37,7 +39,7 @@ MAX_RECORDED = 16384            # problems kept per recorded launch
 
 
 def _clog2(x):
-    return int(np.ceil(np.log2(max(float(x), 1.0))))
+    return math.ceil(math.log2(max(float(x), 1.0)))
 
 
 def launch_class(m, n, k, kind):

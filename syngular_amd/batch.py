@@ -84,6 +84,12 @@ def arena(sizes, dtype=F64):
     return t, (np.uint64(t.data_ptr()) + (off * esz).astype(np.uint64))
 
 
+def _vec(v, dtype, cnt):
+    """v (a scalar or an array of cnt values) as an array of cnt values."""
+    a = np.asarray(v, dtype=dtype)
+    return a if a.ndim else np.full(cnt, a, dtype=dtype)
+
+
 def _blocks(nel):
     return (np.asarray(nel, dtype=np.int64) + _lib.APPLY_BLOCK - 1) // _lib.APPLY_BLOCK
 
@@ -107,28 +113,29 @@ def gemm_table(A, B, C, m, n, k, lda, ldb, ldc, ta=0, tb=0, alpha=1.0, beta=0.0,
     cnt = len(m)
     if cnt == 0:
         return
-    bc = lambda v, dt: np.broadcast_to(np.asarray(v, dtype=dt), (cnt,))
-    n, k = bc(n, np.int64), bc(k, np.int64)
+    n, k = _vec(n, np.int64, cnt), _vec(k, np.int64, cnt)
     live = (m > 0) & (n > 0)
     if not live.all():
         sel = np.nonzero(live)[0]
-        return gemm_table(*(bc(v, np.uint64)[sel] for v in (A, B, C)), m[sel], n[sel], k[sel],
-                          *(bc(v, np.int64)[sel] for v in (lda, ldb, ldc, ta, tb)),
-                          bc(alpha, np.float64)[sel], bc(beta, np.float64)[sel], bc(tri, np.int64)[sel], mixed)
+        return gemm_table(*(_vec(v, np.uint64, cnt)[sel] for v in (A, B, C)), m[sel], n[sel], k[sel],
+                          *(_vec(v, np.int64, cnt)[sel] for v in (lda, ldb, ldc, ta, tb)),
+                          _vec(alpha, np.float64, cnt)[sel], _vec(beta, np.float64, cnt)[sel],
+                          _vec(tri, np.int64, cnt)[sel], mixed)
     # one tile configuration for the launch, from its shape class (autotune)
     tile = autotune.tile(m, n, k, 2 if isinstance(mixed, str) else int(bool(mixed)))
     bm = 128 if tile else 64
     tn = (n + bm - 1) // bm
     tm = (m + bm - 1) // bm
     st, total = _starts(tm * tn)
+    # field assignment broadcasts scalars and casts arrays to the field type
     it = np.zeros(cnt, dtype=_lib.GEMM_ITEM)
-    it["A"], it["B"], it["C"] = bc(A, np.uint64), bc(B, np.uint64), bc(C, np.uint64)
+    it["A"], it["B"], it["C"] = A, B, C
     it["m"], it["n"], it["k"] = m, n, k
-    it["lda"], it["ldb"], it["ldc"] = bc(lda, np.int64), bc(ldb, np.int64), bc(ldc, np.int64)
-    it["alpha"], it["beta"] = bc(alpha, np.float64), bc(beta, np.float64)
-    it["transA"], it["transB"] = bc(ta, np.int32), bc(tb, np.int32)
+    it["lda"], it["ldb"], it["ldc"] = lda, ldb, ldc
+    it["alpha"], it["beta"] = alpha, beta
+    it["transA"], it["transB"] = ta, tb
     it["block_start"], it["tiles_n"] = st, tn
-    it["tri"] = bc(tri, np.int32)
+    it["tri"] = tri
     dev = ops._upload(it)
     if ops._profiling[0]:
         call("syn_prof_note_flops", float((2.0 * m * n * k).sum()))

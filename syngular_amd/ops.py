@@ -18,19 +18,30 @@ F64 = torch.float64
 EPS = float(np.finfo(np.float64).eps)
 
 
+# set once a device has been found and the library loaded: the checks then
+# cost nothing on the per-launch path (a served layer makes thousands of
+# launches from host threads that share one interpreter lock)
+_gpu_ready = [False]
+
+
 def require_gpu():
+    if _gpu_ready[0]:
+        return
     if not torch.cuda.is_available():
         raise _lib.SyngularError("no HIP device visible: syngular_amd runs on an MI355X (gfx950) only")
     _lib.load()
+    _gpu_ready[0] = True
 
 
 def device():
     require_gpu()
-    return torch.device("cuda", torch.cuda.current_device())
+    return torch.device("cuda", torch._C._cuda_getDevice())
 
 
 def _stream():
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    """The calling thread's current HIP stream of the current device, as the
+    raw handle the library's entry points take."""
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
 
 
 class StagingPool:
