@@ -29,6 +29,8 @@
 #include "prof.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace syn {
@@ -998,6 +1000,60 @@ static int64_t tj_work(int64_t nv) {
 constexpr int64_t kWideMinOrder = 192;
 constexpr int64_t kWideMinEntries = int64_t(1) << 23;
 
+// Pinned host staging of the host loop's small transfers.  Copies from and
+// to pageable memory go through the runtime's own staging and wait for the
+// stream, so each readback of a sweep's flags would cost a round trip of its
+// own; from pinned memory they are plain DMA on the stream, and a sweep
+// waits once.  Buffers come from a process-wide pool (any host thread may
+// call the library) and go back to it when the lease ends, after its stream
+// has finished every copy that reads or writes them; they are never freed.
+class PinnedLease {
+  public:
+    PinnedLease(size_t bytes, hipStream_t st) : st_(st) {
+        {
+            std::lock_guard<std::mutex> g(mu());
+            auto &pool = free_list();
+            size_t best = pool.size();
+            for (size_t i = 0; i < pool.size(); ++i)
+                if (pool[i].second >= bytes && (best == pool.size() || pool[i].second < pool[best].second)) best = i;
+            if (best != pool.size()) {
+                buf_ = pool[best].first;
+                cap_ = pool[best].second;
+                pool.erase(pool.begin() + (ptrdiff_t)best);
+                return;
+            }
+        }
+        cap_ = std::max(bytes, (size_t)65536);
+        void *p = nullptr;
+        rc_ = hip_check(hipHostMalloc(&p, cap_, hipHostMallocDefault), "pinned staging");
+        buf_ = rc_ ? nullptr : (char *)p;
+    }
+    ~PinnedLease() {
+        if (!buf_) return;
+        (void)hipStreamSynchronize(st_);
+        std::lock_guard<std::mutex> g(mu());
+        free_list().emplace_back(buf_, cap_);
+    }
+    PinnedLease(const PinnedLease &) = delete;
+    PinnedLease &operator=(const PinnedLease &) = delete;
+    char *data() const { return buf_; }
+    int rc() const { return rc_; }
+
+  private:
+    static std::mutex &mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<std::pair<char *, size_t>> &free_list() {
+        static std::vector<std::pair<char *, size_t>> v;
+        return v;
+    }
+    hipStream_t st_;
+    char *buf_ = nullptr;
+    size_t cap_ = 0;
+    int rc_ = 0;
+};
+
 template <int TB>
 static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
                         int32_t *counters, double tol, int max_sweeps, int *sweeps, hipStream_t st) {
@@ -1011,13 +1067,25 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
         all[i].active = all[i].nv > 1;
         all[i].solve_start = all[i].tile_start = 0;
     }
-    // uploads go through the stream and are waited for, so the host vectors
-    // may change afterwards and earlier kernels reading the table are done
-    auto upload = [&](const void *src, void *dst, size_t bytes, const char *what) -> int {
-        if (int rc = hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what)) return rc;
-        return hip_check(hipStreamSynchronize(st), what);
+    // pinned regions, one per kind of transfer: the item table (first and
+    // last upload), the active table, the offsets, the Gram parities, the
+    // per-sweep readback (counters, then the sweep test flags) and the byte
+    // counters.  An upload is copied into its region and sent on the stream
+    // without a wait (stream order puts it after the kernels that read the
+    // old table); a region is written again only after a wait on the stream
+    // that covers its previous copy (the active table's after the sweep's
+    // readback wait), and the lease waits for the stream before it ends.
+    const size_t n = (size_t)n_items, tbl = (n * sizeof(syn_gbj_item) + 63) / 64 * 64;
+    const size_t r_all0 = 0, r_all1 = tbl, r_act = 2 * tbl, r_off = 3 * tbl, r_par = r_off + (16 * n + 63) / 64 * 64,
+                 r_back = r_par + (4 * n + 63) / 64 * 64, r_bytes = r_back + (8 * n + 63) / 64 * 64,
+                 r_end = r_bytes + 8 * n;
+    PinnedLease pin(r_end, st);
+    if (pin.rc()) return pin.rc();
+    auto upload = [&](size_t region, const void *src, void *dst, size_t bytes, const char *what) -> int {
+        memcpy(pin.data() + region, src, bytes);
+        return hip_check(hipMemcpyAsync(dst, pin.data() + region, bytes, hipMemcpyHostToDevice, st), what);
     };
-    if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
+    if (int rc = upload(r_all0, all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
     hipLaunchKernelGGL(tj_init_kernel<TB>, dim3(n_items), dim3(256), 0, st, items);
     SYN_LAUNCH_CHECK("gram init");
     // byte counter of the update launches, after the sort scratch
@@ -1049,9 +1117,10 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
             acc_nv += all[i].nv;
             pt[n_items + i] = woff[i] + tj_floor_off<TB>(tj_dims<TB>(all[i].nv).half) + 1;
         }
-        if (int rc = upload(pt.data(), dpoff, pt.size() * sizeof(int64_t), "gram offsets")) return rc;
+        if (int rc = upload(r_off, pt.data(), dpoff, pt.size() * sizeof(int64_t), "gram offsets")) return rc;
     }
-    std::vector<int32_t> hc(n_items), hk(n_items);
+    // the sweep's readback: rotation counters, then the sweep test flags
+    const int32_t *hc = (const int32_t *)(pin.data() + r_back), *hk = hc + n_items;
     // buffer holding each problem's Gram matrix (0: P.g, 1: the workspace
     // copy); all problems of a sweep share it
     std::vector<int32_t> gpar(n_items, 0);
@@ -1081,7 +1150,8 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
                 if (D.np > max_np) max_np = D.np;
                 act.push_back(it);
             }
-            if (int rc = upload(act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table")) return rc;
+            if (int rc = upload(r_act, act.data(), items, act.size() * sizeof(syn_gbj_item), "gram active table"))
+                return rc;
             n_act = (int)act.size();
             hipLaunchKernelGGL(tj_owner_kernel<TB>, dim3((unsigned)n_act), dim3(256), 0, st, items, ss, owner);
             SYN_LAUNCH_CHECK("gram owner table");
@@ -1122,11 +1192,10 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
         hipLaunchKernelGGL(tj_check_kernel<TB>, dim3((unsigned)n_act, 64), dim3(256), 0, st, items,
                            cur, tol, counters, chk);
         SYN_LAUNCH_CHECK("gram sweep test");
-        if (int rc = hip_check(hipMemcpyAsync(hc.data(), counters, n_items * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                              st),
+        if (int rc = hip_check(hipMemcpyAsync((void *)hc, counters, n * sizeof(int32_t), hipMemcpyDeviceToHost, st),
                                "gram counters"))
             return rc;
-        if (int rc = hip_check(hipMemcpyAsync(hk.data(), chk, n_items * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+        if (int rc = hip_check(hipMemcpyAsync((void *)hk, chk, n * sizeof(int32_t), hipMemcpyDeviceToHost, st),
                                "gram sweep test flags"))
             return rc;
         if (int rc = hip_check(hipStreamSynchronize(st), "gram sync")) return rc;
@@ -1143,10 +1212,10 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
 
     // eigenvalues, their decreasing order (ties by index) on the device, the
     // permutation of rt rows and s
-    if (int rc = upload(all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
+    if (int rc = upload(r_all1, all.data(), items, all.size() * sizeof(syn_gbj_item), "gram table")) return rc;
     // the Gram matrix is diagonalised in place: bring back the ones the last
     // fused step left in the workspace copy
-    if (int rc = upload(gpar.data(), counters, n_items * sizeof(int32_t), "gram parity")) return rc;
+    if (int rc = upload(r_par, gpar.data(), counters, n * sizeof(int32_t), "gram parity")) return rc;
     hipLaunchKernelGGL(tjf_copyback_kernel<TB>, dim3(n_items, 64), dim3(256), 0, st, items, counters);
     SYN_LAUNCH_CHECK("gram copy back");
     hipLaunchKernelGGL(tj_diag_kernel, dim3(n_items), dim3(256), 0, st, items);
@@ -1158,14 +1227,14 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
     hipLaunchKernelGGL(tj_permute_kernel, dim3(n_items, 64), dim3(256), 0, st, items, dperm, dpoff, work, dtoff, 1);
     SYN_LAUNCH_CHECK("gram permute");
     if (kbytes) {
-        std::vector<unsigned long long> hb(n_items);
-        if (int rc = hip_check(hipMemcpyAsync(hb.data(), dbytes, n_items * sizeof(unsigned long long),
+        const unsigned long long *hb = (const unsigned long long *)(pin.data() + r_bytes);
+        if (int rc = hip_check(hipMemcpyAsync((void *)hb, dbytes, n * sizeof(unsigned long long),
                                               hipMemcpyDeviceToHost, st),
                                "gram bytes"))
             return rc;
         if (int rc = hip_check(hipStreamSynchronize(st), "gram bytes sync")) return rc;
         double tot_b = 0.0;
-        for (unsigned long long b : hb) tot_b += (double)b;
+        for (int i = 0; i < n_items; ++i) tot_b += (double)hb[i];
         prof_add_work(kTagGramBytes, tot_b);
     }
     return hip_check(hipStreamSynchronize(st), "gram final sync");
