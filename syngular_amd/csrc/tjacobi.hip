@@ -147,19 +147,27 @@ struct TjDims {
     int64_t nblk, np, half, ncs;
 };
 
+// rt slab width: the fused 16-row steps give each wave two groups of 16
+// columns (fewer, fuller work-groups, so a launch of a few large problems
+// fits one round of resident work-groups); the 32-row updates one
+template <int TB>
+__host__ __device__ constexpr int tj_slab_cols() {
+    return TB == 16 ? 2 * kTCol : kTCol;
+}
+
 template <int TB>
 __host__ __device__ __forceinline__ TjDims tj_dims(int64_t nv) {
     TjDims d;
     d.nblk = (nv + TB - 1) / TB;
     d.np = d.nblk + (d.nblk & 1);
     d.half = d.np / 2;
-    d.ncs = (nv + kTCol - 1) / kTCol;
+    d.ncs = (nv + tj_slab_cols<TB>() - 1) / tj_slab_cols<TB>();
     return d;
 }
 
 // Update work of a problem at a step: the upper-triangle G tiles (a, b),
 // a <= b, two per work-group (each by two waves, one per 16-column half),
-// then one work-group per rt slab of kTCol columns (one wave per 16 columns).
+// then one work-group per rt slab of tj_slab_cols columns.
 __host__ __device__ __forceinline__ int64_t tj_gtiles(int64_t half) { return half * (half + 1) / 2; }
 // update work-groups of a problem at a step: 16-row blocks put two 32 x 32
 // G tiles in a work-group, 32-row blocks one 64 x 64 tile; one per rt slab
@@ -577,7 +585,8 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
     // ---- updates of step - 1: buffer cur -> buffer cur ^ 1, in registers ----
     // (no barrier from here on: waves return independently)
     constexpr unsigned long long kTileBytes = (unsigned long long)TP * TP * sizeof(double);
-    constexpr unsigned long long kSlabBytes = 2ull * TP * kTCol * sizeof(double);
+    constexpr int kCols = tj_slab_cols<TB>();
+    constexpr unsigned long long kSlabBytes = 2ull * TP * kCols * sizeof(double);
     const int64_t ub = tj_xcd_order(bid - solve_pad, tile_pad);
     if (ub >= tile_count) return;
     const syn_gbj_item &P = items[owner_t[ub]];
@@ -677,33 +686,47 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
         }
         return;
     }
-    // rt slab (a, columns [c0, c0 + kTCol)), in place: wave wv takes columns
-    // c0 + 16 wv .. c0 + 16 wv + 15 of every row of pair a
+    // rt slab (a, columns [c0, c0 + kCols)), in place: wave wv takes columns
+    // c0 + 64 h + 16 wv .. c0 + 64 h + 16 wv + 15 (h = 0, 1) of every row of
+    // pair a; the W operand of a k-step serves both column groups
     const int64_t u = t - gw;
-    const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kTCol;
+    const int64_t a = u / D.ncs, c0 = (u % D.ncs) * kCols;
     if (!flags[a]) return;
     if (bytes && tid == 0) atomicAdd(bytes + owner_t[ub], kSlabBytes);
     const double *wa = wset + a * TP * TP;
-    const int64_t col = c0 + TB * wv + li;
-    const bool cok = col < P.nv;
-    double xs[8];
+    int64_t col[2];
+    bool cok[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        col[h] = c0 + kTCol * h + TB * wv + li;
+        cok[h] = col[h] < P.nv;
+    }
+    double xs[2][8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int r = tj_row<TB>(st, a, D.np, D.nblk, P.nv, 4 * j + lk);
-        const bool ok = r >= 0 && cok;
-        const double val = P.rt[ok ? (int64_t)r * P.ldr + col : 0];
-        xs[j] = ok ? val : 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bool ok = r >= 0 && cok[h];
+            const double val = P.rt[ok ? (int64_t)r * P.ldr + col[h] : 0];
+            xs[h][j] = ok ? val : 0.0;
+        }
     }
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti) {
-        v4d acc = {0.0, 0.0, 0.0, 0.0};
+        v4d acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[(4 * j + lk) * TP + TB * ti + li], xs[j], acc, 0, 0, 0);
+        for (int j = 0; j < 8; ++j) {
+            const double w = wa[(4 * j + lk) * TP + TB * ti + li];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(w, xs[h][j], acc[h], 0, 0, 0);
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int gr = tj_row<TB>(st, a, D.np, D.nblk, P.nv, TB * ti + lk + 4 * r);
-            if (gr >= 0 && cok) P.rt[(int64_t)gr * P.ldr + col] = acc[r];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (gr >= 0 && cok[h]) P.rt[(int64_t)gr * P.ldr + col[h]] = acc[h][r];
         }
     }
 }
@@ -997,7 +1020,7 @@ static int64_t tj_work(int64_t nv) {
 // the wide (32-row) blocks take the problems of order kWideMinOrder and
 // above when together they hold at least kWideMinEntries Gram entries (below
 // that a call is latency-bound and the fused 16-row steps finish first)
-constexpr int64_t kWideMinOrder = 192;
+constexpr int64_t kWideMinOrder = 384;
 constexpr int64_t kWideMinEntries = int64_t(1) << 23;
 
 // Pinned host staging of the host loop's small transfers.  Copies from and

@@ -501,7 +501,8 @@ def _gram_workspace(nv):
             # two W-tile sets, rotation flags, floor, sort scratch, second Gram buffer
             work.append(2 * half * 4 * tb * tb + half + 1 + v * v + v + 4 + v * v)
             gtiles = half * (half + 1) // 2
-            ugroups = ((gtiles + 1) // 2 if tb == 16 else gtiles) + half * ((v + 63) // 64)
+            cols = 128 if tb == 16 else 64        # rt slab width of the block size
+            ugroups = ((gtiles + 1) // 2 if tb == 16 else gtiles) + half * ((v + cols - 1) // cols)
             owners.append(half + ugroups)
         tot += max(work) + 2 * v + (max(owners) + 1) // 2
     return tot + 4 * len(nv) + 16

@@ -120,7 +120,7 @@ def test_householder_qr_matches_lapack(m, n, keep):
 
 
 def test_jacobi_gram_blocked_wide_batch():
-    """A call with hundreds of Gram matrices: those of order 192 and above
+    """A call with hundreds of Gram matrices: those of order 384 and above
     hold more than 2^23 entries together and take the 32-row blocks, the
     rest the fused 16-row steps after them.  Singular values against numpy,
     orthonormal eigenvector rows and the reconstruction of G, over orders
@@ -129,8 +129,8 @@ def test_jacobi_gram_blocked_wide_batch():
 
     rng = np.random.default_rng(21)
     orders = [int(v) for v in rng.integers(17, 520, size=300)]
-    orders[:8] = [17, 32, 33, 64, 191, 192, 193, 519]
-    assert sum(g * g for g in orders if g >= 192) >= 1 << 23
+    orders[:8] = [17, 32, 33, 64, 383, 384, 385, 519]
+    assert sum(g * g for g in orders if g >= 384) >= 1 << 23
     case(f"jacobi gram blocked wide, {len(orders)} problems, orders {min(orders)}..{max(orders)}")
     Cs = [rng.standard_normal((g + 3, g)) for g in orders]
     Cs[5] = Cs[5] * np.logspace(0, -3, orders[5])
