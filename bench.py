@@ -76,6 +76,10 @@ def parse():
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="fp64",
                     help="arithmetic of the served form (the per-chain comparison is always plain FP64); "
                          "FP32 products add ~1e-7 per layer, which long circuits accumulate past 1e-6")
+    ap.add_argument("--serve-jacobi-tol", type=float, default=1e-8,
+                    help="rotation threshold of the served form's blocked Jacobi (the error budget's, 1e-8 by "
+                         "default; 0: the precision's own, 2048 eps in FP64); the per-chain comparison keeps "
+                         "the FP64 one")
     ap.add_argument("--serve-plain-seconds", type=float, default=150.0,
                     help="time box of the per-chain comparison of the served form (its complete circuits "
                          "normally end well inside it)")
@@ -165,7 +169,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 contractions + f64 Gram/Cholesky/Jacobi/truncation (mixed)",
+        "dtype": "f32 contractions + f64 Gram/Cholesky/Jacobi/truncation, Jacobi stopped at 1e-8 (mixed)",
         "data": "synthetic (seeded Gaussian cores drawn on the device, Haar-random orthogonal 2-site gates)",
         "config": head.pop("config"),
     }
@@ -730,7 +734,8 @@ def serve_form(np, torch, args):
 
     jobs = W.serving_jobs(args.serve_jobs, args.serve_rate, seed=4)
     prec = args.serve_precision
-    srv = serving.Server(serving.DeviceExecutor.lanes(precision=prec), hbm_budget=200 * 2 ** 30,
+    jtol = args.serve_jacobi_tol if args.serve_jacobi_tol > 0 else None
+    srv = serving.Server(serving.DeviceExecutor.lanes(precision=prec, jacobi_tol=jtol), hbm_budget=200 * 2 ** 30,
                          keep_states=False, lanes=serving.chi_lanes)
     # warm-up on a tiny stream (kernel loads, allocator)
     serving.Server(serving.DeviceExecutor(), hbm_budget=2 ** 34).run(
@@ -774,7 +779,8 @@ def serve_form(np, torch, args):
     from oracle import rounding as ornd
     log(f"serve: per-chain form {plain:.1f} site-layers/s; oracle check of the two cheapest jobs")
     cheap = sorted(jobs, key=lambda j: j["n"] * j["layers"] * j["chi_max"])[:2]
-    srv2 = serving.Server(serving.DeviceExecutor(precision=prec), hbm_budget=2 ** 36, keep_states=True)
+    srv2 = serving.Server(serving.DeviceExecutor(precision=prec, jacobi_tol=jtol), hbm_budget=2 ** 36,
+                          keep_states=True)
     got = {r.id: r for r in srv2.run([dict(j, arrival=0.0) for j in cheap]).results}
     errs = []
     from oracle import reference as oref
@@ -813,8 +819,10 @@ def serve_form(np, torch, args):
         log(f"serve cpu baseline: {cpu}")
     return {"workload": f"Poisson stream of {len(jobs)} jobs at {args.serve_rate}/s: N~U[16,128], layers~U[4,32] "
                         f"brickwork Haar O(4) gates (MPO bond 4), chi_max~U{{64,128,256,512}}, cutoff 1e-10, "
-                        f"from |0...0>; {prec} (served, grouped launches)",
+                        f"from |0...0>; {prec}" + (f", blocked Jacobi stopped at {jtol:g}" if jtol else "")
+                        + " (served, grouped launches)",
             "precision": prec,
+            "jacobi_tol": jtol,
             "site_layers": rep.site_layers, "makespan_s": round(rep.makespan, 2),
             "sites_per_s": round(rep.sites_per_s, 1), "p50_latency_s": round(lat["p50"], 2),
             "p99_latency_s": round(lat["p99"], 2), "layer_steps": rep.steps, "peak_resident": rep.peak_resident,

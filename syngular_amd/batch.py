@@ -48,6 +48,12 @@ CHOLQR_MIN_COLS = 96            # tall unfoldings with this many columns (or too
 # LDS kernel takes up to 64).
 TOL_LDS = ops.jacobi_tol(64)
 TOL_BLOCKED = ops.jacobi_tol(2048)
+# Stopping threshold of the blocked Jacobi for the forms that work to an
+# error budget of 1e-6 (the mixed form, and served jobs that ask for it):
+# rotations stop once every |G_pq| <= 1e-8 sqrt(G_pp G_qq).  Eigenvalues then
+# carry relative errors of order 1e-16 and the retained subspace moves by
+# angles of order 1e-8 / (relative gap); the FP64 form keeps TOL_BLOCKED.
+TOL_BUDGET = 1e-8
 
 
 # --------------------------------------------------------------------------
@@ -394,9 +400,10 @@ def orth_estimate_table(Q, m, n, hold):
                      for o, ni in zip(off.astype(np.int64), n)])
 
 
-def jacobi_gram_blocked_table(x, g, rt, s):
-    """Two-sided block Jacobi of many symmetric g x g Gram matrices (g > 64),
-    one batched call; x is diagonalised in place."""
+def jacobi_gram_blocked_table(x, g, rt, s, tol=None):
+    """Two-sided block Jacobi of many symmetric g x g Gram matrices (g > 16),
+    one batched call; x is diagonalised in place.  tol: rotation threshold
+    (default TOL_BLOCKED)."""
     cnt = len(x)
     if cnt == 0:
         return 0
@@ -410,7 +417,8 @@ def jacobi_gram_blocked_table(x, g, rt, s):
     dev_items = _empty(it.nbytes, torch.uint8)
     sweeps = ctypes.c_int(0)
     call("syn_jacobi_gram_blocked_f64", dev_items.data_ptr(), it.ctypes.data_as(ctypes.c_void_p), cnt,
-         work.data_ptr(), counters.data_ptr(), TOL_BLOCKED, ops.JACOBI_SWEEPS, ctypes.byref(sweeps), ops._stream())
+         work.data_ptr(), counters.data_ptr(), TOL_BLOCKED if tol is None else float(tol), ops.JACOBI_SWEEPS,
+         ctypes.byref(sweeps), ops._stream())
     if JACOBI_LOG is not None:
         JACOBI_LOG.append((g.copy(), sweeps.value))
     return sweeps.value
@@ -703,11 +711,12 @@ class ChainBatch:
         return out
 
     # ---- right sweep -------------------------------------------------------
-    def right_svd_sweep(self, chi_max, cutoff=0.0, records=None, mixed=False):
+    def right_svd_sweep(self, chi_max, cutoff=0.0, records=None, mixed=False, jacobi_tol=None):
         """Right-to-left SVD truncation of left-orthonormal chains.
         chi_max, cutoff: scalars or per-chain arrays (chi_max 0 = no cap).
         mixed: the products with the singular vectors on the FP32 matrix
-        cores (Gram matrices, Jacobi and truncation stay FP64)."""
+        cores (Gram matrices, Jacobi and truncation stay FP64).  jacobi_tol:
+        the blocked Jacobi's rotation threshold (default TOL_BLOCKED)."""
         J, nmax = self.ptr.shape
         chi_max = np.broadcast_to(np.asarray(0 if chi_max is None else chi_max, dtype=np.int64), (J,))
         cutoff = np.broadcast_to(np.asarray(cutoff, dtype=np.float64), (J,))
@@ -733,7 +742,7 @@ class ChainBatch:
             if small.any():
                 jacobi_table(gp[small], g[small], rtp[small], sp[small])
             if (~small).any():
-                jacobi_gram_blocked_table(gp[~small], g[~small], rtp[~small], sp[~small])
+                jacobi_gram_blocked_table(gp[~small], g[~small], rtp[~small], sp[~small], jacobi_tol)
             keep = _empty(len(a), torch.int32, zero=True)
             disc = _empty(len(a), F64, zero=True)
             kb, db = np.uint64(keep.data_ptr()), np.uint64(disc.data_ptr())
@@ -784,15 +793,18 @@ class ChainBatch:
                 records.append(BondRecord(a, k, sp, g, cc, disc, (stt,)))
         return ChainBatch(self.n, bond, self.phys, ptr, hold)
 
-    def tt_round(self, chi_max, cutoff=0.0, records=None, compact=True, precision="fp64"):
+    def tt_round(self, chi_max, cutoff=0.0, records=None, compact=True, precision="fp64", jacobi_tol=None):
         """TT-rounding of every chain; precision "fp64" or "mixed" (products
         on the FP32 matrix cores, Gram matrices, Cholesky, Jacobi and
-        truncation in FP64)."""
+        truncation in FP64, the blocked Jacobi stopped at TOL_BUDGET).
+        jacobi_tol overrides the blocked Jacobi's threshold of either."""
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
         mixed = precision == "mixed"
+        if jacobi_tol is None:
+            jacobi_tol = TOL_BUDGET if mixed else TOL_BLOCKED
         out = self.left_orthonormalize(mixed, None if mixed else REORTH_BUDGET).right_svd_sweep(
-            chi_max, cutoff, records, mixed)
+            chi_max, cutoff, records, mixed, jacobi_tol)
         if compact:
             out.compact()
         return out
@@ -840,10 +852,13 @@ class CapturedApplyLeft:
         self.graph.replay()
         return self.out
 
-    def round(self, chi_max, cutoff=0.0, records=None, compact=True):
+    def round(self, chi_max, cutoff=0.0, records=None, compact=True, jacobi_tol=None):
         """One layer: replay, then the eager right sweep (TT-rounding of the
-        captured batch to chi_max with the cutoff)."""
-        out = self().right_svd_sweep(chi_max, cutoff, records, self.mixed)
+        captured batch to chi_max with the cutoff; the blocked Jacobi's
+        threshold as in ChainBatch.tt_round)."""
+        if jacobi_tol is None:
+            jacobi_tol = TOL_BUDGET if self.mixed else TOL_BLOCKED
+        out = self().right_svd_sweep(chi_max, cutoff, records, self.mixed, jacobi_tol)
         if compact:
             out.compact()
         return out
@@ -1025,6 +1040,7 @@ def overlap(A: ChainBatch, B: ChainBatch):
     return out
 
 
-def apply_and_round(mps: ChainBatch, mpo: OperatorBatch, chi_max, cutoff=0.0, records=None, precision="fp64"):
+def apply_and_round(mps: ChainBatch, mpo: OperatorBatch, chi_max, cutoff=0.0, records=None, precision="fp64",
+                    jacobi_tol=None):
     """One layer for every chain: materialised apply, then TT-rounding."""
-    return mps.apply(mpo).tt_round(chi_max, cutoff, records, precision=precision)
+    return mps.apply(mpo).tt_round(chi_max, cutoff, records, precision=precision, jacobi_tol=jacobi_tol)

@@ -231,15 +231,18 @@ def chi_lanes(job):
 class DeviceExecutor:
     """Runs the layers with the grouped gfx950 kernels (syngular_amd.batch).
     With a stream, every launch of this executor goes to that HIP stream (one
-    per lane, so lanes overlap on the device)."""
+    per lane, so lanes overlap on the device).  jacobi_tol: the blocked
+    Jacobi's rotation threshold (None: the precision's own, see
+    batch.ChainBatch.tt_round)."""
 
-    def __init__(self, d=2, stream=None, precision="fp64"):
+    def __init__(self, d=2, stream=None, precision="fp64", jacobi_tol=None):
         from . import batch, ops
         self.B = batch
         self.ops = ops
         self.d = d
         self.stream = stream
         self.precision = precision
+        self.jacobi_tol = jacobi_tol
 
     def _ctx(self):
         import contextlib
@@ -247,10 +250,10 @@ class DeviceExecutor:
         return torch.cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
 
     @classmethod
-    def lanes(cls, d=2, precision="fp64"):
+    def lanes(cls, d=2, precision="fp64", jacobi_tol=None):
         """Factory for Server(lanes=...): one executor and one stream per lane."""
         import torch
-        return lambda key: cls(d, torch.cuda.Stream(), precision)
+        return lambda key: cls(d, torch.cuda.Stream(), precision, jacobi_tol)
 
     def empty(self):
         return self.B.concat([])
@@ -278,7 +281,8 @@ class DeviceExecutor:
         recs = []
         chi = np.array([j["chi_max"] for j in jobs], dtype=np.int64)
         cut = np.array([j["cutoff"] for j in jobs], dtype=np.float64)
-        new = self.B.apply_and_round(state, mpo, chi, cut, records=recs, precision=self.precision)
+        new = self.B.apply_and_round(state, mpo, chi, cut, records=recs, precision=self.precision,
+                                     jacobi_tol=self.jacobi_tol)
         disc = np.zeros(len(jobs))
         if recs:
             import torch

@@ -265,3 +265,24 @@ def test_autotune_table_save_load(tmp_path):
         autotune.TABLE.clear()
         autotune.TIMINGS.clear()
         autotune.COUNTS.clear()
+
+
+def test_jacobi_tolerance_reaches_the_blocked_jacobi(emu, monkeypatch):
+    """The served executor's jacobi_tol reaches every blocked Jacobi call;
+    without it FP64 rounding keeps TOL_BLOCKED and the mixed form takes the
+    error budget's TOL_BUDGET."""
+    seen = []
+    orig = B.jacobi_gram_blocked_table
+
+    def record(x, g, rt, s, tol=None):
+        seen.append(tol)
+        return orig(x, g, rt, s, tol)
+
+    monkeypatch.setattr(B, "jacobi_gram_blocked_table", record)
+    jobs = W.serving_jobs(2, 1e6, 3, n_range=(12, 12), layer_range=(8, 8), chis=(32,), cutoff=1e-10)
+    serving.Server(serving.DeviceExecutor(jacobi_tol=1e-8), hbm_budget=10 ** 11).run(jobs)
+    assert seen and all(t == 1e-8 for t in seen)
+    for prec, want in (("fp64", B.TOL_BLOCKED), ("mixed", B.TOL_BUDGET)):
+        seen.clear()
+        serving.Server(serving.DeviceExecutor(precision=prec), hbm_budget=10 ** 11).run(jobs)
+        assert seen and all(t == want for t in seen), prec

@@ -501,6 +501,39 @@ def test_jacobi_gram_blocked(g, rank):
     np.testing.assert_allclose(R.T @ np.diag(s ** 2) @ R, G0, atol=1e-11 * ref[0] ** 2)
 
 
+def test_jacobi_gram_blocked_error_budget_tolerance():
+    """The blocked Jacobi stopped at the error budget's threshold (every
+    |G_pq| <= 1e-8 sqrt(G_pp G_qq)): no more sweeps than at the FP64 one,
+    singular values within 2e-8 (a near-degenerate pair may move by tol
+    lambda), eigenvector rows orthonormal to rounding (a product of
+    rotations) and G reconstructed within the residual bound g tol s_max^2."""
+    from syngular_amd import batch as B
+
+    rng = np.random.default_rng(23)
+    orders = [40, 96, 129, 256, 300]
+    case(f"jacobi gram blocked at tol {B.TOL_BUDGET:g}, orders {orders}")
+    Cs = [rng.standard_normal((g + 3, g)) for g in orders]
+    Gs = [c.T @ c for c in Cs]
+    dev = torch.device("cuda")
+    ptr = lambda ts: np.array([t.data_ptr() for t in ts], dtype=np.uint64)
+    sweeps = {}
+    for tol in (None, B.TOL_BUDGET):
+        G = [torch.tensor(x, device=dev) for x in Gs]
+        R = [torch.empty_like(x) for x in G]
+        S = [torch.empty(x.shape[0], dtype=torch.float64, device=dev) for x in G]
+        sweeps[tol] = B.jacobi_gram_blocked_table(ptr(G), np.array(orders, np.int64), ptr(R), ptr(S), tol)
+        torch.cuda.synchronize()
+    assert 0 < sweeps[B.TOL_BUDGET] <= sweeps[None]
+    for q, g in enumerate(orders):
+        s, r = S[q].cpu().numpy(), R[q].cpu().numpy()
+        ref = np.linalg.svd(Cs[q], compute_uv=False)
+        assert np.all(np.diff(s) <= 0), q
+        np.testing.assert_allclose(s, ref, rtol=2e-8, atol=1e-12 * ref[0], err_msg=str(q))
+        np.testing.assert_allclose(r @ r.T, np.eye(g), atol=1e-12, err_msg=str(q))
+        np.testing.assert_allclose(r.T @ np.diag(s ** 2) @ r, Gs[q], atol=g * B.TOL_BUDGET * ref[0] ** 2,
+                                   err_msg=str(q))
+
+
 def test_jacobi_gram_blocked_batch_mixed_orders():
     """One batched call over Gram matrices of mixed orders (partial last
     blocks, odd block counts, a graded spectrum, a rank-deficient one):
