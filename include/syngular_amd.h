@@ -315,20 +315,24 @@ int syn_jacobi_blocked_many_f64(syn_bjac_item *items, const syn_bjac_item *items
 int64_t syn_jacobi_blocked_many_workspace(const int64_t *nv, const int64_t *len, int n_items);
 
 /* Two-sided block Jacobi eigen-decomposition of many symmetric Gram     */
-/* matrices of uneven size (nv > 64), for the right sweep of the SVD      */
+/* matrices of uneven size (nv > 16), for the right sweep of the SVD      */
 /* rounding.  Per problem: g (nv x nv, ldg) is diagonalised IN PLACE;      */
 /* on exit rt (nv x nv, ldr) holds the eigenvectors as rows sorted by     */
 /* decreasing eigenvalue (G_in = rt^T diag(lambda) rt) and s[i] =          */
-/* sqrt(max(lambda_i, 0)).  Blocks of 16 or 32 rows; with 16 every step  */
-/* is one launch holding its pair solves (LDS, one work-group each) beside */
-/* the previous step's MFMA tile updates, for all problems still rotating */
-/* (32: a launch of each).  A problem stops after a sweep that rotated    */
-/* nothing or when no off-diagonal entry passes the rotation test at the  */
-/* end of a sweep (the next sweep would rotate nothing).  `items` is a DEVICE */
-/* table of n_items entries and `items_host` the same table on the host;  */
-/* w, solve_start, tile_start, counter and active are filled by the       */
-/* library.  work: syn_jacobi_gram_blocked_workspace(nv, n) doubles;      */
-/* counters: n_items ints; *sweeps (host, may be null) gets the sweeps.   */
+/* sqrt(max(lambda_i, 0)).  Blocks of 16 rows: every step is one launch   */
+/* holding its pair solves (LDS, one work-group each) beside the previous */
+/* step's MFMA tile updates, for all problems still rotating.  The        */
+/* problems of order 384 and more take blocks of 32 rows (a solve launch  */
+/* and an update launch per step, half the steps) when together they hold */
+/* at least 2^23 Gram entries; they run first, the rest after them.  A    */
+/* problem stops after a sweep that rotated nothing or when no            */
+/* off-diagonal entry passes the rotation test |G_pq| > tol sqrt(G_pp     */
+/* G_qq) at the end of a sweep (the next sweep would rotate nothing).     */
+/* `items` is a DEVICE table of n_items entries and `items_host` the same */
+/* table on the host; w, solve_start, tile_start, counter and active are  */
+/* filled by the library.  work: syn_jacobi_gram_blocked_workspace(nv, n) */
+/* doubles (enough for either block size); counters: n_items ints;        */
+/* *sweeps (host, may be null) gets the most sweeps any problem took.     */
 /* No reference call site (an addition: the SVD form of the rounding).  */
 typedef struct {
     double *g;
