@@ -178,7 +178,8 @@ def main():
     if served:
         line["served_cfg4"] = {k: served[k] for k in ("sites_per_s", "plain_fp64_per_chain_sites_per_s",
                                                      "speedup_vs_plain", "p50_latency_s", "p99_latency_s",
-                                                     "precision", "norm2_rel_err_max", "overlap_rel_err_max")}
+                                                     "precision", "jacobi_tol", "norm2_rel_err_max",
+                                                     "overlap_rel_err_max")}
     line["forms"] = forms or None
     print(json.dumps(line), flush=True)
 
