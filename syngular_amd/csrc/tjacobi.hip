@@ -478,7 +478,7 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
                                                           const int32_t *__restrict__ owner_t, int64_t step,
                                                           int has_update, int64_t solve_pad, int64_t solve_count,
                                                           int64_t tile_pad, int64_t tile_count, int cur, double tol,
-                                                          unsigned long long *bytes) {
+                                                          int passes, unsigned long long *bytes) {
     constexpr int TP = 2 * TB, NT = TB * TB;
     static_assert(TB == 16 && NT == kTUpdThreads, "fused steps: 16-row blocks, one block size for both roles");
     // LDS: the solve role's tile S and rotation W (17 KiB); the update role
@@ -577,8 +577,8 @@ __global__ __launch_bounds__(TB * TB, 5) void tjf_step_kernel(const syn_gbj_item
         // the first step of a pass meets every block and treats the planes
         // within blocks; the others only the planes between their two blocks,
         // so a pass treats every plane of G once
-        tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, 1, tj_wset<TB>(P, D.half, ps) + pw * TP * TP,
-                          tj_flags<TB>(P, D.half, ps), st != 0);
+        tj_pair_solve<TB>(P, D, pw, S, W, cs, do_rot, any_rot, tol, passes,
+                          tj_wset<TB>(P, D.half, ps) + pw * TP * TP, tj_flags<TB>(P, D.half, ps), st != 0);
         return;
     }
     if (!has_update) return;
@@ -1017,11 +1017,15 @@ static int64_t tj_work(int64_t nv) {
     return tj_g2_off<TB>(half, nv) + nv * nv;
 }
 
+// Gram entries from which a call counts as many problems: its launches are
+// bound by the tile updates' bytes rather than by the pair solves' latency.
+constexpr int64_t kManyEntries = int64_t(1) << 23;
+
 // the wide (32-row) blocks take the problems of order kWideMinOrder and
 // above when together they hold at least kWideMinEntries Gram entries (below
 // that a call is latency-bound and the fused 16-row steps finish first)
 constexpr int64_t kWideMinOrder = 384;
-constexpr int64_t kWideMinEntries = int64_t(1) << 23;
+constexpr int64_t kWideMinEntries = kManyEntries;
 
 // Pinned host staging of the host loop's small transfers.  Copies from and
 // to pageable memory go through the runtime's own staging and wait for the
@@ -1080,6 +1084,14 @@ class PinnedLease {
 template <int TB>
 static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int n_items, double *work,
                         int32_t *counters, double tol, int max_sweeps, int *sweeps, hipStream_t st) {
+    // passes of the pair solves' rounds (while they rotate): a launch of many
+    // problems is bound by its tile updates, beside which a second pass costs
+    // little and saves sweeps (batch of 4096 chains: 7272 -> 5996 launches a
+    // step, 3.5% faster); a launch of a few problems waits on its solves,
+    // where it costs more than it saves (8 order-512 problems: 8.6 -> 10.9 ms)
+    int64_t entries = 0;
+    for (int i = 0; i < n_items; ++i) entries += all[i].nv * all[i].nv;
+    const int passes = entries >= kManyEntries ? 2 : 1;
     std::vector<int64_t> woff(n_items);
     int64_t tot_w = 0;
     for (int i = 0; i < n_items; ++i) {
@@ -1189,7 +1201,8 @@ static int gram_blocked(syn_gbj_item *items, std::vector<syn_gbj_item> &all, int
                 const int64_t sp = hs ? (ss + 7) / 8 * 8 : 0, tp = hu ? (ts + 7) / 8 * 8 : 0;
                 ProfScope ps(kTagGramStep, 0.0, st);
                 hipLaunchKernelGGL(tjf_step_kernel<TB>, dim3((unsigned)(sp + tp)), dim3(256), 0, st, items, owner,
-                                   owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp, ts, cur, tol, kbytes);
+                                   owner + ss, step, hu ? 1 : 0, sp, hs ? ss : (int64_t)0, tp, ts, cur, tol, passes,
+                                   kbytes);
                 if (hu) cur ^= 1;
             }
         } else {
