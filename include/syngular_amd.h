@@ -324,8 +324,10 @@ int64_t syn_jacobi_blocked_many_workspace(const int64_t *nv, const int64_t *len,
 /* step's MFMA tile updates, for all problems still rotating.  The        */
 /* problems of order 384 and more take blocks of 32 rows (a solve launch  */
 /* and an update launch per step, half the steps) when together they hold */
-/* at least 2^23 Gram entries; they run first, the rest after them.  A    */
-/* problem stops after a sweep that rotated nothing or when no            */
+/* at least 2^23 Gram entries; they run first, the rest after them.  The  */
+/* 16-row pair solves make two passes of their rounds (while they rotate) */
+/* when the call's 16-row problems hold at least 2^23 entries, else one.  */
+/* A problem stops after a sweep that rotated nothing or when no          */
 /* off-diagonal entry passes the rotation test |G_pq| > tol sqrt(G_pp     */
 /* G_qq) at the end of a sweep (the next sweep would rotate nothing).     */
 /* `items` is a DEVICE table of n_items entries and `items_host` the same */
